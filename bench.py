@@ -1,0 +1,145 @@
+"""Benchmark: network Fisher matrices per second on one MI355X.
+
+Workload (the metric's configuration): IMRPhenomD, 11 parameters, a
+5-interferometer network -- the ET triangle at the ETS site (ET-0000A-18 ASD,
+null-stream third arm) plus Cosmic Explorer L detectors at the CE1Id and
+CE2NM sites (CE 40 km ASD) -- Earth rotation on, fmin 2 Hz, 2000-point
+geometric grids to the waveform cut, 10^4 synthetic BBH events (seed 0),
+FP64 throughout.
+
+A step is one DetNet.FisherMatr over the whole batch with the parameters
+already resident in HBM: the per-event coefficient kernel plus the fused
+waveform/projection/tangent/Gram kernel.  Also reported: the roofline of the
+dominant (Fisher) kernel, timed with HIP events on the stream it runs on,
+and a CPU baseline (the oracle, a NumPy restatement of the reference's
+algorithm) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_EV = 10000
+RES = 2000
+SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
+GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_net():
+    from gwfast_amd import gwfastGlobals as glob
+    from gwfast_amd.network import DetNet
+    from gwfast_amd.signal import GWSignal
+    from gwfast_amd.waveforms import IMRPhenomD
+    wf = IMRPhenomD()
+    psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
+    sigs = {}
+    for name, p in SITES:
+        d = glob.detectors[name]
+        sigs[name] = GWSignal(wf, psd_path=psd[p], detector_shape=d['shape'], det_lat=d['lat'], det_long=d['long'],
+                              det_xax=d['xax'], useEarthMotion=True, fmin=2., verbose=False)
+    return wf, DetNet(sigs, verbose=False)
+
+
+def cpu_baseline(n_sample=12, res=RES, seed=0):
+    """The oracle (reference-faithful construction: AD-equivalent rows through
+    exp(i Psi), closed-form rows, trapz Gram) on n_sample events of the same
+    workload, single-threaded NumPy."""
+    from gwfast_amd import catalog, gwfastGlobals as glob
+    from oracle import detector as OD, waveforms as OW
+    ev = catalog.draw_bbh(n_sample, seed=seed)
+    psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
+    dets = []
+    for name, p in SITES:
+        d = glob.detectors[name]
+        dets.append(OD.Detector(psd[p], d['shape'], d['lat'], d['long'], d['xax'], useEarthMotion=True, fmin=2.))
+    wf = OW.IMRPhenomD()
+    t = time.perf_counter()
+    OD.network_fisher(dets, wf, ev, res, mode='faithful')
+    dt = time.perf_counter() - t
+    return {'value': n_sample / dt, 'unit': 'Fisher matr/s', 'cores': 1, 'kind': 'port',
+            'sample': '%d events of the same workload (IMRPhenomD, ETS+CE1Id+CE2NM, Earth rotation, %d freqs), '
+                      'NumPy FP64 restatement of gwfast\'s algorithm, %.1f s' % (n_sample, res, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--n-events', type=int, default=N_EV)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-sample', type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path (events shard with no '
+              'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gwfast_amd import _engine as E, _native as N, catalog
+    from gwfast_amd import signal as S
+
+    torch.cuda.set_device(0)
+    wf, net = build_net()
+    n = args.n_events
+    ev = catalog.draw_bbh(n, seed=0)
+    rows = E.to_dev(E.param_rows(ev, False))
+    sigs = list(net.signals.values())
+    F = torch.empty((11, 11, n), dtype=torch.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    dets = S._dets(sigs)
+    wfc = wf.c_struct()
+    st = N.stream_ptr()
+    L = N.lib()
+
+    def step(t_fisher=None):
+        N.check(L.gwf_prepare(wfc, dets, len(sigs), N.ptr(rows), rows.stride(0), n, RES, N.ptr(ws), nb, st), L)
+        if t_fisher is not None:
+            t_fisher[0].record()
+        N.check(L.gwf_fisher_prepared(wfc, dets, len(sigs), n, RES, -1, N.ptr(F), N.ptr(ws), nb, st), L)
+        if t_fisher is not None:
+            t_fisher[1].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    ms_per_step = 1e3 * wall / args.steps
+    value = n * args.steps / wall
+    achieved = GRAM_FLOP_PER_FISHER * n / (kern_ms * 1e-3) / 1e12
+    Fh = F.cpu().numpy()
+    assert np.all(np.isfinite(Fh))
+    out = {
+        'metric': 'Fisher matr/s, IMRPhenomD 3-det 11-par, rel err <=1e-6 vs FP64, vs plain',
+        'value': value, 'unit': 'Fisher matr/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f64', 'data': 'synthetic (seeded BBH catalogue)',
+        'config': {'workload': 'IMRPhenomD network Fisher: ETS triangle + CE1Id + CE2NM (5 IFOs), Earth rotation, '
+                               '11 params, %d events x %d freqs, FP64 fused plain path' % (n, RES),
+                   'n_events': n, 'n_freqs': RES, 'n_params': 11, 'n_ifo': 5, 'parallelism': 'single GPU'},
+        'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': achieved / FP64_PEAK_TFLOPS, 'traffic': None,
+                     'kernel': 'k_fisher<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
+                     'note': 'FP64 dense peak; algorithmic flops = Gram 4 P^2 N_f N_ifo per Fisher'},
+    }
+    if not args.no_cpu_baseline:
+        out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,75 @@
+"""Device-side plumbing between the gwfast-compatible classes and the C ABI.
+
+Owns the device copies of PSD and QNM tables, the per-call workspace and the
+conversion of gwfast ``evParams`` dictionaries to the FP64 SoA parameter rows
+the kernels read (row order = Fisher ParNums order, waveforms.py:78-129).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+PARAM_ROWS = ['Mc', 'eta', 'dL', 'theta', 'phi', 'iota', 'psi', 'tcoal', 'Phicoal', 'chi1z', 'chi2z',
+              'LambdaTilde', 'deltaLambda']
+
+
+def torch():
+    import torch as _t
+    return _t
+
+
+def device():
+    return torch().device('cuda', torch().cuda.current_device())
+
+
+def to_dev(a, dtype=None):
+    t = torch()
+    return t.as_tensor(np.ascontiguousarray(a), dtype=dtype or t.float64, device=device())
+
+
+class Workspace:
+    """Grow-only device scratch for the per-event records."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, n_ev):
+        L = N.lib()
+        need = int(L.gwf_workspace_bytes(int(n_ev)))
+        if self.buf is None or self.buf.numel() < need:
+            self.buf = torch().empty(max(need, 8), dtype=torch().uint8, device=device())
+        return self.buf, need
+
+
+_WS = Workspace()
+
+
+def workspace(n_ev):
+    return _WS.get(n_ev)
+
+
+def param_rows(evParams, is_tidal):
+    """evParams dict -> (P_rows, N) float64 with LambdaTilde/deltaLambda for
+    tidal models (signal.py:858-866 computes them from Lambda1/Lambda2)."""
+    from .gwfastUtils import Lamt_delLam_from_Lam12
+    n = len(np.atleast_1d(evParams['Mc']))
+    rows = [np.broadcast_to(np.asarray(evParams[k], dtype=np.float64), (n,)) for k in PARAM_ROWS[:11]]
+    if is_tidal:
+        if 'LambdaTilde' in evParams and 'deltaLambda' in evParams and 'Lambda1' not in evParams:
+            Lt, dL = evParams['LambdaTilde'], evParams['deltaLambda']
+        else:
+            try:
+                Lt, dL = Lamt_delLam_from_Lam12(np.asarray(evParams['Lambda1'], dtype=np.float64),
+                                                np.asarray(evParams['Lambda2'], dtype=np.float64),
+                                                np.asarray(evParams['eta'], dtype=np.float64))
+            except KeyError:
+                raise ValueError('Two among Lambda1, Lambda2 and LambdaTilde and deltaLambda have to be provided.')
+        rows += [np.broadcast_to(np.asarray(Lt, dtype=np.float64), (n,)),
+                 np.broadcast_to(np.asarray(dL, dtype=np.float64), (n,))]
+    return np.ascontiguousarray(np.array(rows, dtype=np.float64))
+
+
+def call(fn, *args):
+    L = N.lib()
+    N.check(getattr(L, fn)(*args), L)

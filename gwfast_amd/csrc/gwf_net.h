@@ -1,0 +1,83 @@
+// Host-side construction of the network argument block (sites, arms, PSD
+// tables) from the ABI's detector descriptions (signal.py:113-147 for the
+// geometry, 144-147 for the arm angle, 745-765 / 1057 for the triangle's
+// null-stream third arm).
+#pragma once
+#include <cerrno>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "gwf_paths.h"
+
+namespace gwf {
+
+// Builds the network argument block (sites, arms, PSD tables).
+inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, std::string& err) {
+  std::memset(&net, 0, sizeof(net));
+  if (n_det < 1 || n_det > GWF_MAX_DETECTORS) { err = "number of detectors must be 1..8"; return -EINVAL; }
+  net.n_sites = n_det;
+  net.res = res;
+  net.fmin = dets[0].fmin;
+  net.fmax = dets[0].fmax;
+  int na = 0, np = 0;
+  for (int d = 0; d < n_det; ++d) {
+    const gwf_detector& g = dets[d];
+    if (g.shape != GWF_SHAPE_L && g.shape != GWF_SHAPE_T) { err = "Enter valid detector configuration"; return -EINVAL; }
+    if (!g.psd_f || !g.psd_S || g.n_psd < 2) { err = "Enter a valid PSD or ASD path"; return -EINVAL; }
+    if (g.fmin != net.fmin || g.fmax != net.fmax)
+      { err = "detectors of one call must share fmin and fmax"; return -EINVAL; }
+    if (!(g.fmin > 0.0)) { err = "fmin must be positive"; return -EINVAL; }
+    // PSD table index (tables shared between detectors are evaluated once per site anyway)
+    int p = -1;
+    for (int q = 0; q < np; ++q)
+      if (net.psd[q].f == g.psd_f && net.psd[q].S == g.psd_S) p = q;
+    if (p < 0) {
+      p = np++;
+      PsdTab& t = net.psd[p];
+      t.f = g.psd_f;
+      t.S = g.psd_S;
+      t.n = g.n_psd;
+      if (!(g.psd_f_first > 0.0) || !(g.psd_f_last > g.psd_f_first))
+        { err = "PSD frequency bounds must be positive and increasing"; return -EINVAL; }
+      t.lx0 = std::log(g.psd_f_first);
+      t.idl = (double)(g.n_psd - 1) / (std::log(g.psd_f_last) - t.lx0);
+    }
+    SiteK& S = net.site[d];
+    const double lat = g.lat_deg * M_PI / 180., lon = g.long_deg * M_PI / 180., xax = g.xax_deg * M_PI / 180.;
+    S.clat = std::cos(lat);
+    S.slat = std::sin(lat);
+    S.s2lat = std::sin(2. * lat);
+    S.c2lat = std::cos(2. * lat);
+    S.lon = lon;
+    S.sinzeta = std::sin(g.shape == GWF_SHAPE_L ? 0.5 * M_PI : M_PI / 3.);
+    S.motion = g.no_motion ? MOTION_NONE : (g.earth_motion ? MOTION_EARTH : MOTION_FIXED);
+    S.psd = p;
+    if (S.motion == MOTION_EARTH) net.any_earth = 1;
+    const int narms = g.shape == GWF_SHAPE_L ? 1 : 3;
+    if (na + narms > GWF_MAX_ARMS) { err = "too many arms"; return -EINVAL; }
+    double s2[3], c2[3];
+    for (int k = 0; k < 3; ++k) {
+      const double rot = (k * 60.) * M_PI / 180.;
+      s2[k] = std::sin(2 * (xax + rot));
+      c2[k] = std::cos(2 * (xax + rot));
+    }
+    for (int k = 0; k < narms; ++k) {
+      ArmK& A = net.arm[na + k];
+      A.site = d;
+      if (k == 2 && g.compute2arms) {  // null stream: arm 3 = -(arm 1 + arm 2) (signal.py:751, 1057)
+        A.s2x = -(s2[0] + s2[1]);
+        A.c2x = -(c2[0] + c2[1]);
+      } else {
+        A.s2x = s2[k];
+        A.c2x = c2[k];
+      }
+    }
+    na += narms;
+  }
+  net.n_arms = na;
+  return 0;
+}
+
+
+}  // namespace gwf

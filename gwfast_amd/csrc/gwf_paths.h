@@ -1,0 +1,510 @@
+// Detector projection and the per-event bodies of the path's kernels.
+//
+// Each body processes the frequency points lane, lane + nlanes, ... of one
+// event; the HIP kernels run it with one 64-wide wavefront per event and
+// reduce across lanes.
+//
+// Projection follows gwfast/signal.py: _PatternFunction 340-387 (JKS a, b),
+// _DeltLoc 401-423, GWAmplitudes 425-466, GWPhase 468-483, GWstrain 485-655.
+// The Fisher body differentiates the strain in closed form per point: with
+//   h = (Ap + i Ac) exp(i Psi),
+// every derivative is dX h = [(dX Ap - Ac dX Psi) + i (dX Ac + Ap dX Psi)]
+// exp(i Psi), and in Re(conj(da h) db h) the common factor exp(i Psi) cancels
+// exactly, so the Gram never needs the (up to ~1e9 rad) phase itself.
+#pragma once
+#include "gwf_waveforms.h"
+
+namespace gwf {
+
+constexpr int SLOT = NT_MAX + 1;  // doubles per dual slot in a record
+
+// -------------------------------------------------------------- event record
+// [waveform slots: REC_SLOTS * SLOT][event constants][site constants]
+enum EvConst {
+  EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_COUNT
+};
+enum SiteConst {
+  SC_U1 = 0, SC_U2, SC_U3, SC_U1T, SC_U2T, SC_U3T,
+  SC_E1, SC_E2, SC_E3, SC_E4, SC_E5, SC_G1, SC_G2, SC_G3, SC_G4,
+  SC_E1T, SC_E2T, SC_E3T, SC_E4T, SC_E5T, SC_G1T, SC_G2T, SC_G3T, SC_G4T,
+  SC_P1, SC_P2, SC_COUNT
+};
+constexpr int REC_EV = REC_SLOTS * SLOT;
+constexpr int REC_SITE = REC_EV + EV_COUNT;
+constexpr int REC_DOUBLES = REC_SITE + GWF_MAX_DETECTORS * SC_COUNT;
+
+// ------------------------------------------------------------ network (args)
+enum Motion { MOTION_FIXED = 0, MOTION_EARTH = 1, MOTION_NONE = 2 };
+
+struct PsdTab {
+  const double* f;
+  const double* S;
+  int n;
+  double lx0;  // log(f[0])
+  double idl;  // (n-1) / (log f[n-1] - log f[0])
+};
+
+struct SiteK {
+  double clat, slat, s2lat, c2lat, lon, sinzeta;
+  int motion;
+  int psd;
+};
+
+struct ArmK {
+  int site;
+  double s2x, c2x;  // sin/cos(2 (xax + rot)); a derived (null-stream) arm holds -(sum of two)
+};
+
+struct NetK {
+  int n_sites, n_arms, res, any_earth;
+  double fmin, fmax;
+  SiteK site[GWF_MAX_DETECTORS];
+  ArmK arm[GWF_MAX_ARMS];
+  PsdTab psd[GWF_MAX_DETECTORS];
+};
+
+// --------------------------------------------------------------------- grid
+// jnp.geomspace(fmin, fcut, res) = 10**linspace(log10 fmin, log10 fcut)
+// (JAX 0.4.30): lin_i = a (1 - i/div) + b (i/div), last point 10**b.
+GWF_HD double grid_lin(double a, double b, int i, int div) {
+  if (i >= div) return b;
+  const double s = (double)i / (double)div;
+  return a * (1 - s) + b * s;
+}
+GWF_HD double exp10d(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return exp10(x);
+#else
+  return pow(10.0, x);
+#endif
+}
+GWF_HD double grid_f(double a, double b, int i, int div) { return exp10d(grid_lin(a, b, i, div)); }
+
+// Trapezoid weight of point i (np.trapz / jnp.trapezoid as a weighted sum).
+GWF_HD double trap_w(double a, double b, int i, int div, double fi) {
+  if (div <= 0) return 0.0;
+  const double fl = (i > 0) ? grid_f(a, b, i - 1, div) : fi;
+  const double fr = (i < div) ? grid_f(a, b, i + 1, div) : fi;
+  return 0.5 * (fr - fl);
+}
+
+// jnp.interp(f, xp, fp, left=1, right=1) on the PSD table; lnf = log(f).
+GWF_HD double psd_at(const PsdTab& t, double f, double lnf) {
+  const double* x = t.f;
+  const int n = t.n;
+  if (f < x[0] || f > x[n - 1]) return 1.0;
+  int j = (int)((lnf - t.lx0) * t.idl);
+  j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
+  while (j < n - 2 && x[j + 1] <= f) ++j;
+  while (j > 0 && x[j] > f) --j;
+  const int i = j + 1;
+  return t.S[i - 1] + ((f - x[i - 1]) / (x[i] - x[i - 1])) * (t.S[i] - t.S[i - 1]);
+}
+
+GWF_HD double fcut_eff(double fcut, double fmax) { return (fmax > 0.0 && fcut > fmax) ? fmax : fcut; }
+
+// -------------------------------------------------------------- setup body
+// Fills the event record: waveform coefficients (duals) + projection constants.
+template <int MODEL, int NT>
+GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* par, int64_t ld, int64_t e, double f0,
+                        double* rec) {
+  const double Mc = par[0 * ld + e], eta = par[1 * ld + e], dL = par[2 * ld + e];
+  const double theta = par[3 * ld + e], phi = par[4 * ld + e], iota = par[5 * ld + e];
+  const double psi = par[6 * ld + e], tc = par[7 * ld + e];
+  const double chi1 = par[9 * ld + e], chi2 = par[10 * ld + e];
+  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+    const double f0u = (w.fref > 0.0) ? w.fref : f0;
+    pd_setup<NT>(mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3), f0u,
+                 w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT);
+  } else {
+    const double Lt = w.is_tidal ? par[11 * ld + e] : 0.0;
+    const double dLm = w.is_tidal ? par[12 * ld + e] : 0.0;
+    tf2_setup<NT>(w, mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3),
+                  mkseed<NT>(Lt, 4), mkseed<NT>(dLm, 5), rec, SLOT);
+  }
+  double* ev = rec + REC_EV;
+  const double ci = cos(iota), si = sin(iota);
+  ev[EV_DL] = dL;
+  ev[EV_IDL] = 1.0 / dL;
+  ev[EV_CP] = 0.5 * (1. + ci * ci);
+  ev[EV_CC] = ci;
+  ev[EV_DCP] = -(ci * si);
+  ev[EV_DCC] = -si;
+  ev[EV_TC] = tc;
+  ev[EV_PHIC] = par[8 * ld + e];
+  ev[EV_RA] = phi;
+  ev[EV_FCUT] = rec[(MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
+  // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
+  const double ra = phi, dec = 0.5 * PI - theta;
+  const double cd = cos(dec), sd = sin(dec), cr = cos(ra), sr = sin(ra);
+  const double c2d = cos(2. * dec), s2d = sin(2. * dec);
+  const double c2p = cos(2. * psi), s2p = sin(2 * psi);
+  const double k = -REARTH / CLIGHT;
+  for (int s = 0; s < net.n_sites; ++s) {
+    const SiteK& S = net.site[s];
+    double* sc = rec + REC_SITE + s * SC_COUNT;
+    // Delta t = -R/c (cd cr clat cos L + cd sr clat sin L + sd slat), L = lon + 2 pi t
+    sc[SC_U1] = k * (cd * cr * S.clat);
+    sc[SC_U2] = k * (cd * sr * S.clat);
+    sc[SC_U3] = k * (sd * S.slat);
+    sc[SC_U1T] = k * (sd * cr * S.clat);  // explicit d/dtheta (d dec/d theta = -1)
+    sc[SC_U2T] = k * (sd * sr * S.clat);
+    sc[SC_U3T] = k * (-cd * S.slat);
+    // a = s2x (E1 cos2al + E3 cos al + E5) - c2x (E2 sin2al + E4 sin al)
+    // b = c2x (G1 cos2al + G3 cos al) + s2x (G2 sin2al + G4 sin al)
+    sc[SC_E1] = 0.0625 * (3. - S.c2lat) * (3. - c2d);
+    sc[SC_E2] = 0.25 * S.slat * (3. - c2d);
+    sc[SC_E3] = 0.25 * S.s2lat * s2d;
+    sc[SC_E4] = 0.5 * S.clat * s2d;
+    sc[SC_E5] = 3. * 0.25 * (S.clat * cd) * (S.clat * cd);
+    sc[SC_G1] = S.slat * sd;
+    sc[SC_G2] = 0.25 * (3. - S.c2lat) * sd;
+    sc[SC_G3] = S.clat * cd;
+    sc[SC_G4] = 0.5 * S.s2lat * cd;
+    sc[SC_E1T] = 0.0625 * (3. - S.c2lat) * (-2. * s2d);
+    sc[SC_E2T] = 0.25 * S.slat * (-2. * s2d);
+    sc[SC_E3T] = 0.25 * S.s2lat * (-2. * c2d);
+    sc[SC_E4T] = 0.5 * S.clat * (-2. * c2d);
+    sc[SC_E5T] = 2. * 3. * 0.25 * (S.clat * S.clat) * cd * sd;
+    sc[SC_G1T] = -S.slat * cd;
+    sc[SC_G2T] = -0.25 * (3. - S.c2lat) * cd;
+    sc[SC_G3T] = S.clat * sd;
+    sc[SC_G4T] = 0.5 * S.s2lat * sd;
+    sc[SC_P1] = S.sinzeta * c2p;
+    sc[SC_P2] = S.sinzeta * s2p;
+  }
+}
+
+// ------------------------------------------------------- per-point geometry
+struct Geo {
+  double dt, dt_t, dt_th, dt_ph;       // Delta t [s] and partials (per day of t, per rad)
+  double tn;                           // time without the location delay [GMST days]
+  double Sa, Ca, Cb, Sb;               // pattern blocks at this time
+  double Sa_al, Ca_al, Cb_al, Sb_al;   // d/d alpha
+  double Sa_th, Ca_th, Cb_th, Sb_th;   // explicit d/d theta
+};
+
+GWF_HD void sincos_d(double x, double* s, double* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  sincos(x, s, c);
+#else
+  *s = sin(x);
+  *c = cos(x);
+#endif
+}
+
+template <bool WITH_DERIV>
+GWF_HD void site_geo(const SiteK& S, const double* sc, double tn, double ra, Geo& g) {
+  g.tn = tn;
+  double sL, cL;
+  sincos_d(S.lon + 2. * PI * tn, &sL, &cL);
+  g.dt = sc[SC_U1] * cL + sc[SC_U2] * sL + sc[SC_U3];
+  const double t = tn + g.dt / DAY;
+  const double al = ra - S.lon - 2. * PI * t;
+  double sa, ca;
+  sincos_d(al, &sa, &ca);
+  const double c2a = ca * ca - sa * sa;
+  const double s2a = 2. * sa * ca;
+  g.Sa = sc[SC_E1] * c2a + sc[SC_E3] * ca + sc[SC_E5];
+  g.Ca = sc[SC_E2] * s2a + sc[SC_E4] * sa;
+  g.Cb = sc[SC_G1] * c2a + sc[SC_G3] * ca;
+  g.Sb = sc[SC_G2] * s2a + sc[SC_G4] * sa;
+  if (WITH_DERIV) {
+    g.dt_t = 2. * PI * (sc[SC_U2] * cL - sc[SC_U1] * sL);
+    g.dt_th = sc[SC_U1T] * cL + sc[SC_U2T] * sL + sc[SC_U3T];
+    g.dt_ph = sc[SC_U1] * sL - sc[SC_U2] * cL;
+    g.Sa_al = -2. * sc[SC_E1] * s2a - sc[SC_E3] * sa;
+    g.Ca_al = 2. * sc[SC_E2] * c2a + sc[SC_E4] * ca;
+    g.Cb_al = -2. * sc[SC_G1] * s2a - sc[SC_G3] * sa;
+    g.Sb_al = 2. * sc[SC_G2] * c2a + sc[SC_G4] * ca;
+    g.Sa_th = sc[SC_E1T] * c2a + sc[SC_E3T] * ca + sc[SC_E5T];
+    g.Ca_th = sc[SC_E2T] * s2a + sc[SC_E4T] * sa;
+    g.Cb_th = sc[SC_G1T] * c2a + sc[SC_G3T] * ca;
+    g.Sb_th = sc[SC_G2T] * s2a + sc[SC_G4T] * sa;
+  }
+}
+
+// t without location delay for a site (signal.py:444-453).
+GWF_HD double site_tn(int motion, double tc, double tau) {
+  if (motion == MOTION_EARTH) return tc - tau / (3600. * 24);
+  if (motion == MOTION_FIXED) return tc;
+  return 0.0;
+}
+
+// ------------------------------------------------------- waveform at a point
+template <int MODEL, int NT>
+struct WfPoint {
+  D<NT> A;    // amplitude without 1/dL
+  D<NT> Phi;  // GW phase
+  D<2> tau;   // tau_star over (Mc, eta)
+};
+
+template <int MODEL, int NT>
+GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o) {
+  const D<NT> MG = ldd<NT>(rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG) * SLOT);
+  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+    const D<NT> x = MG * f;
+    o.A = pd_amp_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
+    o.Phi = pd_phi_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
+  } else {
+    const D<NT> v = dpow(PI * MG * f, 1. / 3.);
+    o.Phi = tf2_phi_at<NT>(rec, SLOT, v);
+    const double f13 = cbrt(f);
+    o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
+  }
+  if (need_tau) {
+    D<2> mg2, eta2;
+    const double* pm = rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG) * SLOT;
+    const double* pe = rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA) * SLOT;
+    mg2.v = pm[0]; mg2.d[0] = pm[1]; mg2.d[1] = pm[2];
+    eta2.v = pe[0]; eta2.d[0] = pe[1]; eta2.d[1] = pe[2];
+    o.tau = tau_star(mg2, eta2, f);
+  } else {
+    o.tau = mkc<2>(0.0);
+  }
+}
+
+// Value-only amplitude without 1/dL (SNR path).
+template <int MODEL>
+GWF_HD double wf_amp_val(const gwf_waveform& w, const double* rec, double f) {
+  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+    const double x = rec[PD_MG * SLOT] * f;
+    return pd_amp_val(rec, SLOT, x, w.apply_fcut != 0);
+  }
+  const double f13 = cbrt(f);
+  return rec[TF_AMP * SLOT] * (1.0 / (f * sqrt(f13)));
+}
+
+GWF_HD double tau_val(const double* rec, int model, double f) {
+  const int sm = model == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG;
+  const int se = model == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA;
+  return tau_star(rec[sm * SLOT], rec[se * SLOT], f);
+}
+
+// ------------------------------------------------------ strain derivatives
+// Reduced derivative vectors of one arm at one point:
+//   re[X] + i im[X] = dX h * exp(-i Psi), tcoal row already divided by 86400
+// (signal.py:919-920); X in ParNums order.
+template <int MODEL, int NT, int P>
+GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const double* sc, const Geo& g, const ArmK& arm,
+                       int motion, double f, double re[P], double im[P], double* Ap_out, double* Ac_out) {
+  const double idL = ev[EV_IDL];
+  const double cp = ev[EV_CP], cc = ev[EV_CC];
+  const double P1 = sc[SC_P1], P2 = sc[SC_P2];
+  const double s2x = arm.s2x, c2x = arm.c2x;
+  const double a = s2x * g.Sa - c2x * g.Ca;
+  const double b = c2x * g.Cb + s2x * g.Sb;
+  const double a_al = s2x * g.Sa_al - c2x * g.Ca_al;
+  const double b_al = c2x * g.Cb_al + s2x * g.Sb_al;
+  const double a_th = s2x * g.Sa_th - c2x * g.Ca_th;
+  const double b_th = c2x * g.Cb_th + s2x * g.Sb_th;
+  const double Fp = a * P1 + b * P2, Fc = b * P1 - a * P2;
+  const double Fp_al = a_al * P1 + b_al * P2, Fc_al = b_al * P1 - a_al * P2;
+  const double Fp_th = a_th * P1 + b_th * P2, Fc_th = b_th * P1 - a_th * P2;
+
+  // d t_noloc / d(Mc, eta, tc) [days per unit]
+  double tn_mc = 0., tn_eta = 0., tn_tc = 0.;
+  if (motion == MOTION_EARTH) {
+    tn_mc = -wp.tau.d[0] / DAY;
+    tn_eta = -wp.tau.d[1] / DAY;
+    tn_tc = 1.0;
+  } else if (motion == MOTION_FIXED) {
+    tn_tc = 1.0;
+  }
+  const double tfac = 1.0 + g.dt_t / DAY;  // d t / d t_noloc
+  const double tw = 2. * PI;
+  // d alpha / dX with alpha = ra - lon - 2 pi t
+  const double al_mc = -tw * tn_mc * tfac, al_eta = -tw * tn_eta * tfac, al_tc = -tw * tn_tc * tfac;
+  const double al_th = -tw * g.dt_th / DAY, al_ph = 1.0 - tw * g.dt_ph / DAY;
+
+  const double Av = wp.A.v * idL;
+  const double Ap = Av * Fp * cp, Ac = Av * Fc * cc;
+  *Ap_out = Ap;
+  *Ac_out = Ac;
+  const double twopif = 2. * PI * f;
+
+#define GWF_SET(X, dAp, dAc, dPsi)        \
+  {                                       \
+    const double dps_ = (dPsi);           \
+    re[X] = (dAp) - Ac * dps_;            \
+    im[X] = (dAc) + Ap * dps_;            \
+  }
+  // Mc, eta: waveform amplitude/phase + Earth-rotation time dependence
+  {
+    const double dA = wp.A.d[0] * idL;
+    GWF_SET(0, (dA * Fp + Av * Fp_al * al_mc) * cp, (dA * Fc + Av * Fc_al * al_mc) * cc,
+            -wp.Phi.d[0] + twopif * g.dt_t * tn_mc);
+  }
+  {
+    const double dA = wp.A.d[1] * idL;
+    GWF_SET(1, (dA * Fp + Av * Fp_al * al_eta) * cp, (dA * Fc + Av * Fc_al * al_eta) * cc,
+            -wp.Phi.d[1] + twopif * g.dt_t * tn_eta);
+  }
+  // dL
+  GWF_SET(2, -Ap * idL, -Ac * idL, 0.0);
+  // theta, phi: pattern functions (direct + via the location delay) and phiL
+  GWF_SET(3, Av * (Fp_th + Fp_al * al_th) * cp, Av * (Fc_th + Fc_al * al_th) * cc, twopif * g.dt_th);
+  GWF_SET(4, Av * Fp_al * al_ph * cp, Av * Fc_al * al_ph * cc, twopif * g.dt_ph);
+  // iota
+  GWF_SET(5, Av * Fp * ev[EV_DCP], Av * Fc * ev[EV_DCC], 0.0);
+  // psi: dFp/dpsi = 2 Fc, dFc/dpsi = -2 Fp
+  GWF_SET(6, Av * (2. * Fc) * cp, Av * (-2. * Fp) * cc, 0.0);
+  // tcoal, per second
+  GWF_SET(7, Av * Fp_al * al_tc * cp / DAY, Av * Fc_al * al_tc * cc / DAY, twopif + twopif * g.dt_t * tn_tc / DAY);
+  // Phicoal
+  GWF_SET(8, 0.0, 0.0, -1.0);
+  // spins (and tidal parameters)
+  {
+    const double dA = wp.A.d[2] * idL;
+    GWF_SET(9, dA * Fp * cp, dA * Fc * cc, -wp.Phi.d[2]);
+  }
+  {
+    const double dA = wp.A.d[3] * idL;
+    GWF_SET(10, dA * Fp * cp, dA * Fc * cc, -wp.Phi.d[3]);
+  }
+  if (P > 11) {
+    GWF_SET(11, 0.0, 0.0, -wp.Phi.d[NT > 4 ? 4 : 0]);
+    GWF_SET(12, 0.0, 0.0, -wp.Phi.d[NT > 5 ? 5 : 0]);
+  }
+#undef GWF_SET
+}
+
+// Symmetric Gram update G += w (re re^T + im im^T), packed upper triangle.
+template <int P>
+GWF_HD void gram_update(double* G, const double* re, const double* im, double w) {
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+    const double wr = w * re[a], wi = w * im[a];
+#pragma unroll
+    for (int b = a; b < P; ++b) {
+      G[k] += wr * re[b] + wi * im[b];
+      ++k;
+    }
+  }
+}
+
+// ------------------------------------------------------------ Fisher body
+// Accumulates the packed Fisher of event e over points lane, lane+nlanes, ...
+// arm_sel < 0: all arms; else only that network arm.
+template <int MODEL, int NT, int P>
+GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int lane, int nlanes, int arm_sel,
+                         double* G) {
+  const double* ev = rec + REC_EV;
+  const int res = net.res;
+  const int div = res - 1;
+  const double a = log10(net.fmin);
+  const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
+  const double tc = ev[EV_TC], ra = ev[EV_RA];
+  for (int i = lane; i < res; i += nlanes) {
+    const double lin = grid_lin(a, b, i, div);
+    const double f = exp10d(lin);
+    const double wt = 4.0 * trap_w(a, b, i, div, f);
+    const double lnf = lin * 2.302585092994045684;
+    WfPoint<MODEL, NT> wp;
+    wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
+    int arm = 0;
+    for (int s = 0; s < net.n_sites; ++s) {
+      const SiteK& S = net.site[s];
+      // arms of this site are contiguous in the arm table
+      int nA = 0;
+      while (arm + nA < net.n_arms && net.arm[arm + nA].site == s) ++nA;
+      bool any = arm_sel < 0 || (arm_sel >= arm && arm_sel < arm + nA);
+      if (any) {
+        const double* sc = rec + REC_SITE + s * SC_COUNT;
+        const double Sf = psd_at(net.psd[S.psd], f, lnf);
+        const double wS = wt / Sf;
+        Geo g;
+        site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
+        for (int k = 0; k < nA; ++k) {
+          if (arm_sel >= 0 && arm_sel != arm + k) continue;
+          double re[P], im[P], Ap, Ac;
+          arm_derivs<MODEL, NT, P>(wp, ev, sc, g, net.arm[arm + k], S.motion, f, re, im, &Ap, &Ac);
+          gram_update<P>(G, re, im, wS);
+        }
+      }
+      arm += nA;
+    }
+  }
+}
+
+// --------------------------------------------------------------- SNR body
+// Per-arm trapz((Ap^2 + Ac^2)/S, f) of site s; arms of the site accumulate
+// into acc[0..2].
+template <int MODEL>
+GWF_HD void snr_site(const gwf_waveform& w, const NetK& net, const double* rec, int s, int arm0, int nA, int lane,
+                     int nlanes, double acc[3]) {
+  const double* ev = rec + REC_EV;
+  const double* sc = rec + REC_SITE + s * SC_COUNT;
+  const SiteK& S = net.site[s];
+  const int res = net.res;
+  const int div = res - 1;
+  const double a = log10(net.fmin);
+  const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
+  const double tc = ev[EV_TC], ra = ev[EV_RA], idL = ev[EV_IDL], cp = ev[EV_CP], cc = ev[EV_CC];
+  const double P1 = sc[SC_P1], P2 = sc[SC_P2];
+  for (int i = lane; i < res; i += nlanes) {
+    const double lin = grid_lin(a, b, i, div);
+    const double f = exp10d(lin);
+    const double wt = trap_w(a, b, i, div, f);
+    const double lnf = lin * 2.302585092994045684;
+    const double Av = wf_amp_val<MODEL>(w, rec, f) * idL;
+    const double tau = (S.motion == MOTION_EARTH) ? tau_val(rec, MODEL, f) : 0.0;
+    Geo g;
+    site_geo<false>(S, sc, site_tn(S.motion, tc, tau), ra, g);
+    const double wS = wt / psd_at(net.psd[S.psd], f, lnf);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (k < nA) {
+        const ArmK& A = net.arm[arm0 + k];
+        const double aa = A.s2x * g.Sa - A.c2x * g.Ca;
+        const double bb = A.c2x * g.Cb + A.s2x * g.Sb;
+        const double Ap = Av * (aa * P1 + bb * P2) * cp;
+        const double Ac = Av * (bb * P1 - aa * P2) * cc;
+        acc[k] += wS * (Ap * Ap + Ac * Ac);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- strain body
+// h = (Ap + i Ac) exp(i Psi) and the derivative rows dX h of one arm on a
+// caller grid (signal.py:485-655, 1102-1301), Psi = 2 pi f tc 86400 - Phic -
+// Phi + phiL (signal.py:483, 628-636).  f: this event's column, row stride
+// fs.  h: complex pairs with point stride hs; J: complex pairs with parameter
+// stride js (point stride 1 pair).
+template <int MODEL, int NT, int P>
+GWF_HD void strain_event(const gwf_waveform& w, const NetK& net, const double* rec, int armi, const double* f, int64_t fs,
+                         int64_t nf, int lane, int nlanes, double* h, int64_t hs, double* J, int64_t js) {
+  const double* ev = rec + REC_EV;
+  const ArmK& arm = net.arm[armi];
+  const int s = arm.site;
+  const SiteK& S = net.site[s];
+  const double* sc = rec + REC_SITE + s * SC_COUNT;
+  const double tc = ev[EV_TC], ra = ev[EV_RA], phic = ev[EV_PHIC];
+  for (int64_t i = lane; i < nf; i += nlanes) {
+    const double fi = f[i * fs];
+    WfPoint<MODEL, NT> wp;
+    wf_point<MODEL, NT>(w, rec, fi, S.motion == MOTION_EARTH, wp);
+    Geo g;
+    site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
+    double re[P], im[P], Ap, Ac;
+    arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, fi, re, im, &Ap, &Ac);
+    const double psi = (2. * PI * fi * (tc * 3600. * 24.) - phic - wp.Phi.v) + 0. + (2. * PI * fi) * g.dt;
+    double sp, cps;
+    sincos_d(psi, &sp, &cps);
+    if (h) {
+      h[i * hs + 0] = Ap * cps - Ac * sp;
+      h[i * hs + 1] = Ap * sp + Ac * cps;
+    }
+    if (J) {
+#pragma unroll
+      for (int X = 0; X < P; ++X) {
+        J[X * js + 2 * i + 0] = re[X] * cps - im[X] * sp;
+        J[X * js + 2 * i + 1] = re[X] * sp + im[X] * cps;
+      }
+    }
+  }
+}
+
+}  // namespace gwf

@@ -1,0 +1,710 @@
+// libgwfast_hip.so: HIP kernels for gfx950 (MI355X) and the C ABI declared in
+// include/gwfast_hip.h.
+//
+// Launch geometry: one 64-wide wavefront per event (4 events per 256-thread
+// workgroup); the event index is wave-uniform so every per-event coefficient
+// is read with scalar loads from the event record, and each lane walks the
+// frequency grid with stride 64 (consecutive lanes = consecutive
+// frequencies, so region branches of the piecewise models rarely diverge).
+#include <hip/hip_runtime.h>
+
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "gwf_paths.h"
+#include "gwf_net.h"
+
+using namespace gwf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int build_net_checked(const gwf_detector* dets, int n_det, int res, NetK& net) {
+  std::string msg;
+  const int rc = build_net(dets, n_det, res, net, msg);
+  return rc ? fail(rc, msg) : 0;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  return fail(-EIO, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int BLOCK = 64 * WAVES_PER_BLOCK;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int64_t wave_event() {
+  const int wave = threadIdx.x >> 6;
+  const int64_t e = (int64_t)blockIdx.x * WAVES_PER_BLOCK + wave;
+  return (int64_t)__builtin_amdgcn_readfirstlane((int)e);
+}
+
+// ------------------------------------------------------------------ kernels
+template <int MODEL, int NT>
+__global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const double* __restrict__ par, int64_t ld,
+                                               int64_t n_ev, const double* __restrict__ fgrid, int64_t nf,
+                                               double* __restrict__ rec) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  double f0;
+  if (fgrid) {  // minimum of the caller's grid column (waveforms.py:1139)
+    f0 = fgrid[e];
+    for (int64_t i = 1; i < nf; ++i) f0 = fmin(f0, fgrid[i * n_ev + e]);
+  } else {
+    f0 = grid_f(log10(net.fmin), log10(net.fmin), 0, 1);
+  }
+  setup_event<MODEL, NT>(w, net, par, ld, e, f0, rec + e * (int64_t)REC_DOUBLES);
+}
+
+template <int MODEL, int NT, int P>
+__global__ __launch_bounds__(256) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
+                                                int arm_sel, double* __restrict__ F) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;  // wave-uniform exit
+  const int lane = threadIdx.x & 63;
+  constexpr int NG = P * (P + 1) / 2;
+  double G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) G[k] = 0.0;
+  fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, lane, 64, arm_sel, G);
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+#pragma unroll
+    for (int b = a; b < P; ++b) {
+      const double s = wave_sum(G[k]);
+      if (lane == (k & 63)) {
+        F[(int64_t)(a * P + b) * n_ev + e] = s;
+        if (a != b) F[(int64_t)(b * P + a) * n_ev + e] = s;
+      }
+      ++k;
+    }
+  }
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_snr(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
+                                             double* __restrict__ out) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;
+  const int lane = threadIdx.x & 63;
+  const double* r = rec + e * (int64_t)REC_DOUBLES;
+  int arm = 0;
+  for (int s = 0; s < net.n_sites; ++s) {
+    int nA = 0;
+    while (arm + nA < net.n_arms && net.arm[arm + nA].site == s) ++nA;
+    double acc[3] = {0.0, 0.0, 0.0};
+    snr_site<MODEL>(w, net, r, s, arm, nA, lane, 64, acc);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double v = wave_sum(acc[k]);
+      if (k < nA && lane == 0) out[(int64_t)(arm + k) * n_ev + e] = v;
+    }
+    arm += nA;
+  }
+}
+
+template <int MODEL, int NT, int P>
+__global__ __launch_bounds__(256) void k_strain(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
+                                                int armi, const double* __restrict__ f, int64_t nf,
+                                                double* __restrict__ h, double* __restrict__ J) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;
+  const int lane = threadIdx.x & 63;
+  strain_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, armi, f + e, n_ev, nf, lane, 64,
+                             h ? h + 2 * e : nullptr, 2 * n_ev, J ? J + 2 * e * nf : nullptr, 2 * n_ev * nf);
+}
+
+__global__ void k_grid(NetK net, const double* __restrict__ rec, int64_t n_ev, int res, double* __restrict__ fo,
+                       double* __restrict__ So) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)res * n_ev) return;
+  const int64_t i = idx / n_ev, e = idx % n_ev;
+  const double* ev = rec + e * (int64_t)REC_DOUBLES + REC_EV;
+  const double a = log10(net.fmin);
+  const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
+  const double lin = grid_lin(a, b, (int)i, res - 1);
+  const double f = exp10d(lin);
+  fo[idx] = f;
+  if (So) So[idx] = psd_at(net.psd[0], f, lin * 2.302585092994045684);
+}
+
+template <int MODEL, int NT>
+__global__ void k_wfeval(gwf_waveform w, const double* __restrict__ rec, int64_t n_ev, const double* __restrict__ f,
+                         int64_t nf, double* __restrict__ Ao, double* __restrict__ Po, double* __restrict__ To) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nf * n_ev) return;
+  const int64_t e = idx % n_ev;
+  const double* r = rec + e * (int64_t)REC_DOUBLES;
+  WfPoint<MODEL, NT> wp;
+  wf_point<MODEL, NT>(w, r, f[idx], To != nullptr, wp);
+  if (Ao) Ao[idx] = wp.A.v * r[REC_EV + EV_IDL];
+  if (Po) Po[idx] = wp.Phi.v;
+  if (To) To[idx] = wp.tau.v;
+}
+
+__global__ void k_fcut(gwf_waveform w, const double* __restrict__ par, int64_t ld, int64_t n_ev, double fmax,
+                       double* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  const double Mc = par[e], eta = par[ld + e];
+  double fc;
+  if (w.model == GWF_MODEL_IMRPHENOMD)
+    fc = w.fcut_par / (Mc * GMSUN_C3 / pow(eta, 3. / 5.));
+  else
+    fc = tf2_fcut(w, Mc, eta, par[9 * ld + e], par[10 * ld + e]);
+  out[e] = fcut_eff(fc, fmax);
+}
+
+// Plain Gram over materialised derivatives (one wave per event).
+template <int P>
+__global__ __launch_bounds__(256) void k_gram(const double* __restrict__ J, const double* __restrict__ f,
+                                              const double* __restrict__ S, int64_t n_ev, int64_t nf, int accumulate,
+                                              double* __restrict__ F) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;
+  const int lane = threadIdx.x & 63;
+  constexpr int NG = P * (P + 1) / 2;
+  double G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) G[k] = 0.0;
+  const int64_t js = 2 * n_ev * nf;
+  const double* Je = J + 2 * e * nf;
+  for (int64_t i = lane; i < nf; i += 64) {
+    const double fi = f[i * n_ev + e];
+    const double fl = i > 0 ? f[(i - 1) * n_ev + e] : fi;
+    const double fr = i < nf - 1 ? f[(i + 1) * n_ev + e] : fi;
+    const double wS = 4.0 * 0.5 * (fr - fl) / S[i * n_ev + e];
+    double re[P], im[P];
+#pragma unroll
+    for (int X = 0; X < P; ++X) {
+      re[X] = Je[X * js + 2 * i];
+      im[X] = Je[X * js + 2 * i + 1];
+    }
+    gram_update<P>(G, re, im, wS);
+  }
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+#pragma unroll
+    for (int b = a; b < P; ++b) {
+      const double s = wave_sum(G[k]);
+      if (lane == (k & 63)) {
+        const int64_t i1 = (int64_t)(a * P + b) * n_ev + e, i2 = (int64_t)(b * P + a) * n_ev + e;
+        F[i1] = accumulate ? F[i1] + s : s;
+        if (a != b) F[i2] = accumulate ? F[i2] + s : s;
+      }
+      ++k;
+    }
+  }
+}
+
+// ------------------------------------------------- covariance (per thread)
+constexpr int PMAX = 16;
+
+// Cyclic Jacobi eigen-decomposition of a symmetric P x P matrix (row-major
+// in A, destroyed); eigenvalues to w, eigenvectors to the columns of V.
+__device__ void jacobi_eig(double* A, int P, double* w, double* V) {
+  for (int i = 0; i < P; ++i)
+    for (int j = 0; j < P; ++j) V[i * P + j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < P; ++i) {
+      diag += A[i * P + i] * A[i * P + i];
+      for (int j = i + 1; j < P; ++j) off += A[i * P + j] * A[i * P + j];
+    }
+    if (!(off > 1e-36 * diag)) break;
+    for (int p = 0; p < P - 1; ++p) {
+      for (int q = p + 1; q < P; ++q) {
+        const double apq = A[p * P + q];
+        if (apq == 0.0) continue;
+        const double app = A[p * P + p], aqq = A[q * P + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < P; ++k) {
+          const double akp = A[k * P + p], akq = A[k * P + q];
+          A[k * P + p] = c * akp - s * akq;
+          A[k * P + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < P; ++k) {
+          const double apk = A[p * P + k], aqk = A[q * P + k];
+          A[p * P + k] = c * apk - s * aqk;
+          A[q * P + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < P; ++k) {
+          const double vkp = V[k * P + p], vkq = V[k * P + q];
+          V[k * P + p] = c * vkp - s * vkq;
+          V[k * P + q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  for (int i = 0; i < P; ++i) w[i] = A[i * P + i];
+  // ascending order (mpmath.eigh order)
+  for (int i = 1; i < P; ++i) {
+    for (int j = i; j > 0 && w[j - 1] > w[j]; --j) {
+      const double t = w[j - 1];
+      w[j - 1] = w[j];
+      w[j] = t;
+      for (int k = 0; k < P; ++k) {
+        const double v = V[k * P + j - 1];
+        V[k * P + j - 1] = V[k * P + j];
+        V[k * P + j] = v;
+      }
+    }
+  }
+}
+
+__global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, double* __restrict__ C,
+                      double* __restrict__ eps, double* __restrict__ cond, int* __restrict__ method) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  double Fm[PMAX * PMAX], A[PMAX * PMAX], V[PMAX * PMAX], L[PMAX * PMAX], w[PMAX], ws[PMAX];
+  bool allnan = true, anynan = false;
+  for (int i = 0; i < P * P; ++i) {
+    Fm[i] = F[(int64_t)i * n_ev + e];
+    const bool nn = isnan(Fm[i]);
+    allnan = allnan && nn;
+    anynan = anynan || nn;
+  }
+  auto put_nan = [&](int m) {
+    for (int i = 0; i < P * P; ++i) C[(int64_t)i * n_ev + e] = NAN;
+    if (eps) eps[e] = NAN;
+    if (cond) cond[e] = NAN;
+    if (method) method[e] = m;
+  };
+  if (allnan || anynan) {
+    put_nan(-1);
+    return;
+  }
+  // Jacobi scaling ws = diag(1/sqrt(F_ii)); a zero diagonal skips it
+  // (fisherTools.py:81-94)
+  bool reweight = true;
+  for (int i = 0; i < P; ++i) {
+    if (!(Fm[i * P + i] > 0.0)) {
+      if (Fm[i * P + i] == 0.0) {
+        reweight = false;
+      } else {
+        put_nan(-1);
+        return;
+      }
+    }
+  }
+  for (int i = 0; i < P; ++i) ws[i] = reweight ? 1.0 / sqrt(Fm[i * P + i]) : 1.0;
+  for (int i = 0; i < P; ++i)
+    for (int j = 0; j < P; ++j) A[i * P + j] = ws[i] * Fm[i * P + j] * ws[j];
+  for (int i = 0; i < P * P; ++i) L[i] = A[i];
+  jacobi_eig(L, P, w, V);  // L used as scratch
+  double emax = 0.0, emin = INFINITY;
+  bool neg = false;
+  for (int i = 0; i < P; ++i) {
+    emax = fmax(emax, fabs(w[i]));
+    emin = fmin(emin, fabs(w[i]));
+    neg = neg || (w[i] < 0.0);
+  }
+  if (cond) cond[e] = emax / emin;
+  int m = 0;
+  // Cholesky A = L L^T
+  if (!neg) {
+    for (int i = 0; i < P * P; ++i) L[i] = 0.0;
+    for (int j = 0; j < P && m == 0; ++j) {
+      double d = A[j * P + j];
+      for (int k = 0; k < j; ++k) d -= L[j * P + k] * L[j * P + k];
+      if (!(d > 0.0)) {
+        m = 1;
+        break;
+      }
+      const double ljj = sqrt(d);
+      L[j * P + j] = ljj;
+      for (int i = j + 1; i < P; ++i) {
+        double s = A[i * P + j];
+        for (int k = 0; k < j; ++k) s -= L[i * P + k] * L[j * P + k];
+        L[i * P + j] = s / ljj;
+      }
+    }
+  } else {
+    m = 1;
+  }
+  double Cc[PMAX * PMAX];
+  if (m == 0) {
+    // c = L^{-1} (lower triangular), cc = c^T c
+    double Ci[PMAX * PMAX];
+    for (int i = 0; i < P * P; ++i) Ci[i] = 0.0;
+    for (int j = 0; j < P; ++j) {
+      Ci[j * P + j] = 1.0 / L[j * P + j];
+      for (int i = j + 1; i < P; ++i) {
+        double s = 0.0;
+        for (int k = j; k < i; ++k) s -= L[i * P + k] * Ci[k * P + j];
+        Ci[i * P + j] = s / L[i * P + i];
+      }
+    }
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < P; ++k) s += Ci[k * P + i] * Ci[k * P + j];
+        Cc[i * P + j] = s;
+      }
+  } else {
+    // eigen-decomposition inverse V diag(1/w) V^T (the 'svd' fallback for a
+    // symmetric matrix); a zero eigenvalue fails like mpmath's 1/0
+    for (int i = 0; i < P; ++i) {
+      if (w[i] == 0.0) {
+        put_nan(-1);
+        return;
+      }
+    }
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] / w[k];
+        Cc[i * P + j] = s;
+      }
+  }
+  for (int i = 0; i < P; ++i)
+    for (int j = 0; j < P; ++j) {
+      const double s = 0.5 * (Cc[i * P + j] + Cc[j * P + i]);
+      C[(int64_t)(i * P + j) * n_ev + e] = ws[i] * s * ws[j];
+      A[i * P + j] = ws[i] * s * ws[j];
+    }
+  if (eps) {
+    double mx = 0.0;
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < P; ++k) s += A[i * P + k] * Fm[k * P + j];
+        mx = fmax(mx, fabs(s - (i == j ? 1.0 : 0.0)));
+      }
+    eps[e] = mx;
+  }
+  if (method) method[e] = m;
+}
+
+__global__ void k_check(const double* __restrict__ F, int P, int64_t n_ev, double* __restrict__ evals,
+                        double* __restrict__ evecs, double* __restrict__ cond) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  double A[PMAX * PMAX], V[PMAX * PMAX], w[PMAX];
+  bool allnan = true;
+  for (int i = 0; i < P * P; ++i) {
+    A[i] = F[(int64_t)i * n_ev + e];
+    allnan = allnan && isnan(A[i]);
+  }
+  if (allnan) {
+    for (int i = 0; i < P; ++i) evals[e * P + i] = NAN;
+    if (evecs)
+      for (int i = 0; i < P * P; ++i) evecs[e * P * P + i] = NAN;
+    cond[e] = NAN;
+    return;
+  }
+  jacobi_eig(A, P, w, V);
+  double emax = 0.0, emin = INFINITY;
+  for (int i = 0; i < P; ++i) {
+    evals[e * P + i] = w[i];
+    emax = fmax(emax, fabs(w[i]));
+    emin = fmin(emin, fabs(w[i]));
+  }
+  if (evecs)
+    for (int i = 0; i < P * P; ++i) evecs[e * P * P + i] = V[i];
+  cond[e] = emax / emin;
+}
+
+__global__ void k_inv_err(const double* __restrict__ F, const double* __restrict__ C, int P, int64_t n_ev,
+                          double* __restrict__ eps) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  double mx = 0.0;
+  for (int i = 0; i < P; ++i)
+    for (int j = 0; j < P; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < P; ++k) s += C[(int64_t)(i * P + k) * n_ev + e] * F[(int64_t)(k * P + j) * n_ev + e];
+      const double d = fabs(s - (i == j ? 1.0 : 0.0));
+      mx = (d > mx || isnan(d)) ? d : mx;
+    }
+  eps[e] = mx;
+}
+
+// ---------------------------------------------------------------- host side
+int check_wf(const gwf_waveform* wf) {
+  if (!wf) return fail(-EINVAL, "waveform description is NULL");
+  if (wf->model != GWF_MODEL_TAYLORF2 && wf->model != GWF_MODEL_IMRPHENOMD)
+    return fail(-EINVAL, "unknown waveform model");
+  if (wf->model == GWF_MODEL_IMRPHENOMD && (!wf->qnm_a || !wf->qnm_fring || !wf->qnm_fdamp || wf->n_qnm < 2))
+    return fail(-EINVAL, "IMRPhenomD needs the QNM tables");
+  if (wf->model == GWF_MODEL_IMRPHENOMD && wf->is_tidal) return fail(-EINVAL, "IMRPhenomD is not tidal");
+  return 0;
+}
+
+int nt_of(const gwf_waveform* wf) { return (wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) ? 6 : 4; }
+
+int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int64_t ld, int64_t n_ev,
+                 const double* fgrid, int64_t nf, double* rec, hipStream_t st) {
+  const dim3 grid((unsigned)((n_ev + 255) / 256)), block(256);
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_IMRPHENOMD, 4>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
+  else
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 4>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
+  return hip_check(hipGetLastError(), "setup launch");
+}
+
+int check_common(const gwf_waveform* wf, const double* par, int64_t ld, int64_t n_ev, void* ws, size_t ws_bytes) {
+  int rc = check_wf(wf);
+  if (rc) return rc;
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (n_ev > 0 && !par) return fail(-EINVAL, "params is NULL");
+  if (ld < n_ev) return fail(-EINVAL, "ld must be >= n_ev");
+  if (n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "too many events for one call");
+  if (ws_bytes < (size_t)n_ev * REC_DOUBLES * sizeof(double) || (n_ev > 0 && !ws))
+    return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+  return 0;
+}
+
+}  // namespace
+
+// ====================================================================== ABI
+extern "C" {
+
+int gwf_abi_version(void) { return GWF_ABI_VERSION; }
+
+const char* gwf_last_error(void) { return g_err.c_str(); }
+
+size_t gwf_workspace_bytes(int64_t n_ev) { return (size_t)(n_ev < 0 ? 0 : n_ev) * REC_DOUBLES * sizeof(double); }
+
+int gwf_num_arms(const gwf_detector* dets, int32_t n_det) {
+  if (!dets || n_det < 0) return fail(-EINVAL, "invalid detector list");
+  int n = 0;
+  for (int d = 0; d < n_det; ++d) n += dets[d].shape == GWF_SHAPE_T ? 3 : 1;
+  return n;
+}
+
+int gwf_fcut(const gwf_waveform* wf, const double* params, int64_t ld, int64_t n_ev, double fmax, double* fcut_out,
+             void* stream) {
+  int rc = check_wf(wf);
+  if (rc) return rc;
+  if (n_ev == 0) return 0;
+  if (!params || !fcut_out || ld < n_ev) return fail(-EINVAL, "invalid arguments to gwf_fcut");
+  hipLaunchKernelGGL(k_fcut, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *wf, params, ld,
+                     n_ev, fmax, fcut_out);
+  return hip_check(hipGetLastError(), "gwf_fcut launch");
+}
+
+int gwf_grid(const gwf_waveform* wf, const gwf_detector* det, const double* params, int64_t ld, int64_t n_ev,
+                int32_t res, double* f_out, double* S_out, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(wf, params, ld, n_ev, workspace, ws_bytes);
+  if (rc) return rc;
+  if (res < 2) return fail(-EINVAL, "res must be >= 2");
+  if (n_ev == 0) return 0;
+  if (!f_out) return fail(-EINVAL, "f_out is NULL");
+  NetK net;
+  rc = build_net_checked(det, 1, res, net);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_setup(wf, net, params, ld, n_ev, nullptr, 0, (double*)workspace, st);
+  if (rc) return rc;
+  const int64_t tot = (int64_t)res * n_ev;
+  hipLaunchKernelGGL(k_grid, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, net, (const double*)workspace, n_ev,
+                     (int)res, f_out, S_out);
+  return hip_check(hipGetLastError(), "gwf_grid launch");
+}
+
+int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, int64_t n_ev, const double* f,
+                      int64_t n_f, double* ampl_out, double* phi_out, double* tau_out, void* workspace,
+                      size_t ws_bytes, void* stream) {
+  int rc = check_common(wf, params, ld, n_ev, workspace, ws_bytes);
+  if (rc) return rc;
+  if (n_ev == 0 || n_f == 0) return 0;
+  if (!f || n_f < 1) return fail(-EINVAL, "frequency grid is NULL");
+  NetK net;
+  std::memset(&net, 0, sizeof(net));
+  net.fmin = 1.0;
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_setup(wf, net, params, ld, n_ev, f, n_f, (double*)workspace, st);
+  if (rc) return rc;
+  const int64_t tot = n_f * n_ev;
+  const dim3 grid((unsigned)((tot + 255) / 256)), block(256);
+  const double* rec = (const double*)workspace;
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_IMRPHENOMD, 4>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out,
+                       phi_out, tau_out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
+                       tau_out);
+  else
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 4>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
+                       tau_out);
+  return hip_check(hipGetLastError(), "gwf_waveform_eval launch");
+}
+
+int gwf_prepare(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                int64_t n_ev, int32_t res, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_common(wf, params, ld, n_ev, workspace, ws_bytes);
+  if (rc) return rc;
+  if (res < 2) return fail(-EINVAL, "res must be >= 2");
+  NetK net;
+  rc = build_net_checked(dets, n_det, res, net);
+  if (rc) return rc;
+  if (n_ev == 0) return 0;
+  return launch_setup(wf, net, params, ld, n_ev, nullptr, 0, (double*)workspace, (hipStream_t)stream);
+}
+
+int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                     double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_wf(wf);
+  if (rc) return rc;
+  if (res < 2) return fail(-EINVAL, "res must be >= 2");
+  if (n_ev < 0 || n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "invalid n_ev");
+  if (ws_bytes < (size_t)n_ev * REC_DOUBLES * sizeof(double) || (n_ev > 0 && !workspace))
+    return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+  NetK net;
+  rc = build_net_checked(dets, n_det, res, net);
+  if (rc) return rc;
+  if (n_ev == 0) return 0;
+  if (!snrsq_out) return fail(-EINVAL, "snrsq_out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  const double* rec = (const double*)workspace;
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_snr<GWF_MODEL_IMRPHENOMD>), grid, block, 0, st, *wf, net, rec, n_ev, snrsq_out);
+  else
+    hipLaunchKernelGGL((k_snr<GWF_MODEL_TAYLORF2>), grid, block, 0, st, *wf, net, rec, n_ev, snrsq_out);
+  return hip_check(hipGetLastError(), "gwf_snr launch");
+}
+
+int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                        int32_t arm, double* fisher_out, const void* workspace, size_t ws_bytes, void* stream) {
+  int rc = check_wf(wf);
+  if (rc) return rc;
+  if (res < 2) return fail(-EINVAL, "res must be >= 2");
+  if (n_ev < 0 || n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "invalid n_ev");
+  if (ws_bytes < (size_t)n_ev * REC_DOUBLES * sizeof(double) || (n_ev > 0 && !workspace))
+    return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+  NetK net;
+  rc = build_net_checked(dets, n_det, res, net);
+  if (rc) return rc;
+  if (arm >= net.n_arms) return fail(-EINVAL, "arm index out of range");
+  if (n_ev == 0) return 0;
+  if (!fisher_out) return fail(-EINVAL, "fisher_out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  const double* rec = (const double*)workspace;
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
+                       fisher_out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
+                       fisher_out);
+  else
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
+                       fisher_out);
+  return hip_check(hipGetLastError(), "gwf_fisher launch");
+}
+
+int gwf_snr(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+            int64_t n_ev, int32_t res, double* snrsq_out, void* workspace, size_t ws_bytes, void* stream) {
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  return gwf_snr_prepared(wf, dets, n_det, n_ev, res, snrsq_out, workspace, ws_bytes, stream);
+}
+
+int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+               int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes,
+               void* stream) {
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  return gwf_fisher_prepared(wf, dets, n_det, n_ev, res, arm, fisher_out, workspace, ws_bytes, stream);
+}
+
+int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,
+               int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
+               size_t ws_bytes, void* stream) {
+  int rc = check_common(wf, params, ld, n_ev, workspace, ws_bytes);
+  if (rc) return rc;
+  NetK net;
+  rc = build_net_checked(det, 1, 2, net);
+  if (rc) return rc;
+  if (arm < 0 || arm >= net.n_arms) return fail(-EINVAL, "arm index out of range");
+  if (n_ev == 0 || n_f == 0) return 0;
+  if (!f) return fail(-EINVAL, "frequency grid is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  rc = launch_setup(wf, net, params, ld, n_ev, f, n_f, (double*)workspace, st);
+  if (rc) return rc;
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  const double* rec = (const double*)workspace;
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f,
+                       n_f, h_out, J_out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
+                       h_out, J_out);
+  else
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
+                       h_out, J_out);
+  return hip_check(hipGetLastError(), "gwf_strain launch");
+}
+
+int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
+             int32_t accumulate, double* fisher_out, void* stream) {
+  if (P != 11 && P != 13) return fail(-EINVAL, "gwf_gram supports P = 11 or 13");
+  if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
+  if (n_ev == 0) return 0;
+  if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
+  if (n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "too many events for one call");
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  hipStream_t st = (hipStream_t)stream;
+  if (P == 11)
+    hipLaunchKernelGGL((k_gram<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else
+    hipLaunchKernelGGL((k_gram<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  return hip_check(hipGetLastError(), "gwf_gram launch");
+}
+
+int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
+                   int32_t* method_out, void* stream) {
+  if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (n_ev == 0) return 0;
+  if (!F || !cov_out) return fail(-EINVAL, "NULL array");
+  hipLaunchKernelGGL(k_cov, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
+                     cov_out, eps_out, cond_out, (int*)method_out);
+  return hip_check(hipGetLastError(), "gwf_cov_invert launch");
+}
+
+int gwf_inversion_error(const double* F, const double* C, int32_t P, int64_t n_ev, double* eps_out, void* stream) {
+  if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (n_ev == 0) return 0;
+  if (!F || !C || !eps_out) return fail(-EINVAL, "NULL array");
+  hipLaunchKernelGGL(k_inv_err, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, F, C, (int)P,
+                     n_ev, eps_out);
+  return hip_check(hipGetLastError(), "gwf_inversion_error launch");
+}
+
+int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out, double* evecs_out, double* cond_out,
+                     void* stream) {
+  if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (n_ev == 0) return 0;
+  if (!F || !evals_out || !cond_out) return fail(-EINVAL, "NULL array");
+  hipLaunchKernelGGL(k_check, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
+                     evals_out, evecs_out, cond_out);
+  return hip_check(hipGetLastError(), "gwf_check_fisher launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""Detector networks with gwfast's DetNet API (gwfast/network.py:12-152).
+
+When every detector of the network shares the waveform model, fmin and fmax
+(the usual case) the whole network is evaluated by ONE fused launch per
+quantity: each event's waveform and tangents are computed once per frequency
+and projected onto every arm of every detector before the Gram update.
+"""
+import numpy as np
+
+from . import signal as S
+from .gwfastUtils import check_evparams
+
+
+class DetNet(object):
+    """gwfast/network.py:12-30."""
+
+    def __init__(self, signals, verbose=True):
+        self.signals = signals
+        self.verbose = verbose
+
+    def _groups(self):
+        """Detectors that can share one launch (same model object, fmin, fmax)."""
+        groups = []
+        for name, sig in self.signals.items():
+            for g in groups:
+                s0 = g[0][1]
+                if (sig.wf_model is s0.wf_model and sig.fmin == s0.fmin and sig.fmax == s0.fmax
+                        and len(g) < 8):
+                    g.append((name, sig))
+                    break
+            else:
+                groups.append([(name, sig)])
+        return groups
+
+    def _prepare(self, evParams):
+        check_evparams(evParams)
+        sig0 = next(iter(self.signals.values()))
+        sig0._complete(evParams)
+
+    def SNR(self, evParams, res=1000, return_all=False):
+        """network.py:53-81: sqrt(sum_d SNR_d^2)."""
+        self._prepare(evParams)
+        snrs = {}
+        for g in self._groups():
+            sigs = [s for _, s in g]
+            rows = sigs[0]._rows(evParams)
+            sq = S.snr_sq_device(sigs, rows, int(res)).cpu().numpy()
+            k = 0
+            for name, s in g:
+                part = sq[k:k + s.n_arms]
+                k += s.n_arms
+                if s.detector_shape == 'T' and return_all:
+                    for i in range(3):
+                        snrs[name + '_%s' % i] = 2 * np.sqrt(part[i])
+                else:
+                    snrs[name] = 2 * np.sqrt(part.sum(axis=0))
+        net_snr = np.sqrt(np.array([snrs[k] ** 2 for k in snrs.keys()]).sum(axis=0))
+        if return_all:
+            snrs['net'] = net_snr
+            return snrs
+        return net_snr
+
+    def FisherMatr(self, evParams, return_all=False, return_derivatives=False, return_SNR_derivatives=False,
+                   **kwargs):
+        """network.py:84-152: sum of the detectors' Fisher matrices."""
+        self._prepare(evParams)
+        res = int(kwargs.pop('res', 1000))
+        S._check_fisher_options(res, kwargs.get('df'), kwargs.get('spacing', 'geom'), kwargs.get('use_m1m2', False),
+                                kwargs.get('use_chi1chi2', True), kwargs.get('computeDerivFinDiff', False))
+        if return_derivatives or return_SNR_derivatives:
+            allF, allD = {}, {}
+            for name, sig in self.signals.items():
+                F_, D_ = sig.FisherMatr(evParams, res=res, return_derivatives=return_derivatives,
+                                        return_SNR_derivatives=return_SNR_derivatives)
+                if sig.detector_shape == 'T':
+                    for i in range(3):
+                        allF[name + '_%s' % i] = F_[i]
+                        allD[name + '_%s' % i] = D_[i]
+                else:
+                    allF[name] = F_[0]
+                    allD[name] = D_[0]
+            if return_SNR_derivatives:
+                SNRs_net = kwargs['SNRs'] if 'SNRs' in kwargs else self.SNR(evParams, res=res)
+                allD['net'] = np.array([allD[k] for k in allD.keys()]).sum(axis=0) / SNRs_net
+            allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
+            return allF, allD
+        if return_all:
+            allF = {}
+            for name, sig in self.signals.items():
+                F_ = sig.FisherMatr(evParams, res=res, return_all=True)
+                if sig.detector_shape == 'T':
+                    for i in range(3):
+                        allF[name + '_%s' % i] = F_[i]
+                else:
+                    allF[name] = F_[0]
+            allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
+            return allF
+        total = None
+        for g in self._groups():
+            sigs = [s for _, s in g]
+            rows = sigs[0]._rows(evParams)
+            F = S.fisher_device(sigs, sigs[0].wf_model, rows, res)
+            total = F if total is None else total + F
+        return total.cpu().numpy()
+
+    # ------------------------------------------------------- device-resident
+    def fisher_device(self, rows, res=1000, out=None):
+        """Network Fisher of device-resident parameter rows (P_rows, N_ev):
+        returns a (P, P, N_ev) device tensor.  Single fused launch when the
+        network forms one group."""
+        groups = self._groups()
+        total = out
+        for gi, g in enumerate(groups):
+            sigs = [s for _, s in g]
+            if gi == 0:
+                total = S.fisher_device(sigs, sigs[0].wf_model, rows, res, out=total)
+            else:
+                total += S.fisher_device(sigs, sigs[0].wf_model, rows, res)
+        return total
+
+    def snr_device(self, rows, res=1000):
+        """Network SNR of device-resident parameter rows: (N_ev,) tensor."""
+        total = None
+        for g in self._groups():
+            sigs = [s for _, s in g]
+            sq = S.snr_sq_device(sigs, rows, res).sum(dim=0)
+            total = sq if total is None else total + sq
+        return 2.0 * total.sqrt()

@@ -1,0 +1,261 @@
+"""Single-detector signal, SNR and Fisher matrix with gwfast's GWSignal API.
+
+Mirrors gwfast/signal.py:37-160 (constructor), 485-655 (GWstrain),
+658-777 (SNRInteg) and 782-1099 (FisherMatr).  Every computation runs in the
+HIP library: SNR and Fisher in one fused pass per event (waveform, detector
+projection with Earth rotation, strain tangents, trapezoid integral), the
+strain and its derivative rows only when they are asked for.
+"""
+import numpy as np
+
+from . import _engine as E
+from . import _native as N
+from .gwfastUtils import check_evparams, Lam12_from_Lamt_delLam
+
+
+class GWSignal(object):
+    """gwfast/signal.py:37-160."""
+
+    def __init__(self, wf_model, psd_path=None, detector_shape='T', det_lat=40.44, det_long=9.45, det_xax=0.,
+                 verbose=True, is_ASD=True, useEarthMotion=False, noMotion=False, fmin=2., fmax=None,
+                 IntTablePath=None, DutyFactor=None, compute2arms=True, jitCompileDerivs=False):
+        if (detector_shape != 'L') and (detector_shape != 'T'):
+            raise ValueError('Enter valid detector configuration')
+        if psd_path is None:
+            raise ValueError('Enter a valid PSD or ASD path')
+        if DutyFactor is not None:
+            raise NotImplementedError('DutyFactor masks are not supported on this path')
+        self.wf_model = wf_model
+        self.verbose = verbose
+        self.detector_shape = detector_shape
+        self.det_lat, self.det_long, self.det_xax = float(det_lat), float(det_long), float(det_xax)
+        self.det_lat_rad = det_lat * np.pi / 180.
+        self.det_long_rad = det_long * np.pi / 180.
+        self.det_xax_rad = det_xax * np.pi / 180.
+        self.IntTablePath = IntTablePath
+        self.DutyFactor = DutyFactor
+        noise = np.loadtxt(psd_path, usecols=(0, 1))
+        self.strainFreq = np.ascontiguousarray(noise[:, 0])
+        self.noiseCurve = np.ascontiguousarray(noise[:, 1] ** 2 if is_ASD else noise[:, 1])
+        if np.any(np.diff(self.strainFreq) <= 0):
+            raise ValueError('PSD frequencies must be strictly increasing')
+        self.useEarthMotion = useEarthMotion
+        self.noMotion = noMotion
+        if self.noMotion and self.useEarthMotion:
+            self.useEarthMotion = False
+        self.fmin = fmin
+        self.fmax = fmax
+        self.angbtwArms = 0.5 * np.pi if detector_shape == 'L' else np.pi / 3.
+        self.compute2arms = compute2arms
+        self.jitCompileDerivs = jitCompileDerivs
+        self._psd_dev = None
+
+    # ------------------------------------------------------------------ ABI
+    def c_struct(self):
+        if self._psd_dev is None:
+            self._psd_dev = (E.to_dev(self.strainFreq), E.to_dev(self.noiseCurve))
+        d = N.GwfDetector()
+        d.shape = N.SHAPE_L if self.detector_shape == 'L' else N.SHAPE_T
+        d.earth_motion = int(self.useEarthMotion)
+        d.no_motion = int(self.noMotion)
+        d.compute2arms = int(self.compute2arms)
+        d.lat_deg, d.long_deg, d.xax_deg = self.det_lat, self.det_long, self.det_xax
+        d.fmin = float(self.fmin)
+        d.fmax = float(self.fmax) if self.fmax is not None else 0.0
+        d.psd_f = self._psd_dev[0].data_ptr()
+        d.psd_S = self._psd_dev[1].data_ptr()
+        d.n_psd = len(self.strainFreq)
+        d.psd_f_first = float(self.strainFreq[0])
+        d.psd_f_last = float(self.strainFreq[-1])
+        return d
+
+    @property
+    def n_arms(self):
+        return 1 if self.detector_shape == 'L' else 3
+
+    def _complete(self, evParams):
+        """Input completion of signal.py:684-713."""
+        check_evparams(evParams)
+        if 'chi1z' not in evParams:
+            try:
+                evParams['chi1z'] = evParams['chiS'] + evParams['chiA']
+                evParams['chi2z'] = evParams['chiS'] - evParams['chiA']
+            except KeyError:
+                raise ValueError('Two among chi1z, chi2z and chiS, chiA have to be provided.')
+        if self.wf_model.is_tidal and 'Lambda1' not in evParams:
+            try:
+                evParams['Lambda1'], evParams['Lambda2'] = Lam12_from_Lamt_delLam(
+                    evParams['LambdaTilde'], evParams['deltaLambda'], evParams['eta'])
+            except KeyError:
+                raise ValueError('Two among Lambda1, Lambda2 and LambdaTilde and deltaLambda have to be provided.')
+        return evParams
+
+    def _rows(self, evParams):
+        return E.to_dev(E.param_rows(evParams, self.wf_model.is_tidal))
+
+    # --------------------------------------------------------------- public
+    def SNRInteg(self, evParams, res=1000, return_all=False):
+        """signal.py:658-777."""
+        self._complete(evParams)
+        snrsq = snr_sq_device([self], self._rows(evParams), int(res)).cpu().numpy()
+        if return_all:
+            return 2 * np.sqrt(snrsq) if self.detector_shape == 'T' else 2 * np.sqrt(snrsq[0])
+        return 2 * np.sqrt(snrsq.sum(axis=0))
+
+    def FisherMatr(self, evParams, res=1000, df=None, spacing='geom', use_m1m2=False, use_chi1chi2=True,
+                   use_prec_ang=True, computeDerivFinDiff=False, computeAnalyticalDeriv=True, return_all=False,
+                   return_derivatives=False, return_SNR_derivatives=False, **kwargs):
+        """signal.py:782-1099.  Derivatives are exact (forward-mode) for every
+        row, which equals the reference's JAX + closed-form rows up to
+        rounding; the finite-difference option is not provided."""
+        _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff)
+        self._complete(evParams)
+        rows = self._rows(evParams)
+        if return_derivatives or return_SNR_derivatives:
+            return self._fisher_with_derivs(evParams, rows, int(res), return_derivatives)
+        if return_all:
+            return [fisher_device([self], self.wf_model, rows, int(res), arm=k).cpu().numpy() for k in range(self.n_arms)]
+        return fisher_device([self], self.wf_model, rows, int(res)).cpu().numpy()
+
+    def _fisher_with_derivs(self, evParams, rows, res, return_derivatives):
+        n = rows.shape[1]
+        f, S = grid_device(self, rows, res)
+        P = self.wf_model.nParams
+        allF, allD = [], []
+        for k in range(self.n_arms):
+            h, J = strain_device(self, rows, f, k, want_h=not return_derivatives)
+            F = gram_device(J, f, S, P)
+            allF.append(F.cpu().numpy())
+            Jc = J.cpu().numpy()
+            Jc = Jc[..., 0] + 1j * Jc[..., 1]
+            if return_derivatives:
+                allD.append(Jc)
+            else:
+                hc = h.cpu().numpy()
+                hc = hc[..., 0] + 1j * hc[..., 1]
+                fg, Sg = f.cpu().numpy(), S.cpu().numpy()
+                D = np.zeros((P, n))
+                for a in range(P):
+                    D[a] = 4. * np.trapezoid((hc * np.conjugate(Jc[a]).T).real / Sg, fg, axis=0)
+                allD.append(D)
+        return allF, allD
+
+    def GWstrain(self, f, Mc, eta, dL, theta, phi, iota, psi, tcoal, Phicoal, chiS, chiA, chi1x=0., chi2x=0.,
+                 chi1y=0., chi2y=0., LambdaTilde=0., deltaLambda=0., ecc=0., rot=0., is_m1m2=False,
+                 is_chi1chi2=False, is_prec_ang=False, is_Lam1Lam2=False, return_single_comp=None):
+        """signal.py:485-655: complex strain on the grid f (N_f, N_ev) for the
+        arm at rotation rot (0, 60, 120 degrees)."""
+        if is_m1m2 or is_prec_ang or return_single_comp is not None:
+            raise NotImplementedError('is_m1m2, is_prec_ang and return_single_comp are not on this path')
+        chi1z, chi2z = (chiS, chiA) if is_chi1chi2 else (np.asarray(chiS) + chiA, np.asarray(chiS) - chiA)
+        ev = {'Mc': Mc, 'eta': eta, 'dL': dL, 'theta': theta, 'phi': phi, 'iota': iota, 'psi': psi,
+              'tcoal': tcoal, 'Phicoal': Phicoal, 'chi1z': chi1z, 'chi2z': chi2z}
+        if self.wf_model.is_tidal:
+            if is_Lam1Lam2:
+                ev['Lambda1'], ev['Lambda2'] = LambdaTilde, deltaLambda
+            else:
+                ev['LambdaTilde'], ev['deltaLambda'] = LambdaTilde, deltaLambda
+        ev = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in ev.items()}
+        arm = int(round(rot / 60.))
+        if abs(rot - 60. * arm) > 1e-12 or arm not in range(self.n_arms):
+            raise ValueError('rot must be 0 (L) or 0, 60, 120 degrees (T)')
+        rows = self._rows(ev)
+        f2 = np.asarray(f, dtype=np.float64)
+        if f2.ndim == 1:
+            f2 = f2[:, None] * np.ones((1, rows.shape[1]))
+        fd = E.to_dev(f2)
+        h, _ = strain_device(self, rows, fd, arm, want_h=True, want_J=False, null_stream=False)
+        hc = h.cpu().numpy()
+        return hc[..., 0] + 1j * hc[..., 1]
+
+    def GWPhase(self, evParams, f):
+        """signal.py:468-483: 2 pi f tcoal - Phicoal - Phi_GW."""
+        return (2. * np.pi * f * (evParams['tcoal'] * 3600. * 24.) - evParams['Phicoal']
+                - self.wf_model.Phi(f, **evParams))
+
+
+def _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff):
+    if res is None:
+        raise NotImplementedError('df-spaced grids are not supported: pass res')
+    if spacing != 'geom':
+        raise NotImplementedError("only spacing='geom' is supported")
+    if use_m1m2 or not use_chi1chi2:
+        raise NotImplementedError('Fisher matrices are computed in (Mc, eta, chi1z, chi2z)')
+    if computeDerivFinDiff:
+        raise NotImplementedError('finite-difference derivatives are not provided (derivatives are exact)')
+
+
+# ------------------------------------------------------------ device helpers
+def _dets(signals):
+    arr = (N.GwfDetector * len(signals))(*[s.c_struct() for s in signals])
+    return arr
+
+
+def snr_sq_device(signals, rows, res):
+    """Per-arm trapz((Ap^2+Ac^2)/S, f) for a list of GWSignal sharing one model,
+    fmin and fmax: (n_arms, N_ev) device tensor."""
+    t = E.torch()
+    wf = signals[0].wf_model
+    n = rows.shape[1]
+    narms = sum(s.n_arms for s in signals)
+    out = t.empty((narms, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    E.call('gwf_snr', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, res, N.ptr(out),
+           N.ptr(ws), nb, N.stream_ptr())
+    return out
+
+
+def fisher_device(signals, wf, rows, res, arm=-1, out=None):
+    """Fused network Fisher (P, P, N_ev) device tensor; arm >= 0 selects one arm."""
+    t = E.torch()
+    n = rows.shape[1]
+    P = wf.nParams
+    if out is None:
+        out = t.empty((P, P, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    E.call('gwf_fisher', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, res, arm,
+           N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    return out
+
+
+def grid_device(sig, rows, res):
+    t = E.torch()
+    n = rows.shape[1]
+    f = t.empty((res, n), dtype=t.float64, device=rows.device)
+    S = t.empty((res, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    E.call('gwf_grid', sig.wf_model.c_struct(), _dets([sig]), N.ptr(rows), rows.stride(0), n, res, N.ptr(f), N.ptr(S),
+           N.ptr(ws), nb, N.stream_ptr())
+    return f, S
+
+
+def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True):
+    """h (N_f, N_ev, 2) and J (P, N_ev, N_f, 2) device tensors for one arm."""
+    t = E.torch()
+    n = rows.shape[1]
+    nf = f.shape[0]
+    P = sig.wf_model.nParams
+    h = t.empty((nf, n, 2), dtype=t.float64, device=rows.device) if want_h else None
+    J = t.empty((P, n, nf, 2), dtype=t.float64, device=rows.device) if want_J else None
+    ws, nb = E.workspace(n)
+    det = sig.c_struct()
+    if not null_stream:
+        det.compute2arms = 0
+    E.call('gwf_strain', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
+           N.ptr(h), N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
+    return h, J
+
+
+def gram_device(J, f, S, P, out=None, accumulate=False):
+    t = E.torch()
+    n = J.shape[1]
+    nf = J.shape[2]
+    if out is None:
+        out = t.empty((P, P, n), dtype=t.float64, device=J.device)
+    E.call('gwf_gram', N.ptr(J), N.ptr(f), N.ptr(S), P, n, nf, int(accumulate), N.ptr(out), N.stream_ptr())
+    return out
+
+
+def ctypes_byref(x):
+    import ctypes
+    return ctypes.byref(x)

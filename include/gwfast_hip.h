@@ -1,0 +1,185 @@
+/*
+ * gwfast_hip.h -- C ABI of libgwfast_hip.so, the MI355X (gfx950) kernels of
+ * gwfast's event-batched waveform -> detector projection -> SNR / Fisher ->
+ * covariance path.
+ *
+ * gwfast v1.1.2 is pure Python (JAX on CPU) and has no FFI of its own; its
+ * plug-in point for this path is the WaveFormModel ABC consumed by GWSignal
+ * (gwfast/waveforms.py:49-199, gwfast/signal.py:68-160).  Each entry point
+ * below replaces one reference operation, cited at its declaration.  The
+ * Python layer (gwfast_amd.waveforms / signal / network / fisherTools) keeps
+ * the reference's class and method signatures and calls these through
+ * ctypes.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (caller-owned, e.g. torch
+ *     tensors); structs are passed by host pointer and copied by value.
+ *   - Event parameters are FP64 SoA rows of length >= n_ev with row stride
+ *     `ld` (elements), rows in gwfast's Fisher order (waveforms.py:78, 106-111,
+ *     125-129):
+ *        0 Mc [Msun, detector frame]  1 eta  2 dL [Gpc]  3 theta  4 phi
+ *        5 iota  6 psi  7 tcoal [GMST days]  8 Phicoal  9 chi1z  10 chi2z
+ *       11 LambdaTilde  12 deltaLambda            (rows 11-12 tidal only)
+ *   - Fisher/covariance outputs use the reference layout (P, P, n_ev):
+ *     element (a, b) of event e at [(a * P + b) * n_ev + e].
+ *   - `stream` is a hipStream_t; every call is stream-ordered and returns
+ *     before the work completes.  Calls never allocate device memory.
+ *   - Return 0 on success, a negative errno-style code on failure; the
+ *     message is available from gwf_last_error() (thread-local).
+ */
+#ifndef GWFAST_HIP_H
+#define GWFAST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GWF_ABI_VERSION 1
+
+enum { GWF_MODEL_TAYLORF2 = 0, GWF_MODEL_IMRPHENOMD = 1 };
+enum { GWF_SHAPE_L = 0, GWF_SHAPE_T = 1 };
+#define GWF_MAX_DETECTORS 8
+#define GWF_MAX_ARMS 24
+
+/* Waveform model description: TaylorF2_RestrictedPN(...) (waveforms.py:726)
+ * or IMRPhenomD(fRef=None) (waveforms.py:973). */
+typedef struct gwf_waveform {
+  int32_t model;            /* GWF_MODEL_* */
+  int32_t is_tidal;         /* TaylorF2 is_tidal: 13 parameters */
+  int32_t use_3p5PN_SpinHO; /* TaylorF2 */
+  int32_t phiref_vlso;      /* TaylorF2 */
+  int32_t kerr_isco;        /* TaylorF2 which_ISCO == 'Kerr' */
+  int32_t use_quadmon;      /* TaylorF2 use_QuadMonTid */
+  int32_t apply_fcut;       /* WaveFormModel apply_fcut (IMRPhenomD) */
+  int32_t n_qnm;            /* IMRPhenomD QNM table length */
+  double fcut_par;          /* TaylorF2 fHigh [Hz Msun]; IMRPhenomD 0.2 */
+  double fref;              /* IMRPhenomD fixed fRef [Hz]; <= 0: grid minimum */
+  const double* qnm_a;      /* device, IMRPhenomD QNMData_a */
+  const double* qnm_fring;  /* device, QNMData_fring */
+  const double* qnm_fdamp;  /* device, QNMData_fdamp */
+} gwf_waveform;
+
+/* One detector site: GWSignal(...) (signal.py:68-82). */
+typedef struct gwf_detector {
+  int32_t shape;        /* GWF_SHAPE_L or GWF_SHAPE_T */
+  int32_t earth_motion; /* useEarthMotion */
+  int32_t no_motion;    /* noMotion */
+  int32_t compute2arms; /* T only: third arm = -(arm1 + arm2) */
+  double lat_deg, long_deg, xax_deg;
+  double fmin;          /* Hz */
+  double fmax;          /* Hz; <= 0: none */
+  const double* psd_f;  /* device, strictly increasing frequencies [Hz] */
+  const double* psd_S;  /* device, PSD (ASD^2 when the file is an ASD) */
+  int32_t n_psd;
+  int32_t reserved;
+  double psd_f_first;   /* host copy of psd_f[0] */
+  double psd_f_last;    /* host copy of psd_f[n_psd - 1] */
+} gwf_detector;
+
+int gwf_abi_version(void);
+const char* gwf_last_error(void);
+
+/* Bytes of device workspace the calls below need for n_ev events
+ * (per-event coefficient records). */
+size_t gwf_workspace_bytes(int64_t n_ev);
+
+/* Number of interferometer arms of a network: L -> 1, T -> 3. */
+int gwf_num_arms(const gwf_detector* dets, int32_t n_det);
+
+/* WaveFormModel.fcut(**ev) (waveforms.py:190-199, 903-953, 1324-1333),
+ * clipped to fmax when fmax > 0 (signal.py:715-718). fcut_out: (n_ev,). */
+int gwf_fcut(const gwf_waveform* wf, const double* params, int64_t ld, int64_t n_ev, double fmax,
+             double* fcut_out, void* stream);
+
+/* Frequency grid and PSD on it, as GWSignal.SNRInteg / FisherMatr build them
+ * (signal.py:715-723, 884-901): jnp.geomspace(fmin, fcut, res) and
+ * jnp.interp(f, psd_f, psd_S, left=1, right=1).  f_out, S_out: (res, n_ev);
+ * S_out may be NULL. */
+int gwf_grid(const gwf_waveform* wf, const gwf_detector* det, const double* params, int64_t ld, int64_t n_ev,
+             int32_t res, double* f_out, double* S_out, void* workspace, size_t ws_bytes, void* stream);
+
+/* WaveFormModel.Ampl / Phi / tau_star on a caller grid f (n_f, n_ev)
+ * (waveforms.py:148-188).  Any output pointer may be NULL.  IMRPhenomD's
+ * fRef is the minimum of each event's grid column (waveforms.py:1139). */
+int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, int64_t n_ev, const double* f,
+                      int64_t n_f, double* ampl_out, double* phi_out, double* tau_out, void* workspace,
+                      size_t ws_bytes, void* stream);
+
+/* GWSignal.SNRInteg for every arm of a network (signal.py:658-777; network
+ * sum in network.py:53-81 is left to the caller).  snrsq_out: (n_arms, n_ev)
+ * holding trapz((Ap^2 + Ac^2)/S, f), i.e. SNR_arm = 2 sqrt(snrsq). */
+int gwf_snr(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+            int64_t n_ev, int32_t res, double* snrsq_out, void* workspace, size_t ws_bytes, void* stream);
+
+/* GWSignal.FisherMatr summed over arms and detectors, i.e. DetNet.FisherMatr
+ * (signal.py:782-1099, network.py:84-152), with derivatives in ParNums order
+ * and the tcoal row in seconds (signal.py:919-920).  Fused: strain tangents
+ * and the trapezoid Gram are computed in one pass and never written to
+ * memory.  arm < 0 sums all arms; arm = k returns arm k only (network arm
+ * order: detectors in order, T detectors contributing 3 arms).
+ * fisher_out: (P, P, n_ev), P = 11 (13 when the model is tidal). */
+int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+               int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes,
+               void* stream);
+
+/* The two halves of gwf_snr / gwf_fisher, for callers that reuse the
+ * per-event records (e.g. SNR then Fisher of the same events) or time the
+ * integral kernels alone.  gwf_prepare fills the workspace with the event
+ * records of a (waveform, network, res) triple; the *_prepared calls must be
+ * given the same triple and workspace. */
+int gwf_prepare(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                int64_t n_ev, int32_t res, void* workspace, size_t ws_bytes, void* stream);
+int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                     double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream);
+int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                        int32_t arm, double* fisher_out, const void* workspace, size_t ws_bytes, void* stream);
+
+/* GWSignal.GWstrain and the derivative rows of _SignalDerivatives for one arm
+ * of one detector on the caller grid f (n_f, n_ev) (signal.py:485-655,
+ * 1102-1301): h_out (n_f, n_ev) complex128 (interleaved re, im) and
+ * J_out (P, n_ev, n_f) complex128 with the tcoal row divided by 86400, as
+ * FisherMatr(return_derivatives=True) returns them.  arm: 0..2 (T) or 0 (L);
+ * arm 2 of a compute2arms triangle is -(arm0 + arm1).  Either output may be
+ * NULL. */
+int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,
+               int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
+               size_t ws_bytes, void* stream);
+
+/* The reference's plain Fisher integral from materialised derivatives
+ * (signal.py:922-931): F_ab = 4 trapz(Re(conj(J_a) J_b)/S, f) for J
+ * (P, n_ev, n_f) complex128, f and S (n_f, n_ev).  fisher_out (P, P, n_ev);
+ * accumulate != 0 adds into fisher_out. */
+int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
+             int32_t accumulate, double* fisher_out, void* stream);
+
+/* fisherTools.CovMatr(F, invMethodIn='cho', alt_method='svd') and
+ * compute_inversion_error (fisherTools.py:32-210): per event, Jacobi scaling
+ * ws = diag(1/sqrt(F_ii)), eigenvalues of the scaled matrix, Cholesky
+ * inverse when all eigenvalues are >= 0 and the factorisation succeeds,
+ * otherwise the eigen-decomposition ('svd') inverse; symmetrised and
+ * unscaled.  All-NaN input gives NaN output.
+ *   cov_out (P, P, n_ev), eps_out (n_ev,) = max|Cov F - I|,
+ *   cond_out (n_ev,) condition number of the scaled matrix,
+ *   method_out (n_ev,) 0 cholesky, 1 eigen fallback, -1 NaN.
+ * Any output except cov_out may be NULL. P <= 16. */
+int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
+                   int32_t* method_out, void* stream);
+
+/* fisherTools.compute_inversion_error (fisherTools.py:199-210) for given
+ * Fisher and covariance (P, P, n_ev): eps_out (n_ev,) = max|C F - I|. */
+int gwf_inversion_error(const double* F, const double* C, int32_t P, int64_t n_ev, double* eps_out, void* stream);
+
+/* fisherTools.CheckFisher (fisherTools.py:216-279): eigenvalues ascending
+ * evals_out (n_ev, P), eigenvectors evecs_out (n_ev, P, P) (column j is the
+ * eigenvector of eigenvalue j), cond_out (n_ev,) = max|l| / min|l|.
+ * evecs_out may be NULL. P <= 16. */
+int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out, double* evecs_out, double* cond_out,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GWFAST_HIP_H */

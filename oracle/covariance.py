@@ -1,0 +1,85 @@
+"""Oracle restatement of gwfast's covariance inversion and Fisher checks
+(test infrastructure only).
+
+Follows gwfast/fisherTools.py:32-196 (CovMatr, default 'cho' method with the
+'svd' fallback), 199-210 (compute_inversion_error) and 216-279 (CheckFisher).
+Like the reference it works per event in mpmath at the default 53-bit
+precision; the output is float64 here (the reference stores float128 where the
+platform has it, fisherTools.py:22-27).
+"""
+import mpmath
+import numpy as np
+
+
+def cov_matr(F, alt_method='svd'):
+    """Returns (Cov (P,P,N), eps (N,), method (N,) with 0 = cholesky,
+    1 = svd fallback, -1 = NaN input/failure)."""
+    P, _, N = F.shape
+    cov = np.zeros(F.shape)
+    method = np.zeros(N, dtype=int)
+    for k in range(N):
+        Fk = F[:, :, k]
+        if np.all(np.isnan(Fk)):
+            cov[:, :, k] = np.nan
+            method[k] = -1
+            continue
+        ff = mpmath.matrix(Fk.tolist())
+        try:
+            E, _ = mpmath.eigh(ff)
+            E = np.array([float(e) for e in E])
+            reweighted = False
+            try:
+                ws = mpmath.diag([1 / mpmath.sqrt(ff[i, i]) for i in range(P)])
+                Fp = ws * ff * ws
+                EE, _ = mpmath.eigh(Fp)
+                E = np.array([float(e) for e in EE])
+                reweighted = True
+            except ZeroDivisionError:
+                Fp = ff
+            inv_method = 'cho'
+            c = None
+            if np.any(E < 0):
+                inv_method = alt_method
+            else:
+                try:
+                    c = (mpmath.cholesky(Fp)) ** -1
+                except Exception:
+                    inv_method = alt_method
+            if inv_method == 'cho':
+                cc = c.T * c
+                method[k] = 0
+            else:
+                U, Sm, V = mpmath.svd_r(Fp)
+                cc = V.T * mpmath.diag([1 / s for s in Sm]) * U.T
+                method[k] = 1
+            cc = (cc + cc.T) / 2
+            C = ws * cc * ws if reweighted else cc
+            cov[:, :, k] = np.array(C.tolist(), dtype=float)
+        except Exception:
+            cov[:, :, k] = np.nan
+            method[k] = -1
+    return cov, compute_inversion_error(F, cov), method
+
+
+def compute_inversion_error(F, C):
+    """fisherTools.py:199-210: max |C F - I| per event."""
+    P = F.shape[0]
+    return np.array([np.max(np.abs(C[:, :, i] @ F[:, :, i] - np.eye(P))) for i in range(F.shape[-1])])
+
+
+def check_fisher(F):
+    """fisherTools.py:216-279 with use_mpmath=True: eigenvalues (N,P),
+    eigenvectors (N,P,P), condition number max|l|/min|l| (N,)."""
+    P, _, N = F.shape
+    evals = np.zeros((N, P))
+    evecs = np.zeros((N, P, P))
+    for k in range(N):
+        if np.all(np.isnan(F[:, :, k])):
+            evals[k] = np.nan
+            evecs[k] = np.nan
+            continue
+        E, ER = mpmath.eigh(mpmath.matrix(F[:, :, k].tolist()))
+        evals[k] = np.array([float(e) for e in E])
+        evecs[k] = np.array(ER.tolist(), dtype=float)
+    cond = np.abs(evals).max(axis=1) / np.abs(evals).min(axis=1)
+    return evals, evecs, cond

@@ -1,0 +1,353 @@
+"""Oracle restatement of gwfast's single-detector projection, SNR and Fisher
+(test infrastructure only).
+
+Follows gwfast/signal.py:
+  * constructor semantics 68-160 (PSD = ASD^2, radians, arm angle);
+  * _PatternFunction 340-387, _DeltLoc 401-423, GWAmplitudes 425-466,
+    GWPhase 468-483, GWstrain 485-655 (aligned-spin, non-LAL branch);
+  * SNRInteg 658-777, FisherMatr 782-1099 (L and T with compute2arms);
+  * _SignalDerivatives 1102-1301 (AD rows: Mc, eta, chi1z, chi2z
+    [LambdaTilde, deltaLambda]; analytic rows spliced in ParNums order);
+  * _AnalyticalDerivatives 1303-1584.
+and gwfast/network.py:53-152 (network SNR and Fisher sums).
+"""
+import numpy as np
+
+from . import dual as D
+from .waveforms import lam12_from_lamt_dellam, lamt_dellam_from_lam12
+from gwfast_amd import gwfastGlobals as glob
+
+DAY = 3600. * 24.
+
+
+class Detector:
+    """Geometry + noise of one detector site (signal.py:68-160)."""
+
+    def __init__(self, psd_path, shape, lat, long, xax, is_ASD=True, useEarthMotion=False,
+                 fmin=2., fmax=None, compute2arms=True, noMotion=False):
+        noise = np.loadtxt(psd_path, usecols=(0, 1))
+        self.strainFreq = noise[:, 0]
+        self.noiseCurve = noise[:, 1] ** 2 if is_ASD else noise[:, 1]
+        self.detector_shape = shape
+        self.det_lat_rad = lat * np.pi / 180.
+        self.det_long_rad = long * np.pi / 180.
+        self.det_xax_rad = xax * np.pi / 180.
+        self.useEarthMotion = useEarthMotion and not noMotion
+        self.noMotion = noMotion
+        self.fmin = fmin
+        self.fmax = fmax
+        self.compute2arms = compute2arms
+        self.angbtwArms = 0.5 * np.pi if shape == 'L' else np.pi / 3.
+
+    # -- geometry ------------------------------------------------------------
+    def _ab(self, ra, dec, t, rot):
+        """JKS a(t), b(t) of signal.py:359-376."""
+        phir = self.det_long_rad
+        xr = self.det_xax_rad + rot
+        lat = self.det_lat_rad
+        arg = ra - phir - 2. * np.pi * t
+        a1 = 0.0625 * np.sin(2 * xr) * (3. - np.cos(2. * lat)) * (3. - D.cos(2. * dec)) * D.cos(2. * arg)
+        a2 = 0.25 * np.cos(2 * xr) * np.sin(lat) * (3. - D.cos(2. * dec)) * D.sin(2. * arg)
+        a3 = 0.25 * np.sin(2 * xr) * np.sin(2. * lat) * D.sin(2. * dec) * D.cos(arg)
+        a4 = 0.5 * np.cos(2 * xr) * np.cos(lat) * D.sin(2. * dec) * D.sin(arg)
+        a5 = 3. * 0.25 * np.sin(2 * xr) * (np.cos(lat) * D.cos(dec)) ** 2.
+        b1 = np.cos(2 * xr) * np.sin(lat) * D.sin(dec) * D.cos(2. * arg)
+        b2 = 0.25 * np.sin(2 * xr) * (3. - np.cos(2. * lat)) * D.sin(dec) * D.sin(2. * arg)
+        b3 = np.cos(2 * xr) * np.cos(lat) * D.cos(dec) * D.cos(arg)
+        b4 = 0.5 * np.sin(2 * xr) * np.sin(2. * lat) * D.cos(dec) * D.sin(arg)
+        return a1 - a2 + a3 - a4 + a5, b1 + b2 + b3 + b4
+
+    def pattern_function(self, theta, phi, t, psi, rot=0.):
+        """signal.py:340-387; rot in degrees."""
+        rot_rad = rot * np.pi / 180.
+        ras, decs = phi, 0.5 * np.pi - theta
+        afac, bfac = self._ab(ras, decs, t, rot_rad)
+        Fp = np.sin(self.angbtwArms) * (afac * D.cos(2. * psi) + bfac * D.sin(2 * psi))
+        Fc = np.sin(self.angbtwArms) * (bfac * D.cos(2. * psi) - afac * D.sin(2 * psi))
+        return Fp, Fc
+
+    def delt_loc(self, theta, phi, t):
+        """signal.py:401-423, seconds."""
+        ras, decs = phi, 0.5 * np.pi - theta
+        comp1 = D.cos(decs) * D.cos(ras) * np.cos(self.det_lat_rad) * D.cos(self.det_long_rad + 2. * np.pi * t)
+        comp2 = D.cos(decs) * D.sin(ras) * np.cos(self.det_lat_rad) * D.sin(self.det_long_rad + 2. * np.pi * t)
+        comp3 = D.sin(decs) * np.sin(self.det_lat_rad)
+        return -glob.REarth * (comp1 + comp2 + comp3) / glob.clight
+
+    def _times(self, wf, ev, f):
+        """(t_noloc, Delta t(t_noloc), t) as in signal.py:444-453 / 612-626."""
+        tcoal = ev['tcoal']
+        if self.noMotion:
+            tnoloc = 0. * tcoal
+        elif self.useEarthMotion:
+            tnoloc = tcoal - wf.tau_star(f, ev) / (3600. * 24)
+        else:
+            tnoloc = tcoal
+        dt = self.delt_loc(ev['theta'], ev['phi'], tnoloc)
+        return tnoloc, dt, tnoloc + dt / (3600. * 24.)
+
+    def gw_amplitudes(self, wf, ev, f, rot=0.):
+        """signal.py:425-466."""
+        _, _, t = self._times(wf, ev, f)
+        Fp, Fc = self.pattern_function(ev['theta'], ev['phi'], t, ev['psi'], rot=rot)
+        wfAmpl = wf.Ampl(f, ev)
+        Ap = wfAmpl * Fp * 0.5 * (1. + (D.cos(ev['iota'])) ** 2)
+        Ac = wfAmpl * Fc * D.cos(ev['iota'])
+        return Ap, Ac
+
+    def gw_phase(self, wf, ev, f):
+        """signal.py:468-483."""
+        return 2. * np.pi * f * (ev['tcoal'] * 3600. * 24.) - ev['Phicoal'] - wf.Phi(f, ev)
+
+    def gw_strain(self, wf, ev, f, rot=0.):
+        """signal.py:485-655 for aligned-spin, non-LAL models: complex strain."""
+        _, dt, _ = self._times(wf, ev, f)
+        phiL = (2. * np.pi * f) * dt
+        Ap, Ac = self.gw_amplitudes(wf, ev, f, rot=rot)
+        Psi = self.gw_phase(wf, ev, f)
+        Psi = Psi + 0. + phiL
+        return (Ap + 1j * Ac) * D.exp(Psi * 1j)
+
+    # -- integrals -------------------------------------------------------------
+    def grid(self, wf, ev, res):
+        fcut = wf.fcut(ev)
+        if self.fmax is not None:
+            fcut = np.where(fcut > self.fmax, self.fmax, fcut)
+        fminarr = np.full(np.shape(fcut), self.fmin)
+        fgrids = geomspace_jax(fminarr, fcut, int(res))
+        strainGrids = np.interp(fgrids, self.strainFreq, self.noiseCurve, left=1., right=1.)
+        return fgrids, strainGrids
+
+    def snr_sq_arms(self, wf, ev, res=1000):
+        """Per-arm SNR^2/4 integrals (signal.py:715-765): list of (N_ev,)."""
+        fgrids, strainGrids = self.grid(wf, ev, res)
+        if self.detector_shape == 'L':
+            Ap, Ac = self.gw_amplitudes(wf, ev, fgrids)
+            return [np.trapezoid((Ap * Ap + Ac * Ac) / strainGrids, fgrids, axis=0)]
+        if not self.compute2arms:
+            out = []
+            for i in range(3):
+                Ap, Ac = self.gw_amplitudes(wf, ev, fgrids, rot=i * 60.)
+                out.append(np.trapezoid((Ap * Ap + Ac * Ac) / strainGrids, fgrids, axis=0))
+            return out
+        Ap1, Ac1 = self.gw_amplitudes(wf, ev, fgrids, rot=0.)
+        Ap2, Ac2 = self.gw_amplitudes(wf, ev, fgrids, rot=60.)
+        Ap3, Ac3 = -(Ap1 + Ap2), -(Ac1 + Ac2)
+        return [np.trapezoid((Ap * Ap + Ac * Ac) / strainGrids, fgrids, axis=0)
+                for Ap, Ac in ((Ap1, Ac1), (Ap2, Ac2), (Ap3, Ac3))]
+
+    def snr(self, wf, ev, res=1000, return_all=False):
+        """signal.py:658-777."""
+        allSNRsq = np.array(self.snr_sq_arms(wf, ev, res))
+        if return_all:
+            return 2 * np.sqrt(allSNRsq) if self.detector_shape == 'T' else np.squeeze(2 * np.sqrt(allSNRsq), axis=0)
+        if self.detector_shape == 'T':
+            return 2 * np.sqrt(allSNRsq.sum(axis=0))
+        return np.squeeze(2 * np.sqrt(allSNRsq), axis=0)
+
+    # -- derivatives -------------------------------------------------------------
+    def ad_derivatives(self, wf, ev, fgrids, rot=0.):
+        """AD rows of signal.py:1143-1189 via forward-mode duals: derivatives
+        of GWstrain w.r.t. Mc, eta, chi1z, chi2z (+ LambdaTilde, deltaLambda
+        for tidal models).  Returns complex array (n_AD, N_ev, N_f)."""
+        tidal = wf.is_tidal
+        K = 6 if tidal else 4
+        evd = dict(ev)
+        evd['Mc'] = D.Dual.seed(ev['Mc'], 0, K)
+        evd['eta'] = D.Dual.seed(ev['eta'], 1, K)
+        evd['chi1z'] = D.Dual.seed(ev['chi1z'], 2, K)
+        evd['chi2z'] = D.Dual.seed(ev['chi2z'], 3, K)
+        if tidal:
+            Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
+            Ltd = D.Dual.seed(Lt, 4, K)
+            dLd = D.Dual.seed(dLm, 5, K)
+            L1, L2 = lam12_from_lamt_dellam(Ltd, dLd, evd['eta'])
+            evd['Lambda1'], evd['Lambda2'] = L1, L2
+        h = self.gw_strain(wf, evd, fgrids, rot=rot)
+        return np.transpose(h.d, (0, 2, 1))
+
+    def analytic_derivatives(self, wf, ev, f, rot=0.):
+        """signal.py:1303-1584: closed-form rows (dL, theta, phi, iota, psi,
+        tcoal, Phicoal), each (N_f, N_ev) complex."""
+        theta, phi, iota, psi = ev['theta'], ev['phi'], ev['iota'], ev['psi']
+        tcoal, Phicoal, dL = ev['tcoal'], ev['Phicoal'], ev['dL']
+        wfPhiGw = wf.Phi(f, ev)
+        wfAmpl = wf.Ampl(f, ev)
+        wfhp = wfAmpl * np.exp(-1j * wfPhiGw) * 0.5 * (1. + (np.cos(iota)) ** 2)
+        wfhc = 1j * wfAmpl * np.exp(-1j * wfPhiGw) * np.cos(iota)
+        tnoloc, tmpDeltLoc, t = self._times(wf, ev, f)
+        phiL = (2. * np.pi * f) * tmpDeltLoc
+        rot_rad = rot * np.pi / 180.
+        ras, decs = phi, 0.5 * np.pi - theta
+        lat, phir = self.det_lat_rad, self.det_long_rad
+        xr = self.det_xax_rad + rot_rad
+        s2x, c2x = np.sin(2 * xr), np.cos(2 * xr)
+        sz = np.sin(self.angbtwArms)
+        afac, bfac = self._ab(ras, decs, t, rot_rad)
+        Fp = sz * (afac * np.cos(2. * psi) + bfac * np.sin(2 * psi))
+        Fc = sz * (bfac * np.cos(2. * psi) - afac * np.sin(2 * psi))
+        ephase = np.exp(1j * (2. * np.pi * f * (tcoal * 3600. * 24.) - Phicoal + 0. + phiL))
+        hp, hc = wfhp * Fp * ephase, wfhc * Fc * ephase
+
+        def ab_rader(t):
+            # derivative of a, b w.r.t. the angle (ra - phir - 2 pi t) (1441-1467)
+            arg = ras - phir - 2. * np.pi * t
+            a1 = 0.0625 * s2x * (3. - np.cos(2. * lat)) * (3. - np.cos(2. * decs)) * (-2.) * np.sin(2. * arg)
+            a2 = 0.25 * c2x * np.sin(lat) * (3. - np.cos(2. * decs)) * 2. * np.cos(2. * arg)
+            a3 = -0.25 * s2x * np.sin(2. * lat) * np.sin(2. * decs) * np.sin(arg)
+            a4 = 0.5 * c2x * np.cos(lat) * np.sin(2. * decs) * np.cos(arg)
+            b1 = c2x * np.sin(lat) * np.sin(decs) * (-2.) * np.sin(2. * arg)
+            b2 = 0.25 * s2x * (3. - np.cos(2. * lat)) * np.sin(decs) * 2. * np.cos(2. * arg)
+            b3 = -c2x * np.cos(lat) * np.cos(decs) * np.sin(arg)
+            b4 = 0.5 * s2x * np.sin(2. * lat) * np.cos(decs) * np.cos(arg)
+            return a1 - a2 + a3 - a4, b1 + b2 + b3 + b4
+
+        # psi (1432-1437)
+        Fp_psider = 2 * sz * (-afac * np.sin(2. * psi) + bfac * np.cos(2 * psi))
+        Fc_psider = 2 * sz * (-bfac * np.sin(2. * psi) - afac * np.cos(2 * psi))
+        psi_deriv = wfhp * Fp_psider * ephase + wfhc * Fc_psider * ephase
+
+        # phi (1439-1480)
+        c1 = -np.cos(decs) * np.sin(ras) * np.cos(lat) * np.cos(phir + 2. * np.pi * tnoloc)
+        c2 = np.cos(decs) * np.cos(ras) * np.cos(lat) * np.sin(phir + 2. * np.pi * tnoloc)
+        locDt_phider = (-glob.REarth * (c1 + c2) / glob.clight) / (3600. * 24.)
+        ar, br = ab_rader(t)
+        afac_phider = ar * (1. - 2. * np.pi * locDt_phider)
+        bfac_phider = br * (1. - 2. * np.pi * locDt_phider)
+        Fp_phider = sz * (afac_phider * np.cos(2. * psi) + bfac_phider * np.sin(2 * psi))
+        Fc_phider = sz * (bfac_phider * np.cos(2. * psi) - afac_phider * np.sin(2 * psi))
+        phiL_phideriv = 2. * np.pi * f * locDt_phider * (3600. * 24.)
+        phi_deriv = (wfhp * Fp_phider * ephase + 1j * (0. + phiL_phideriv) * hp
+                     + wfhc * Fc_phider * ephase + 1j * (0. + phiL_phideriv) * hc)
+
+        # theta (1482-1523)
+        c1 = np.sin(decs) * np.cos(ras) * np.cos(lat) * np.cos(phir + 2. * np.pi * tnoloc)
+        c2 = np.sin(decs) * np.sin(ras) * np.cos(lat) * np.sin(phir + 2. * np.pi * tnoloc)
+        c3 = -np.cos(decs) * np.sin(lat)
+        lth = (-glob.REarth * (c1 + c2 + c3) / glob.clight) / (3600. * 24.)
+        arg = ras - phir - 2. * np.pi * t
+        a1 = (0.0625 * s2x * (3. - np.cos(2. * lat)) * (-2. * np.sin(2. * decs)) * np.cos(2. * arg)
+              + 0.0625 * s2x * (3. - np.cos(2. * lat)) * (3. - np.cos(2. * decs)) * np.sin(2. * arg) * (4. * np.pi * lth))
+        a2 = (0.25 * c2x * np.sin(lat) * (-2. * np.sin(2. * decs)) * np.sin(2. * arg)
+              - 0.25 * c2x * np.sin(lat) * (3. - np.cos(2. * decs)) * np.cos(2. * arg) * (4. * np.pi * lth))
+        a3 = (-0.25 * s2x * np.sin(2. * lat) * 2. * np.cos(2. * decs) * np.cos(arg)
+              + 0.25 * s2x * np.sin(2. * lat) * np.sin(2. * decs) * np.sin(arg) * (2. * np.pi * lth))
+        a4 = (-0.5 * c2x * np.cos(lat) * 2. * np.cos(2. * decs) * np.sin(arg)
+              - 0.5 * c2x * np.cos(lat) * np.sin(2. * decs) * np.cos(arg) * (2. * np.pi * lth))
+        a5 = 2. * 3. * 0.25 * s2x * ((np.cos(lat)) ** 2) * np.cos(decs) * np.sin(decs)
+        afac_thder = a1 - a2 + a3 - a4 + a5
+        b1 = (-c2x * np.sin(lat) * np.cos(decs) * np.cos(2. * arg)
+              + c2x * np.sin(lat) * np.sin(decs) * np.sin(2. * arg) * (4. * np.pi * lth))
+        b2 = (-0.25 * s2x * (3. - np.cos(2. * lat)) * np.cos(decs) * np.sin(2. * arg)
+              - 0.25 * s2x * (3. - np.cos(2. * lat)) * np.sin(decs) * np.cos(2. * arg) * (4. * np.pi * lth))
+        b3 = (c2x * np.cos(lat) * np.sin(decs) * np.cos(arg)
+              + c2x * np.cos(lat) * np.cos(decs) * np.sin(arg) * (2. * np.pi * lth))
+        b4 = (0.5 * s2x * np.sin(2. * lat) * np.sin(decs) * np.sin(arg)
+              - 0.5 * s2x * np.sin(2. * lat) * np.cos(decs) * np.cos(arg) * (2. * np.pi * lth))
+        bfac_thder = b1 + b2 + b3 + b4
+        Fp_thder = sz * (afac_thder * np.cos(2. * psi) + bfac_thder * np.sin(2 * psi))
+        Fc_thder = sz * (bfac_thder * np.cos(2. * psi) - afac_thder * np.sin(2 * psi))
+        phiL_thderiv = 2. * np.pi * f * lth * (3600. * 24.)
+        theta_deriv = (wfhp * Fp_thder * ephase + 1j * (0. + phiL_thderiv) * hp
+                       + wfhc * Fc_thder * ephase + 1j * (0. + phiL_thderiv) * hc)
+
+        # tcoal (1525-1565)
+        c1 = -np.cos(decs) * np.cos(ras) * np.cos(lat) * np.sin(phir + 2. * np.pi * tnoloc)
+        c2 = np.cos(decs) * np.sin(ras) * np.cos(lat) * np.cos(phir + 2. * np.pi * tnoloc)
+        locDt_tcder = (-2. * np.pi * glob.REarth * (c1 + c2) / glob.clight) / (3600. * 24.)
+        ar, br = ab_rader(t)
+        afac_tcder = -2. * np.pi * ar * (1. + locDt_tcder)
+        bfac_tcder = -2. * np.pi * br * (1. + locDt_tcder)
+        Fp_tcder = sz * (afac_tcder * np.cos(2. * psi) + bfac_tcder * np.sin(2 * psi))
+        Fc_tcder = sz * (bfac_tcder * np.cos(2. * psi) - afac_tcder * np.sin(2 * psi))
+        phiL_tcderiv = 2. * np.pi * f * locDt_tcder * (3600. * 24.)
+        tc_deriv = (wfhp * Fp_tcder * ephase + 1j * (0. + phiL_tcderiv + 2. * np.pi * f * 3600. * 24.) * hp
+                    + wfhc * Fc_tcder * ephase + 1j * (0. + phiL_tcderiv + 2. * np.pi * f * 3600. * 24.) * hc)
+
+        # iota (1567-1574)
+        wfhp_i = -wfAmpl * np.exp(-1j * wfPhiGw) * (np.cos(iota) * np.sin(iota))
+        wfhc_i = -1j * wfAmpl * np.exp(-1j * wfPhiGw) * np.sin(iota)
+        iota_deriv = wfhp_i * Fp * ephase + wfhc_i * Fc * ephase
+
+        dL_deriv = -(hp + hc) / dL
+        Phicoal_deriv = -1j * (hp + hc)
+        return dL_deriv, theta_deriv, phi_deriv, iota_deriv, psi_deriv, tc_deriv, Phicoal_deriv
+
+    def signal_derivatives(self, wf, ev, fgrids, rot=0.):
+        """(P, N_ev, N_f) complex in ParNums order (signal.py:1283-1299)."""
+        ad = self.ad_derivatives(wf, ev, fgrids, rot=rot)
+        an = self.analytic_derivatives(wf, ev, fgrids, rot=rot)
+        rows = [ad[0], ad[1]] + [np.asarray(a).T for a in an] + [ad[k] for k in range(2, ad.shape[0])]
+        return np.array(rows)
+
+    def full_dual_derivatives(self, wf, ev, fgrids, rot=0.):
+        """All P rows by forward-mode duals of GWstrain (no closed forms):
+        used to pin the closed-form rows.  Returns (P, N_ev, N_f)."""
+        P = wf.nParams
+        keys = ['Mc', 'eta', 'dL', 'theta', 'phi', 'iota', 'psi', 'tcoal', 'Phicoal', 'chi1z', 'chi2z']
+        evd = dict(ev)
+        for k, key in enumerate(keys):
+            evd[key] = D.Dual.seed(ev[key], k, P)
+        if wf.is_tidal:
+            Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
+            L1, L2 = lam12_from_lamt_dellam(D.Dual.seed(Lt, 11, P), D.Dual.seed(dLm, 12, P), evd['eta'])
+            evd['Lambda1'], evd['Lambda2'] = L1, L2
+        h = self.gw_strain(wf, evd, fgrids, rot=rot)
+        return np.transpose(h.d, (0, 2, 1))
+
+    def _fisher_from_derivs(self, derivs, strainGrids, fgrids, P, tcelem=7):
+        derivs = np.array(derivs)
+        derivs[tcelem] /= (3600. * 24.)
+        F = np.zeros((P, P, derivs.shape[1]))
+        for a in range(P):
+            for b in range(a, P):
+                integ = (np.conjugate(derivs[a]) * derivs[b]).T
+                F[a, b] = np.trapezoid(integ.real / strainGrids.real, fgrids.real, axis=0) * 4.
+                F[b, a] = F[a, b]
+        return F
+
+    def fisher_arms(self, wf, ev, res=1000, mode='faithful'):
+        """Per-arm Fisher matrices (signal.py:782-1099), list of (P, P, N_ev).
+
+        mode='faithful' follows the reference's derivative construction (AD
+        rows through exp(i Psi), closed-form rows with their own phase
+        association); mode='dual' takes every row from duals of GWstrain."""
+        fgrids, strainGrids = self.grid(wf, ev, res)
+        P = wf.nParams
+        deriv = self.signal_derivatives if mode == 'faithful' else self.full_dual_derivatives
+        if self.detector_shape == 'L':
+            return [self._fisher_from_derivs(deriv(wf, ev, fgrids), strainGrids, fgrids, P)]
+        if not self.compute2arms:
+            return [self._fisher_from_derivs(deriv(wf, ev, fgrids, rot=i * 60.), strainGrids, fgrids, P) for i in range(3)]
+        d1 = deriv(wf, ev, fgrids, rot=0.)
+        d2 = deriv(wf, ev, fgrids, rot=60.)
+        d3 = -(d1 + d2)
+        return [self._fisher_from_derivs(d, strainGrids, fgrids, P) for d in (d1, d2, d3)]
+
+    def fisher(self, wf, ev, res=1000, mode='faithful'):
+        return np.sum(self.fisher_arms(wf, ev, res, mode), axis=0)
+
+
+def geomspace_jax(start, stop, num):
+    """jnp.geomspace as logspace(log10 start, log10 stop): interior points from
+    jnp.linspace's start*(1-s) + stop*s with s = i/(num-1), the last point
+    10**log10(stop) (JAX 0.4.30).  Axis 0 is frequency."""
+    start = np.asarray(start, dtype=float)
+    stop = np.asarray(stop, dtype=float)
+    a, b = np.log10(start), np.log10(stop)
+    div = num - 1
+    s = (np.arange(div, dtype=float) / div)[:, None]
+    lin = a[None, :] * (1 - s) + b[None, :] * s
+    lin = np.concatenate([lin, b[None, :]], axis=0)
+    return 10.0 ** lin
+
+
+def network_snr(dets, wf, ev, res=1000):
+    """network.py:53-81: sqrt(sum_d SNR_d^2)."""
+    snrs = [d.snr(wf, ev, res) for d in dets]
+    return np.sqrt(np.array([s ** 2 for s in snrs]).sum(axis=0))
+
+
+def network_fisher(dets, wf, ev, res=1000, mode='faithful'):
+    """network.py:84-152: sum of detector Fishers, (P, P, N_ev)."""
+    return np.array([d.fisher(wf, ev, res, mode) for d in dets]).sum(axis=0)

@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line('markers', 'gpu: needs an MI355X (ROCm GPU) and the built HIP library')
+    config.addinivalue_line('markers', 'slow: takes more than a few seconds on the CPU')
+
+
+@pytest.fixture(scope='session')
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail('GPU test selected but no ROCm GPU is visible')
+    from gwfast_amd import _native
+    _native.lib()
+    return torch.device('cuda', 0)

@@ -1,0 +1,217 @@
+"""GPU parity of the HIP path against the CPU oracle (FP64).
+
+Tolerances: SNR <= 1e-10 relative; Fisher <= 1e-6 normalised
+(|dF_ij| / sqrt(F_ii F_jj)) against the oracle's reference-faithful
+construction, and <= 1e-10 against the oracle's all-dual construction (the
+same mathematics without the reference's phase-association rounding).
+"""
+import numpy as np
+import pytest
+
+from tests._common import NET_C0, NET_C1, bbh, bns, hip_net, norm_err, oracle_net
+
+pytestmark = pytest.mark.gpu
+
+
+def _models(name, **kw):
+    from gwfast_amd import waveforms as W
+    from oracle import waveforms as OW
+    if name == 'TF2':
+        return W.TaylorF2_RestrictedPN(**kw), OW.TaylorF2(**kw)
+    return W.IMRPhenomD(**kw), OW.IMRPhenomD(**kw)
+
+
+CASES = [
+    # id, model, model kwargs, sites, earth motion, n events, res, event draw
+    ('C0-TF2-L', 'TF2', {}, NET_C0, False, 16, 1000, bbh),
+    ('C0-TF2-L-spinHO', 'TF2', {'use_3p5PN_SpinHO': True}, NET_C0, False, 6, 500, bbh),
+    ('C1-PhenomD-net', 'PD', {}, NET_C1, True, 8, 2000, bbh),
+    ('C1-PhenomD-net-static', 'PD', {}, NET_C1, False, 6, 700, bbh),
+    ('C3-TF2-tidal-ET', 'TF2', {'is_tidal': True}, NET_C1[:1], True, 6, 1500, bns),
+]
+
+
+@pytest.mark.parametrize('case', CASES, ids=[c[0] for c in CASES])
+def test_snr_parity(gpu, case):
+    cid, model, kw, sites, earth, n, res, draw = case
+    from oracle import detector as OD
+    wf, owf = _models(model, **kw)
+    ev = draw(n, seed=3)
+    net = hip_net(wf, sites, earth)
+    snr = net.SNR(dict(ev), res=res)
+    ref = OD.network_snr(oracle_net(sites, earth), owf, ev, res)
+    err = np.max(np.abs(snr - ref) / ref)
+    print(cid, 'SNR max rel err', err, flush=True)
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize('case', CASES, ids=[c[0] for c in CASES])
+def test_fisher_parity(gpu, case):
+    cid, model, kw, sites, earth, n, res, draw = case
+    from oracle import detector as OD
+    wf, owf = _models(model, **kw)
+    ev = draw(n, seed=5)
+    net = hip_net(wf, sites, earth)
+    F = net.FisherMatr(dict(ev), res=res)
+    odets = oracle_net(sites, earth)
+    Fd = OD.network_fisher(odets, owf, ev, res, mode='dual')
+    Ff = OD.network_fisher(odets, owf, ev, res, mode='faithful')
+    e_d, e_f = norm_err(F, Fd), norm_err(F, Ff)
+    print(cid, 'Fisher err vs dual %.3e vs faithful %.3e' % (e_d, e_f), flush=True)
+    assert F.shape == (wf.nParams, wf.nParams, n)
+    assert e_d <= 1e-10
+    assert e_f <= 1e-6
+
+
+def test_fisher_per_arm_sums_to_network(gpu):
+    wf, _ = _models('PD')
+    ev = bbh(5, seed=7)
+    net = hip_net(wf, NET_C1, True)
+    allF = net.FisherMatr(dict(ev), res=400, return_all=True)
+    assert set(allF) == {'ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM', 'net'}
+    tot = net.FisherMatr(dict(ev), res=400)
+    assert norm_err(allF['net'], tot) <= 1e-13
+
+
+def test_snr_return_all(gpu):
+    from oracle import detector as OD
+    wf, owf = _models('PD')
+    ev = bbh(5, seed=8)
+    net = hip_net(wf, NET_C1, True)
+    s = net.SNR(dict(ev), res=300, return_all=True)
+    od = oracle_net(NET_C1, True)
+    arms = od[0].snr(owf, ev, 300, return_all=True)
+    for i in range(3):
+        assert np.max(np.abs(s['ETS_%d' % i] - arms[i]) / arms[i]) <= 1e-10
+    assert np.max(np.abs(s['CE1Id'] - od[1].snr(owf, ev, 300)) / s['CE1Id']) <= 1e-10
+
+
+@pytest.mark.parametrize('model', ['TF2', 'PD'])
+def test_strain_and_derivatives(gpu, model):
+    """GWstrain and FisherMatr(return_derivatives=True) against the oracle's
+    strain and derivative rows, and the plain (materialised-J) Fisher against
+    the fused one."""
+    from oracle import detector as OD
+    wf, owf = _models(model)
+    ev = bbh(4, seed=9)
+    sites = NET_C1[:1]
+    net = hip_net(wf, sites, True)
+    sig = net.signals['ETS']
+    od = oracle_net(sites, True)[0]
+    res = 300
+    fgrids, _ = od.grid(owf, ev, res)
+    h = sig.GWstrain(fgrids, ev['Mc'], ev['eta'], ev['dL'], ev['theta'], ev['phi'], ev['iota'], ev['psi'],
+                     ev['tcoal'], ev['Phicoal'], ev['chi1z'], ev['chi2z'], rot=60., is_chi1chi2=True)
+    ho = od.gw_strain(owf, ev, fgrids, rot=60.)
+    assert np.max(np.abs(h - ho)) <= 1e-6 * np.max(np.abs(ho))
+    Fs, Ds = sig.FisherMatr(dict(ev), res=res, return_derivatives=True)
+    Do = od.signal_derivatives(owf, ev, fgrids, rot=0.)
+    Do[7] /= 86400.
+    scale = np.max(np.abs(Do), axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(Ds[0] - Do) / scale) <= 1e-5
+    Ffused = sig.FisherMatr(dict(ev), res=res, return_all=True)
+    for k in range(3):
+        assert norm_err(Fs[k], Ffused[k]) <= 1e-9
+
+
+@pytest.mark.parametrize('model', ['TF2', 'PD'])
+def test_waveform_methods(gpu, model):
+    wf, owf = _models(model)
+    ev = bbh(5, seed=12)
+    od = oracle_net(NET_C0, False)[0]
+    f, _ = od.grid(owf, ev, 256)
+    for meth in ('Ampl', 'tau_star'):
+        g = getattr(wf, meth)(f, **ev)
+        o = getattr(owf, meth)(f, ev)
+        assert np.max(np.abs(g - o) / np.abs(o)) <= 1e-11, meth
+    g = wf.Phi(f, **ev)
+    o = owf.Phi(f, ev)
+    assert np.max(np.abs(g - o)) <= 1e-12 * np.max(np.abs(o)) + 1e-9
+    assert np.allclose(wf.fcut(**ev), owf.fcut(ev), rtol=1e-14, atol=0)
+
+
+def test_covariance_parity(gpu):
+    from gwfast_amd import fisherTools as FT
+    from oracle import covariance as OC
+    wf, _ = _models('PD')
+    ev = bbh(12, seed=13)
+    net = hip_net(wf, NET_C1, True)
+    F = net.FisherMatr(dict(ev), res=500)
+    C, eps, cond, meth = FT.CovMatr(F, return_details=True)
+    Co, epso, metho = OC.cov_matr(F)
+    # compare marginalised errors sigma_i = sqrt(Cov_ii) where the reference used cholesky
+    ok = (meth == 0) & (metho == 0)
+    assert ok.sum() >= 8
+    s = np.sqrt(np.abs(np.einsum('iin->in', C)))[:, ok]
+    so = np.sqrt(np.abs(np.einsum('iin->in', Co)))[:, ok]
+    rel = np.abs(s - so) / so
+    print('sigma rel err max', rel.max(), 'cond range', cond.min(), cond.max(), flush=True)
+    assert np.max(rel / np.maximum(1.0, cond[ok] * 1e-13)) <= 1e-6
+    assert np.all(np.isfinite(eps))
+    # inversion error agrees with the standalone computation
+    eps2 = FT.compute_inversion_error(F, C)
+    assert np.allclose(eps, eps2, rtol=1e-12, atol=0)
+
+
+def test_check_fisher_parity(gpu):
+    from gwfast_amd import fisherTools as FT
+    from oracle import covariance as OC
+    wf, _ = _models('TF2')
+    ev = bbh(6, seed=14)
+    net = hip_net(wf, NET_C0, False)
+    F = net.FisherMatr(dict(ev), res=400)
+    ev_, vec, cond = FT.CheckFisher(F)
+    evo, veco, condo = OC.check_fisher(F)
+    lam_max = np.max(np.abs(evo), axis=1, keepdims=True)
+    assert np.max(np.abs(ev_ - evo) / lam_max) <= 1e-12
+    # F v = lambda v for every returned pair
+    for k in range(F.shape[-1]):
+        Fk = F[:, :, k]
+        r = Fk @ vec[k] - vec[k] * ev_[k][None, :]
+        assert np.max(np.abs(r)) <= 1e-10 * lam_max[k, 0]
+
+
+def test_cov_failure_modes(gpu):
+    """All-NaN input -> NaN; indefinite -> eigen fallback; zero diagonal with a
+    singular matrix -> NaN (fisherTools.py:63-66, 98-105, 183-187)."""
+    from gwfast_amd import fisherTools as FT
+    P = 11
+    F = np.zeros((P, P, 3))
+    F[:, :, 0] = np.nan
+    A = np.diag(np.arange(1.0, P + 1))
+    A[0, 1] = A[1, 0] = 5.0  # indefinite 2x2 block
+    F[:, :, 1] = A
+    F[:, :, 2] = np.diag(np.r_[0.0, np.ones(P - 1)])
+    C, eps, cond, meth = FT.CovMatr(F, return_details=True)
+    assert np.all(np.isnan(C[:, :, 0])) and meth[0] == -1
+    assert meth[1] == 1 and np.allclose(C[:, :, 1] @ A, np.eye(P), atol=1e-12)
+    assert np.all(np.isnan(C[:, :, 2])) and meth[2] == -1
+
+
+def test_full_size_properties(gpu):
+    """Config-1 shapes (1e4 events, 2000 frequencies, 5 interferometers):
+    symmetry, the closed forms F[dL,dL] = SNR^2/dL^2 and F[Phic,Phic] = SNR^2
+    (signal.py: the strain is linear in 1/dL and exp(-i Phic)), and bitwise
+    determinism of two runs."""
+    import torch
+    from gwfast_amd import _engine as E
+    wf, _ = _models('PD')
+    n, res = 10000, 2000
+    ev = bbh(n, seed=21)
+    net = hip_net(wf, NET_C1, True)
+    sig0 = next(iter(net.signals.values()))
+    sig0._complete(ev)
+    rows = E.to_dev(E.param_rows(ev, False))
+    F1 = net.fisher_device(rows, res)
+    F2 = net.fisher_device(rows, res)
+    snr = net.snr_device(rows, res)
+    torch.cuda.synchronize()
+    assert torch.equal(F1, F2)
+    F = F1.cpu().numpy()
+    snr = snr.cpu().numpy()
+    assert np.all(np.isfinite(F))
+    assert np.max(np.abs(F - F.transpose(1, 0, 2))) == 0.0
+    dL = ev['dL']
+    assert np.max(np.abs(F[2, 2] * dL ** 2 / snr ** 2 - 1.0)) <= 1e-10
+    assert np.max(np.abs(F[8, 8] / snr ** 2 - 1.0)) <= 1e-10
+    assert np.max(np.abs(F[2, 8]) / np.sqrt(F[2, 2] * F[8, 8])) <= 1e-12
