@@ -1,0 +1,161 @@
+"""CPU tests of the boundary: the C-ABI library loads and exports every symbol
+the header declares (no compute calls), the host-side API logic, the
+catalogue generator, and the fit-coefficient tables against the reference's
+source text."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gwfast_amd import _native as N
+from gwfast_amd import catalog, gwfastGlobals as glob, gwfastUtils as U
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'gwfast_hip.h')
+REF = '/root/reference/gwfast/waveforms.py'
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r'/\*.*?\*/', '', txt, flags=re.S)
+    return sorted(set(re.findall(r'\b(gwf_[a-z_0-9]+)\s*\(', txt)))
+
+
+@pytest.fixture(scope='module')
+def so_path():
+    if not os.path.exists(N.LIB_PATH):
+        from gwfast_amd import build
+        build.build(verbose=False)
+    return N.LIB_PATH
+
+
+def test_library_exports_every_declared_symbol(so_path):
+    L = ctypes.CDLL(so_path)
+    decl = _declared()
+    assert len(decl) >= 15
+    for name in decl:
+        assert hasattr(L, name), name
+    # ctypes binding covers exactly the declared ABI
+    assert sorted(N.SIGNATURES) == decl
+
+
+def test_library_binds_without_gpu(so_path):
+    L = N.load(so_path)
+    assert L.gwf_abi_version() == 1
+    assert L.gwf_workspace_bytes(10) > 0 and L.gwf_workspace_bytes(10) % 80 == 0
+    d = (N.GwfDetector * 3)()
+    d[0].shape, d[1].shape, d[2].shape = N.SHAPE_T, N.SHAPE_L, N.SHAPE_L
+    assert L.gwf_num_arms(d, 3) == 5
+    # argument validation happens before any device work
+    rc = L.gwf_cov_invert(None, 40, 1, None, None, None, None, None)
+    assert rc == -22 and b'P must be' in L.gwf_last_error()
+
+
+def test_struct_layouts_match_header():
+    # field order and sizes of the ABI structs
+    assert ctypes.sizeof(N.GwfWaveform) == 8 * 4 + 2 * 8 + 3 * 8
+    assert ctypes.sizeof(N.GwfDetector) == 4 * 4 + 5 * 8 + 2 * 8 + 2 * 4 + 2 * 8
+
+
+# ------------------------------------------------------------------ host API
+def test_parnums_and_nparams():
+    from gwfast_amd.waveforms import IMRPhenomD, TaylorF2_RestrictedPN
+    assert list(TaylorF2_RestrictedPN().ParNums) == ['Mc', 'eta', 'dL', 'theta', 'phi', 'iota', 'psi', 'tcoal',
+                                                     'Phicoal', 'chi1z', 'chi2z']
+    t = TaylorF2_RestrictedPN(is_tidal=True)
+    assert t.nParams == 13 and t.ParNums['LambdaTilde'] == 11 and t.ParNums['deltaLambda'] == 12
+    assert t.objType == 'BNS' and t.is_holomorphic
+    d = IMRPhenomD()
+    assert d.nParams == 11 and d.fcutPar == 0.2 and d.objType == 'BBH'
+    assert len(d.QNMgrid_a) == 1003
+    with pytest.raises(NotImplementedError):
+        TaylorF2_RestrictedPN(is_eccentric=True)
+
+
+def test_gwsignal_validation():
+    from gwfast_amd.signal import GWSignal
+    from gwfast_amd.waveforms import IMRPhenomD
+    with pytest.raises(ValueError, match='valid detector configuration'):
+        GWSignal(IMRPhenomD(), psd_path=glob.PSD_ET, detector_shape='X')
+    with pytest.raises(ValueError, match='PSD or ASD path'):
+        GWSignal(IMRPhenomD())
+    s = GWSignal(IMRPhenomD(), psd_path=glob.PSD_ET, detector_shape='T', useEarthMotion=True, noMotion=True)
+    assert not s.useEarthMotion  # noMotion wins (signal.py:139-141)
+    assert s.noiseCurve[0] == pytest.approx(3.4959517e-17 ** 2)
+
+
+def test_check_evparams_completion_and_errors():
+    ev = {'m1': np.array([30.]), 'm2': np.array([20.]), 'ra': np.array([1.]), 'dec': np.array([0.2]),
+          'thetaJN': np.array([0.3]), 'tcoal': np.array([0.])}
+    U.check_evparams(ev)
+    assert np.allclose(ev['theta'], np.pi / 2 - 0.2) and ev['phi'][0] == 1. and ev['iota'][0] == 0.3
+    Mc, eta = U.Mceta_from_m1m2(30., 20.)
+    assert ev['Mc'][0] == pytest.approx(Mc) and ev['eta'][0] == pytest.approx(0.24)
+    with pytest.raises(ValueError, match='tGPS and tcoal'):
+        U.check_evparams({'Mc': 1.})
+
+
+def test_tidal_conversions_round_trip():
+    eta = np.array([0.24, 0.2499, 0.2])
+    L1, L2 = np.array([300., 10., 1500.]), np.array([700., 2000., 100.])
+    Lt, dL = U.Lamt_delLam_from_Lam12(L1, L2, eta)
+    a, b = U.Lam12_from_Lamt_delLam(Lt, dL, eta)
+    assert np.allclose(a, L1, rtol=1e-11) and np.allclose(b, L2, rtol=1e-11)
+    # equal masses: LambdaTilde = (L1 + L2)/2 ... for L1 = L2 = L it equals L
+    Lt, _ = U.Lamt_delLam_from_Lam12(400., 400., 0.25)
+    assert Lt == pytest.approx(400., rel=1e-12)
+
+
+# ---------------------------------------------------------------- catalogue
+def test_luminosity_distance_quadrature():
+    from scipy.integrate import quad
+    z = np.array([0.1, 1.0, 2.0])
+    ref = [(1 + zz) * 2.99792458e5 / 67.7 * quad(lambda x: 1 / np.sqrt(0.31 * (1 + x) ** 3 + 0.69), 0, zz)[0] / 1e3
+           for zz in z]
+    assert np.allclose(catalog.dL_Gpc(z), ref, rtol=1e-12)
+
+
+def test_bbh_draws_in_ranges():
+    ev = catalog.draw_bbh(5000, seed=0)
+    assert np.all((ev['eta'] > 0.055) & (ev['eta'] <= 0.25))
+    assert np.all((ev['Mc'] > 4.7) & (ev['Mc'] < 210.))
+    assert np.all((ev['dL'] > 0.45) & (ev['dL'] < 16.))
+    assert abs(np.mean(np.cos(ev['theta']))) < 0.05
+    a = catalog.draw_bbh(7, seed=3)
+    b = catalog.draw_bbh(7, seed=3)
+    assert all(np.array_equal(a[k], b[k]) for k in a)
+
+
+# ---------------------------------------------------- fit tables vs reference
+def _ref_fit_numbers():
+    src = open(REF).read().split('\n')[958:1333]
+    names = ['sigma1', 'sigma2', 'sigma3', 'sigma4', 'beta1', 'beta2', 'beta3', 'alpha1', 'alpha2', 'alpha3',
+             'alpha4', 'alpha5', 'gamma1', 'gamma2', 'gamma3', 'rho1', 'rho2', 'rho3', 'v2']
+    out = {}
+    for line in src:
+        s = line.strip()
+        for n in names:
+            if s.startswith(n + ' =') and n not in out:
+                toks = re.findall(r'([-+]?)\s*(?<![A-Za-z_\d.])((?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?)',
+                                  s.split('=', 1)[1])
+                out[n] = [(-1 if sg == '-' else 1) * float(m) for sg, m in toks]
+    return out
+
+
+@pytest.mark.skipif(not os.path.exists(REF), reason='reference source not present')
+def test_phenomd_fit_tables_match_reference_text():
+    """The 19 IMRPhenomD fits (11 numbers each) in the oracle and in the HIP
+    source equal the numbers in the reference's text (waveforms.py:1039-1227)."""
+    from oracle.waveforms import PHENOMD_FITS
+    ref = _ref_fit_numbers()
+    assert len(ref) == 19
+    for n, nums in ref.items():
+        assert list(PHENOMD_FITS[n]) == nums, n
+    hip = open(os.path.join(ROOT, 'gwfast_amd', 'csrc', 'gwf_waveforms.h')).read()
+    for n, nums in ref.items():
+        m = re.search(r'/\* %s \*/ \{([^}]*)\}' % n, hip)
+        assert m, n
+        vals = [float(x) for x in m.group(1).split(',')]
+        assert vals == nums, n

@@ -123,7 +123,9 @@ def test_waveform_methods(gpu, model):
     for meth in ('Ampl', 'tau_star'):
         g = getattr(wf, meth)(f, **ev)
         o = getattr(owf, meth)(f, ev)
-        assert np.max(np.abs(g - o) / np.abs(o)) <= 1e-11, meth
+        nz = o != 0
+        assert np.all(g[~nz] == 0), meth  # IMRPhenomD is cut at Mf = 0.2 (waveforms.py:1250)
+        assert np.max(np.abs(g[nz] - o[nz]) / np.abs(o[nz])) <= 1e-11, meth
     g = wf.Phi(f, **ev)
     o = owf.Phi(f, ev)
     assert np.max(np.abs(g - o)) <= 1e-12 * np.max(np.abs(o)) + 1e-9

@@ -1,0 +1,374 @@
+"""Pins of the CPU oracle (no GPU): reference-held known answers, textbook
+formulas, geometric identities and derivative cross-checks.  Each test is
+chosen so that a plausible transcription mistake in the oracle (dropped term,
+wrong sign, swapped operand) fails it.
+"""
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+from gwfast_amd import gwfastGlobals as glob
+from oracle import dual as D
+from oracle import detector as OD
+from oracle import waveforms as OW
+
+
+def gw170817_like():
+    # notebooks/new_features_tutorial.ipynb:72-84 (dL from Planck18 is irrelevant for fcut)
+    z = np.array([0.00980])
+    return {'Mc': np.array([1.1859]) * (1. + z), 'dL': np.array([0.04]), 'theta': np.array([np.pi / 2. + 0.408084]),
+            'phi': np.array([3.44616]), 'iota': np.array([2.545065595974997]), 'psi': np.array([0.]),
+            'tcoal': np.array([0.]), 'eta': np.array([0.24786618323504223]), 'Phicoal': np.array([0.]),
+            'chi1z': np.array([0.005136138323169717]), 'chi2z': np.array([0.003235146993487445]),
+            'Lambda1': np.array([368.17802383555687]), 'Lambda2': np.array([586.5487031450857])}
+
+
+# ------------------------------------------------------------ known answers
+def test_tf2_fcut_known_answers():
+    """new_features_tutorial.ipynb:115: 'Schwarzschild ... 1590.1 Hz',
+    'Kerr ... 3433.5 Hz' (printed with %.1f)."""
+    ev = gw170817_like()
+    assert '%.1f' % OW.TaylorF2(is_tidal=True).fcut(ev)[0] == '1590.1'
+    assert '%.1f' % OW.TaylorF2(is_tidal=True, which_ISCO='Kerr').fcut(ev)[0] == '3433.5'
+
+
+def test_et_optimal_location_known_answer():
+    """gwfast_tutorial.ipynb:662-671: DetNet({ETS}).optimal_location(0.) =
+    [0.8636426, 0.1643505]; the network version sums Fp^2 + Fc^2 over the
+    three arms at psi = 0 and minimises -sqrt of it from (1, 1) within bounds
+    (network.py:174-196)."""
+    d = glob.detectors['ETS']
+    det = OD.Detector(glob.PSD_ET, 'T', d['lat'], d['long'], d['xax'])
+
+    def negF(p):
+        th, ph = p
+        s = 0.
+        for i in range(3):
+            Fp, Fc = det.pattern_function(th, ph, 0., 0., rot=i * 60.)
+            s += Fp ** 2 + Fc ** 2
+        return -np.sqrt(s)
+
+    x = minimize(negF, [1., 1.], bounds=((0., np.pi), (0., 2. * np.pi))).x
+    assert np.allclose(x, [0.8636426, 0.1643505], atol=2e-6)
+
+
+# ------------------------------------------------------------ TaylorF2 phase
+def tf2_textbook_nonspinning(f, Mc, eta):
+    """3.5PN TaylorF2 phase of arXiv:0907.0700 eq. (3.18) with (3.3) coefficients,
+    written with canonical fractions, reference frequency at v = 1 (so the
+    v_lso logarithm becomes ln v) and the constant pi - pi/4 of the
+    reference's LAL convention."""
+    g = np.euler_gamma
+    M = Mc / eta ** 0.6 * glob.GMsun_over_c3
+    v = (np.pi * M * f) ** (1. / 3.)
+    a = [1.0, 0.0,
+         3715. / 756. + 55. / 9. * eta,
+         -16. * np.pi,
+         15293365. / 508032. + 27145. / 504. * eta + 3085. / 72. * eta ** 2,
+         np.pi * (38645. / 756. - 65. / 9. * eta) * (1. + 3. * np.log(v)),
+         (11583231236531. / 4694215680. - 640. / 3. * np.pi ** 2 - 6848. / 21. * g - 6848. / 21. * np.log(4. * v)
+          + (-15737765635. / 3048192. + 2255. / 12. * np.pi ** 2) * eta + 76055. / 1728. * eta ** 2
+          - 127825. / 1296. * eta ** 3),
+         np.pi * (77096675. / 254016. + 378515. / 1512. * eta - 74045. / 756. * eta ** 2)]
+    s = sum(a[k] * v ** k for k in range(8))
+    return 3. / (128. * eta * v ** 5) * s + np.pi - np.pi / 4.
+
+
+def test_tf2_phase_matches_textbook_nonspinning():
+    f = np.geomspace(2., 300., 50)[:, None]
+    ev = {'Mc': np.array([1.2, 10., 30.]), 'eta': np.array([0.2499, 0.2, 0.12]),
+          'chi1z': np.zeros(3), 'chi2z': np.zeros(3)}
+    ph = OW.TaylorF2().Phi(f, ev)
+    ref = tf2_textbook_nonspinning(f, ev['Mc'], ev['eta'])
+    assert np.max(np.abs(ph - ref) / np.abs(ref)) < 1e-13
+
+
+def test_tf2_spin_orbit_15pn():
+    """The aligned-spin 1.5PN coefficient is
+    113/3 delta chi_a + (113/3 - 76 eta/3) chi_s (arXiv:1107.1267 eq. 5.3,
+    delta = sqrt(1 - 4 eta)).  The spin-dependent part of the phase times
+    128 eta v^2 / 3 lies exactly in span{1, v, v^2, v^2 ln v, v^3, v^4}
+    (1.5 to 3.5PN), so a least-squares fit returns the v^0 coefficient."""
+    eta, chi1, chi2 = 0.2, 0.6, -0.3
+    wf = OW.TaylorF2()
+    Mc = 10.
+    M = Mc / eta ** 0.6 * glob.GMsun_over_c3
+    vs = np.linspace(0.05, 0.4, 25)
+    f = (vs ** 3 / (np.pi * M))[:, None]
+    ev1 = {'Mc': np.array([Mc]), 'eta': np.array([eta]), 'chi1z': np.array([chi1]), 'chi2z': np.array([chi2])}
+    ev0 = dict(ev1, chi1z=np.zeros(1), chi2z=np.zeros(1))
+    v = (np.pi * M * f[:, 0]) ** (1. / 3.)
+    d = (wf.Phi(f, ev1) - wf.Phi(f, ev0))[:, 0] * 128. * eta * v ** 2 / 3.
+    B = np.stack([np.ones_like(v), v, v ** 2, v ** 2 * np.log(v), v ** 3, v ** 4], axis=1)
+    coef = np.linalg.lstsq(B, d, rcond=None)[0]
+    chi_s, chi_a, delta = 0.5 * (chi1 + chi2), 0.5 * (chi1 - chi2), np.sqrt(1 - 4 * eta)
+    expect = 113. / 3. * delta * chi_a + (113. / 3. - 76. * eta / 3.) * chi_s
+    assert abs(coef[0] - expect) < 1e-8 * abs(expect)
+    # 2.5PN: the v^2 ln v coefficient is 3 x (minus the spin-orbit part of five)
+    so5 = -((732985. / 2268. - 24260. * eta / 81. - 340. * eta ** 2 / 9.) * chi_s
+            + (732985. / 2268. + 140. * eta / 9.) * delta * chi_a)
+    assert abs(coef[3] - 3 * so5) < 1e-6 * abs(so5)
+
+
+def test_tau_star_is_spa_time_of_phase():
+    """Stationary phase: with h ~ exp(i (2 pi f tc - Phi_GW)) (signal.py:483),
+    -dPhi_GW/df = 2 pi tau(f) up to terms beyond 3.5PN; at v ~ 0.05 the
+    mismatch is < 1e-6 relative for non-spinning TaylorF2."""
+    wf = OW.TaylorF2()
+    ev = {'Mc': np.array([5.]), 'eta': np.array([0.22]), 'chi1z': np.zeros(1), 'chi2z': np.zeros(1)}
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    f0 = (0.05 ** 3 / (np.pi * M))
+    h = f0 * 1e-5
+    f = np.array([f0 - h, f0 + h])[:, None] * np.ones((1, 1))
+    dphi = (wf.Phi(f[1:], ev) - wf.Phi(f[:1], ev)) / (2 * h)
+    tau = wf.tau_star(np.array([[f0[0]]]), ev)
+    assert abs(-dphi[0, 0] / (2 * np.pi * tau[0, 0]) - 1.) < 1e-6
+
+
+# ----------------------------------------------------------- IMRPhenomD
+def _pd_ev(n=4, seed=1):
+    from gwfast_amd import catalog
+    ev = catalog.draw_bbh(n, seed=seed)
+    return ev
+
+
+def test_phenomd_remnant_equal_mass_nonspinning():
+    """Equal-mass non-spinning remnant: a_f = 0.6864, E_rad = 0.0484 of M
+    (arXiv:1508.07250 fits, waveforms.py:1256-1297)."""
+    wf = OW.IMRPhenomD()
+    eta = np.array([0.2499999])
+    z = np.zeros(1)
+    assert abs(wf._finalspin(eta, z, z)[0] - 0.6864) < 5e-4
+    assert abs(wf._radiatednrg(eta, z, z)[0] - 0.0484) < 5e-4
+
+
+def test_phenomd_newtonian_amplitude_matches_taylorf2():
+    """At low frequency IMRPhenomD's amplitude tends to the Newtonian one, which
+    is TaylorF2's amplitude; the two differ only by the two Gpc conversions
+    (gwfastGlobals.py:68 vs 111, ~9e-8 relative)."""
+    ev = _pd_ev()
+    f = np.array([[1e-3]]) * np.ones((1, 4))
+    r = OW.IMRPhenomD().Ampl(f, ev) / OW.TaylorF2().Ampl(f, ev)
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    assert np.all(np.abs(r - 1.) < 1e-6 + 50 * (M * 1e-3) ** (2. / 3.))
+
+
+def _pd_region_edges(wf, ev):
+    M = ev['Mc'] / ev['eta'] ** 0.6
+    fring, fdamp = wf._ringdown(ev['eta'], ev['chi1z'], ev['chi2z'])
+    return M * glob.GMsun_over_c3, fring
+
+
+@pytest.mark.parametrize('seed', [1, 2])
+def test_phenomd_phase_c1_at_joins(seed):
+    """Phase is C1 at Mf = 0.018 and Mf = fring/2 (waveforms.py:1098-1129)."""
+    wf = OW.IMRPhenomD(fRef=10.)
+    ev = _pd_ev(4, seed)
+    MG, fring = _pd_region_edges(wf, ev)
+    for xj in (0.018 * np.ones(4), 0.5 * fring):
+        fj = xj / MG
+        eps = 1e-7
+        f = np.array([fj * (1 - 2 * eps), fj * (1 - eps), fj * (1 + eps), fj * (1 + 2 * eps)])
+        p = wf.Phi(f, ev)
+        left_slope = (p[1] - p[0]) / (f[1] - f[0])
+        right_slope = (p[3] - p[2]) / (f[3] - f[2])
+        jump = (p[2] - p[1]) - left_slope * (f[2] - f[1])
+        scale = np.abs(left_slope * fj)
+        assert np.all(np.abs(jump) < 1e-6 * scale)
+        assert np.all(np.abs(left_slope - right_slope) < 1e-4 * np.abs(left_slope))
+
+
+@pytest.mark.parametrize('seed', [1, 2, 3])
+def test_phenomd_amplitude_c1_at_joins(seed):
+    """Amplitude is C1 at Mf = 0.014 and at fpeak: the intermediate quartic
+    matches value and slope of the inspiral and merger-ringdown models there
+    (collocation, waveforms.py:1199-1245)."""
+    wf = OW.IMRPhenomD()
+    ev = _pd_ev(4, seed)
+    M = ev['Mc'] / ev['eta'] ** 0.6
+    MG = M * glob.GMsun_over_c3
+    fring, fdamp = wf._ringdown(ev['eta'], ev['chi1z'], ev['chi2z'])
+    eta = ev['eta']
+    xi = -1.0 + (0.5 * (ev['chi1z'] + ev['chi2z']) * (1.0 - eta * 76.0 / 113.0)
+                 + np.sqrt(1 - 4 * eta) * 0.5 * (ev['chi1z'] - ev['chi2z']))
+    g2 = OW._fit('gamma2', eta, eta * eta, xi)
+    g3 = OW._fit('gamma3', eta, eta * eta, xi)
+    fpeak = np.where(g2 >= 1.0, np.abs(fring - fdamp * g3 / g2),
+                     fring + fdamp * (-1.0 + np.sqrt(1.0 - g2 * g2)) * g3 / g2)
+    for xj in (0.014 * np.ones(4), fpeak):
+        fj = xj / MG
+        eps = 1e-7
+        f = np.array([fj * (1 - 2 * eps), fj * (1 - eps), fj * (1 + eps), fj * (1 + 2 * eps)])
+        a = wf.Ampl(f, ev) * f ** (7. / 6.)  # remove the smooth f^-7/6
+        ls = (a[1] - a[0]) / (f[1] - f[0])
+        rs = (a[3] - a[2]) / (f[3] - f[2])
+        assert np.all(np.abs((a[2] - a[1]) - ls * (f[2] - f[1])) < 1e-7 * np.abs(a[1]))
+        # one-sided difference quotients sit 1.5 eps from the join, so their
+        # difference carries a curvature term ~3 eps a'' f; a C1 defect (wrong
+        # d1/d2 or delta_i) shows up at O(0.1 |a|/f)
+        assert np.all(np.abs(ls - rs) < 1e-4 * np.abs(a[1] / fj))
+
+
+# ------------------------------------------------------------- projection
+def _tensor_pattern(lat, lon, xax_deg, zeta, ra, dec, t):
+    """F+^2 + Fx^2 from detector tensor D = (u u - v v)/2 contracted with the
+    TT polarisation tensors, arms at xax -/+ zeta/2 measured counter-clockwise
+    from local East (the angle convention of signal.py:114-116)."""
+    L = lon + 2 * np.pi * t
+    E_ = np.array([-np.sin(L), np.cos(L), 0.])
+    N_ = np.array([-np.sin(lat) * np.cos(L), -np.sin(lat) * np.sin(L), np.cos(lat)])
+    b = np.deg2rad(xax_deg)
+    # bisector at angle xax from East towards North; arms at -/+ zeta/2 about it
+    u = np.cos(b - zeta / 2) * E_ + np.sin(b - zeta / 2) * N_
+    v = np.cos(b + zeta / 2) * E_ + np.sin(b + zeta / 2) * N_
+    Dt = 0.5 * (np.outer(u, u) - np.outer(v, v))
+    n = np.array([np.cos(dec) * np.cos(ra), np.cos(dec) * np.sin(ra), np.sin(dec)])
+    m = np.cross(n, [0., 0., 1.])
+    m /= np.linalg.norm(m)
+    k = np.cross(n, m)
+    ep = np.outer(m, m) - np.outer(k, k)
+    ec = np.outer(m, k) + np.outer(k, m)
+    return np.sum(Dt * ep) ** 2 + np.sum(Dt * ec) ** 2
+
+
+@pytest.mark.parametrize('name', ['Virgo', 'ETS', 'CE1Id', 'H1'])
+def test_pattern_functions_vs_detector_tensor(name):
+    d = glob.detectors[name]
+    det = OD.Detector(glob.PSD_ET, d['shape'], d['lat'], d['long'], d['xax'])
+    zeta = det.angbtwArms
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        th, ph, t, psi = np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), rng.uniform(), rng.uniform(0, np.pi)
+        Fp, Fc = det.pattern_function(th, ph, t, psi)
+        ref = _tensor_pattern(det.det_lat_rad, det.det_long_rad, d['xax'], zeta, ph, 0.5 * np.pi - th, t)
+        assert abs(Fp ** 2 + Fc ** 2 - ref) < 1e-12
+
+
+def test_pattern_identities():
+    d = glob.detectors['ETS']
+    det = OD.Detector(glob.PSD_ET, 'T', d['lat'], d['long'], d['xax'])
+    th, ph, t = 1.1, 4.0, 0.3
+    s = [det.pattern_function(th, ph, t, 0.7, rot=r) for r in (0., 60., 120.)]
+    assert abs(sum(x[0] for x in s)) < 1e-15 and abs(sum(x[1] for x in s)) < 1e-15
+    Fp, Fc = det.pattern_function(th, ph, t, 0.7)
+    Fp2, Fc2 = det.pattern_function(th, ph, t, 0.7 + np.pi / 2)
+    assert abs(Fp + Fp2) < 1e-15 and abs(Fc + Fc2) < 1e-15
+    for psi in (0.1, 1.3, 2.9):
+        a, b = det.pattern_function(th, ph, t, psi)
+        assert abs(a * a + b * b - (Fp * Fp + Fc * Fc)) < 1e-15
+
+
+def test_delt_loc_unit_vectors():
+    d = glob.detectors['CE2NM']
+    det = OD.Detector(glob.PSD_CE, 'L', d['lat'], d['long'], d['xax'])
+    rng = np.random.default_rng(5)
+    for _ in range(10):
+        th, ph, t = np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi), rng.uniform()
+        dec = 0.5 * np.pi - th
+        n = np.array([np.cos(dec) * np.cos(ph), np.cos(dec) * np.sin(ph), np.sin(dec)])
+        L = det.det_long_rad + 2 * np.pi * t
+        r = np.array([np.cos(det.det_lat_rad) * np.cos(L), np.cos(det.det_lat_rad) * np.sin(L), np.sin(det.det_lat_rad)])
+        assert abs(det.delt_loc(th, ph, t) - (-glob.REarth * n @ r / glob.clight)) < 1e-17
+    # source at the detector's zenith: -R/c
+    th = 0.5 * np.pi - det.det_lat_rad
+    assert abs(det.delt_loc(th, det.det_long_rad, 0.) + glob.REarth / glob.clight) < 1e-15
+
+
+# ------------------------------------------------------------ derivatives
+def test_dual_vs_complex_step_taylorf2():
+    """TaylorF2 is holomorphic (waveforms.py:741): complex-step derivatives of
+    the phase and amplitude are exact to rounding."""
+    wf = OW.TaylorF2()
+    ev = {'Mc': np.array([3., 25.]), 'eta': np.array([0.24, 0.16]), 'dL': np.array([1., 2.]),
+          'chi1z': np.array([0.3, -0.5]), 'chi2z': np.array([-0.2, 0.7])}
+    f = np.geomspace(3., 200., 40)[:, None] * np.ones((1, 2))
+    keys = ['Mc', 'eta', 'chi1z', 'chi2z']
+    evd = dict(ev)
+    for k, key in enumerate(keys):
+        evd[key] = D.Dual.seed(ev[key], k, 4)
+    ph = wf.Phi(f, evd)
+    am = wf.Ampl(f, evd)
+    for k, key in enumerate(keys):
+        h = 1e-30
+        evc = {kk: v.astype(complex) for kk, v in ev.items()}
+        evc[key] = evc[key] + 1j * h
+        dph = np.imag(wf.Phi(f, evc)) / h
+        dam = np.imag(wf.Ampl(f, evc)) / h
+        assert np.max(np.abs(ph.d[k] - dph)) <= 1e-12 * np.max(np.abs(dph)) + 1e-300
+        assert np.max(np.abs(am.d[k] - dam)) <= 1e-12 * np.max(np.abs(dam)) + 1e-300
+
+
+def test_dual_vs_central_fd_phenomd():
+    wf = OW.IMRPhenomD(fRef=5.)
+    ev = _pd_ev(3, 6)
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    f = np.geomspace(0.002, 0.15, 60)[:, None] / M[None, :]
+    keys = ['Mc', 'eta', 'chi1z', 'chi2z']
+    evd = dict(ev)
+    for k, key in enumerate(keys):
+        evd[key] = D.Dual.seed(ev[key], k, 4)
+    ph = wf.Phi(f, evd)
+    am = wf.Ampl(f, evd)
+    for k, key in enumerate(keys):
+        h = 1e-7 * np.maximum(np.abs(ev[key]), 1e-2)
+        evp, evm = dict(ev), dict(ev)
+        evp[key] = ev[key] + h
+        evm[key] = ev[key] - h
+        dph = (wf.Phi(f, evp) - wf.Phi(f, evm)) / (2 * h)
+        dam = (wf.Ampl(f, evp) - wf.Ampl(f, evm)) / (2 * h)
+        assert np.max(np.abs(ph.d[k] - dph)) <= 2e-5 * np.max(np.abs(dph)), key
+        assert np.max(np.abs(am.d[k] - dam)) <= 2e-5 * np.max(np.abs(dam)), key
+
+
+@pytest.mark.parametrize('model,earth,shape', [('TF2', False, 'L'), ('PD', True, 'T'), ('PD', True, 'L')])
+def test_closed_form_rows_vs_duals(model, earth, shape):
+    """The reference's closed-form derivative rows (signal.py:1432-1577) equal
+    the dual-number derivatives of GWstrain, up to the phase-association
+    rounding of exp(i Psi) (|Psi| up to ~1e8 rad here)."""
+    wf = OW.TaylorF2() if model == 'TF2' else OW.IMRPhenomD()
+    name = 'Virgo' if shape == 'L' else 'ETS'
+    d = glob.detectors[name]
+    det = OD.Detector(glob.PSD_ET, shape, d['lat'], d['long'], d['xax'], useEarthMotion=earth)
+    from gwfast_amd import catalog
+    ev = catalog.draw_bbh(3, seed=8)
+    f, _ = det.grid(wf, ev, 200)
+    a = det.signal_derivatives(wf, ev, f, rot=60. if shape == 'T' else 0.)
+    b = det.full_dual_derivatives(wf, ev, f, rot=60. if shape == 'T' else 0.)
+    for row in range(a.shape[0]):
+        scale = np.max(np.abs(b[row]))
+        assert np.max(np.abs(a[row] - b[row])) <= 2e-7 * scale, row
+
+
+# ---------------------------------------------------------------- integrals
+def test_fisher_closed_forms_oracle():
+    """F[dL,dL] = SNR^2/dL^2, F[Phic,Phic] = SNR^2, F[dL,Phic] = 0
+    (gwfast_tutorial.ipynb:296-304 checks the first at 2.2e-16)."""
+    wf = OW.IMRPhenomD()
+    from gwfast_amd import catalog
+    ev = catalog.draw_bbh(3, seed=9)
+    d = glob.detectors['CE1Id']
+    det = OD.Detector(glob.PSD_CE, 'L', d['lat'], d['long'], d['xax'], useEarthMotion=True)
+    F = det.fisher(wf, ev, 300)
+    snr = det.snr(wf, ev, 300)
+    assert np.max(np.abs(F[2, 2] * ev['dL'] ** 2 / snr ** 2 - 1)) < 1e-12
+    assert np.max(np.abs(F[8, 8] / snr ** 2 - 1)) < 1e-12
+    assert np.max(np.abs(F[2, 8]) / np.sqrt(F[2, 2] * F[8, 8])) < 1e-12
+    assert np.all(np.linalg.eigvalsh(F.transpose(2, 0, 1) / np.sqrt(np.einsum('iin->ni', F)[:, :, None] *
+                                                                      np.einsum('iin->ni', F)[:, None, :])) > -1e-9)
+
+
+def test_geomspace_jax_grid():
+    g = OD.geomspace_jax(np.array([2., 3.]), np.array([100., 4000.]), 1000)
+    assert g.shape == (1000, 2)
+    assert np.allclose(g, np.geomspace([2., 3.], [100., 4000.], 1000), rtol=1e-13)
+    assert np.all(np.diff(g, axis=0) > 0)
+
+
+def test_jnp_interp_semantics():
+    xp = np.array([0., 1., 3.])
+    fp = np.array([1., 3., 7.])
+    x = D.Dual.seed(np.array([-1., 0., 0.5, 1., 2., 3., 4.]), 0, 1)
+    y = D.interp(x, xp, fp)
+    assert np.allclose(y.v, [1., 1., 2., 3., 5., 7., 7.])
+    # slope of the right-hand segment at a knot, of the last segment at the end, 0 outside
+    assert np.allclose(y.d[0], [0., 2., 2., 2., 2., 2., 0.])
