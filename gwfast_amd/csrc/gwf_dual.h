@@ -116,9 +116,11 @@ GWF_HD D<N> operator*(double a, const D<N>& b) {
 }
 template <int N>
 GWF_HD D<N> operator/(const D<N>& a, const D<N>& b) {
+  // value by a true division (the collocation and join formulas of the
+  // setup are ill-conditioned: a reciprocal-multiply would perturb them)
   D<N> r;
-  r.v = a.v / b.v;
   const double ib = 1.0 / b.v;
+  r.v = a.v / b.v;
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
   return r;
@@ -126,16 +128,18 @@ GWF_HD D<N> operator/(const D<N>& a, const D<N>& b) {
 template <int N>
 GWF_HD D<N> operator/(const D<N>& a, double b) {
   D<N> r;
+  const double ib = 1.0 / b;
   r.v = a.v / b;
 #pragma unroll
-  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] / b;
+  for (int k = 0; k < N; ++k) r.d[k] = a.d[k] * ib;
   return r;
 }
 template <int N>
 GWF_HD D<N> operator/(double a, const D<N>& b) {
   D<N> r;
+  const double ib = 1.0 / b.v;
   r.v = a / b.v;
-  const double s = -r.v / b.v;
+  const double s = -r.v * ib;
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = b.d[k] * s;
   return r;
@@ -182,6 +186,14 @@ GWF_HD D<N> dfabs(const D<N>& a) {
   const double s = (a.v > 0.0) ? 1.0 : ((a.v < 0.0) ? -1.0 : 0.0);
   return chain(a, fabs(a.v), s);
 }
+// x^(1/3) via cbrt; derivative (1/3) x^(1/3) / x
+GWF_HD double dcbrt(double x) { return cbrt(x); }
+template <int N>
+GWF_HD D<N> dcbrt(const D<N>& a) {
+  const double c = cbrt(a.v);
+  return chain(a, c, c / (3.0 * a.v));
+}
+
 template <int N>
 GWF_HD D<N> dpow(const D<N>& a, double c) {
   const double p = pow(a.v, c);

@@ -72,6 +72,35 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
         A.s2x = s2[k];
         A.c2x = c2[k];
       }
+      A.wgt = 1.0;
+    }
+    // network-sum arms of this site: eigenpairs of M = sum_k (s_k, c_k)^T (s_k, c_k)
+    double m11 = 0., m12 = 0., m22 = 0.;
+    for (int k = 0; k < narms; ++k) {
+      const ArmK& A = net.arm[na + k];
+      m11 += A.s2x * A.s2x;
+      m12 += A.s2x * A.c2x;
+      m22 += A.c2x * A.c2x;
+    }
+    if (narms == 1) {
+      net.sum_arm[net.n_sum_arms++] = net.arm[na];
+    } else {
+      // Jacobi rotation diagonalising the 2x2 symmetric M
+      double c = 1.0, s = 0.0;
+      if (m12 != 0.0) {
+        const double th = (m22 - m11) / (2.0 * m12);
+        const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+        c = 1.0 / std::sqrt(t * t + 1.0);
+        s = t * c;
+      }
+      const double l1 = m11 - (s / c) * m12 * (c != 0 ? 1.0 : 0.0);
+      const double l2 = m22 + (s / c) * m12 * (c != 0 ? 1.0 : 0.0);
+      ArmK A1, A2;
+      A1.site = A2.site = d;
+      A1.s2x = c; A1.c2x = -s; A1.wgt = l1;
+      A2.s2x = s; A2.c2x = c; A2.wgt = l2;
+      if (l1 != 0.0) net.sum_arm[net.n_sum_arms++] = A1;
+      if (l2 != 0.0) net.sum_arm[net.n_sum_arms++] = A2;
     }
     na += narms;
   }

@@ -17,6 +17,7 @@
 namespace gwf {
 
 constexpr int SLOT = NT_MAX + 1;  // doubles per dual slot in a record
+constexpr double INV_DAY = 1.0 / (3600. * 24.);
 
 // -------------------------------------------------------------- event record
 // [waveform slots: REC_SLOTS * SLOT][event constants][site constants]
@@ -53,13 +54,22 @@ struct SiteK {
 struct ArmK {
   int site;
   double s2x, c2x;  // sin/cos(2 (xax + rot)); a derived (null-stream) arm holds -(sum of two)
+  double wgt;       // Gram weight (1 for a physical arm)
 };
 
+// Two arm tables: the physical arms (per-arm outputs, SNR, strain) and the
+// arms the network Fisher sum runs over.  Every strain quantity of an arm is
+// linear in its (s2x, c2x), so the sum over the arms k of one site of
+// G(D_k) = G(s_k U + c_k V) equals lambda_1 G(D(q_1)) + lambda_2 G(D(q_2))
+// with (lambda_j, q_j) the eigenpairs of sum_k (s_k, c_k)^T (s_k, c_k): a
+// triangle's three arms reduce to two weighted arms.
 struct NetK {
   int n_sites, n_arms, res, any_earth;
+  int n_sum_arms, pad_;
   double fmin, fmax;
   SiteK site[GWF_MAX_DETECTORS];
   ArmK arm[GWF_MAX_ARMS];
+  ArmK sum_arm[2 * GWF_MAX_DETECTORS];
   PsdTab psd[GWF_MAX_DETECTORS];
 };
 
@@ -199,7 +209,7 @@ GWF_HD void site_geo(const SiteK& S, const double* sc, double tn, double ra, Geo
   double sL, cL;
   sincos_d(S.lon + 2. * PI * tn, &sL, &cL);
   g.dt = sc[SC_U1] * cL + sc[SC_U2] * sL + sc[SC_U3];
-  const double t = tn + g.dt / DAY;
+  const double t = tn + g.dt * INV_DAY;
   const double al = ra - S.lon - 2. * PI * t;
   double sa, ca;
   sincos_d(al, &sa, &ca);
@@ -226,7 +236,7 @@ GWF_HD void site_geo(const SiteK& S, const double* sc, double tn, double ra, Geo
 
 // t without location delay for a site (signal.py:444-453).
 GWF_HD double site_tn(int motion, double tc, double tau) {
-  if (motion == MOTION_EARTH) return tc - tau / (3600. * 24);
+  if (motion == MOTION_EARTH) return tc - tau * INV_DAY;
   if (motion == MOTION_FIXED) return tc;
   return 0.0;
 }
@@ -247,7 +257,7 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
     o.A = pd_amp_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
     o.Phi = pd_phi_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
   } else {
-    const D<NT> v = dpow(PI * MG * f, 1. / 3.);
+    const D<NT> v = dcbrt(PI * MG * f);
     o.Phi = tf2_phi_at<NT>(rec, SLOT, v);
     const double f13 = cbrt(f);
     o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
@@ -305,17 +315,17 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   // d t_noloc / d(Mc, eta, tc) [days per unit]
   double tn_mc = 0., tn_eta = 0., tn_tc = 0.;
   if (motion == MOTION_EARTH) {
-    tn_mc = -wp.tau.d[0] / DAY;
-    tn_eta = -wp.tau.d[1] / DAY;
+    tn_mc = -wp.tau.d[0] * INV_DAY;
+    tn_eta = -wp.tau.d[1] * INV_DAY;
     tn_tc = 1.0;
   } else if (motion == MOTION_FIXED) {
     tn_tc = 1.0;
   }
-  const double tfac = 1.0 + g.dt_t / DAY;  // d t / d t_noloc
+  const double tfac = 1.0 + g.dt_t * INV_DAY;  // d t / d t_noloc
   const double tw = 2. * PI;
   // d alpha / dX with alpha = ra - lon - 2 pi t
   const double al_mc = -tw * tn_mc * tfac, al_eta = -tw * tn_eta * tfac, al_tc = -tw * tn_tc * tfac;
-  const double al_th = -tw * g.dt_th / DAY, al_ph = 1.0 - tw * g.dt_ph / DAY;
+  const double al_th = -tw * g.dt_th * INV_DAY, al_ph = 1.0 - tw * g.dt_ph * INV_DAY;
 
   const double Av = wp.A.v * idL;
   const double Ap = Av * Fp * cp, Ac = Av * Fc * cc;
@@ -350,7 +360,8 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   // psi: dFp/dpsi = 2 Fc, dFc/dpsi = -2 Fp
   GWF_SET(6, Av * (2. * Fc) * cp, Av * (-2. * Fp) * cc, 0.0);
   // tcoal, per second
-  GWF_SET(7, Av * Fp_al * al_tc * cp / DAY, Av * Fc_al * al_tc * cc / DAY, twopif + twopif * g.dt_t * tn_tc / DAY);
+  GWF_SET(7, Av * Fp_al * al_tc * cp * INV_DAY, Av * Fc_al * al_tc * cc * INV_DAY,
+          twopif + twopif * g.dt_t * tn_tc * INV_DAY);
   // Phicoal
   GWF_SET(8, 0.0, 0.0, -1.0);
   // spins (and tidal parameters)
@@ -386,7 +397,7 @@ GWF_HD void gram_update(double* G, const double* re, const double* im, double w)
 
 // ------------------------------------------------------------ Fisher body
 // Accumulates the packed Fisher of event e over points lane, lane+nlanes, ...
-// arm_sel < 0: all arms; else only that network arm.
+// arm_sel < 0: the network sum (over net.sum_arm); else only that physical arm.
 template <int MODEL, int NT, int P>
 GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int lane, int nlanes, int arm_sel,
                          double* G) {
@@ -395,35 +406,40 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
   const int div = res - 1;
   const double a = log10(net.fmin);
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
-  const double tc = ev[EV_TC], ra = ev[EV_RA];
+  const bool sum_mode = arm_sel < 0;
+  const int narm = sum_mode ? net.n_sum_arms : 1;
   for (int i = lane; i < res; i += nlanes) {
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
+    // keep the per-event record in memory (scalar loads) instead of letting
+    // loop-invariant loads occupy vector registers for the whole loop
+    __asm__ volatile("" ::: "memory");
+#endif
     const double lin = grid_lin(a, b, i, div);
     const double f = exp10d(lin);
     const double wt = 4.0 * trap_w(a, b, i, div, f);
     const double lnf = lin * 2.302585092994045684;
     WfPoint<MODEL, NT> wp;
     wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
-    int arm = 0;
-    for (int s = 0; s < net.n_sites; ++s) {
+    const double tc = ev[EV_TC], ra = ev[EV_RA];
+    int cur_site = -1, cur_psd = -1;
+    double Sf = 1.0;
+    Geo g;
+    for (int k = 0; k < narm; ++k) {
+      const ArmK& arm = sum_mode ? net.sum_arm[k] : net.arm[arm_sel];
+      const int s = arm.site;
       const SiteK& S = net.site[s];
-      // arms of this site are contiguous in the arm table
-      int nA = 0;
-      while (arm + nA < net.n_arms && net.arm[arm + nA].site == s) ++nA;
-      bool any = arm_sel < 0 || (arm_sel >= arm && arm_sel < arm + nA);
-      if (any) {
-        const double* sc = rec + REC_SITE + s * SC_COUNT;
-        const double Sf = psd_at(net.psd[S.psd], f, lnf);
-        const double wS = wt / Sf;
-        Geo g;
-        site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
-        for (int k = 0; k < nA; ++k) {
-          if (arm_sel >= 0 && arm_sel != arm + k) continue;
-          double re[P], im[P], Ap, Ac;
-          arm_derivs<MODEL, NT, P>(wp, ev, sc, g, net.arm[arm + k], S.motion, f, re, im, &Ap, &Ac);
-          gram_update<P>(G, re, im, wS);
+      const double* sc = rec + REC_SITE + s * SC_COUNT;
+      if (s != cur_site) {
+        if (S.psd != cur_psd) {
+          Sf = psd_at(net.psd[S.psd], f, lnf);
+          cur_psd = S.psd;
         }
+        site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
+        cur_site = s;
       }
-      arm += nA;
+      double re[P], im[P], Ap, Ac;
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+      gram_update<P>(G, re, im, wt * arm.wgt / Sf);
     }
   }
 }

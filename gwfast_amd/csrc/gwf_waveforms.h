@@ -40,7 +40,7 @@ enum PdSlot {
   PD_FRING, PD_FDAMP, PD_FMRDJ, PD_T0, PD_FREF, PD_PHIREF,
   PD_AM23, PD_AM43, PD_AM53, PD_AM73, PD_AM83, PD_AM1, PD_AM2, PD_AM3,
   PD_DL0, PD_DL1, PD_DL2, PD_DL3, PD_DL4,
-  PD_G1, PD_G2, PD_G3, PD_FPEAKA, PD_OAMP, PD_FCUT,
+  PD_G1, PD_G2, PD_G3, PD_FPEAKA, PD_OAMP, PD_FCUT, PD_IETA,
   PD_COUNT
 };
 // TaylorF2 event record: duals of 4 (6 when tidal) tangents.
@@ -521,6 +521,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   put(PD_G1, gamma1); put(PD_G2, gamma2); put(PD_G3, gamma3); put(PD_FPEAKA, fpeakA); put(PD_OAMP, OAmp);
   // fcut = 0.2 / (Mc G / eta^(3/5)) (waveforms.py:1333)
   put(PD_FCUT, mkc<N>(0.2 / (Mc.v * GMSUN_C3 / pow(eta.v, 3. / 5.))));
+  put(PD_IETA, 1.0 / eta);
 }
 
 template <int N>
@@ -549,29 +550,30 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, bool apply_f
   typedef D<N> T;
   const double xv = x.v;
 #define GWF_L(slot) ldd<N>(rec + (slot) * stride)
-  const T eta = GWF_L(PD_ETA);
+  const T ieta = GWF_L(PD_IETA);
   T phis;
   if (xv < 0.018) {
-    const T lpx = dlog(PI * x);
-    const T x13 = dpow(x, 1. / 3.);
+    const T lpx3 = dlog(PI * x) * (1.0 / 3.0);
+    const T x13 = dcbrt(x);
     const T x23 = x13 * x13;
     const T ix13 = 1.0 / x13;
     const T ix23 = ix13 * ix13;
-    const T ix = 1.0 / x;
-    phis = GWF_L(PD_PIP) + GWF_L(PD_P23) * x23 + GWF_L(PD_P13) * x13 + GWF_L(PD_P13L) * x13 * lpx / 3. +
-           GWF_L(PD_PLOG) * lpx / 3. + GWF_L(PD_PM13) * ix13 + GWF_L(PD_PM23) * ix23 + GWF_L(PD_PM1) * ix +
+    const T ix = ix13 * ix23;
+    phis = GWF_L(PD_PIP) + GWF_L(PD_P23) * x23 + GWF_L(PD_P13) * x13 + GWF_L(PD_P13L) * x13 * lpx3 +
+           GWF_L(PD_PLOG) * lpx3 + GWF_L(PD_PM13) * ix13 + GWF_L(PD_PM23) * ix23 + GWF_L(PD_PM1) * ix +
            GWF_L(PD_PM53) * (ix * ix23) +
-           (GWF_L(PD_S1) * x + GWF_L(PD_S43) * (x * x13) + GWF_L(PD_S53) * (x * x23) + GWF_L(PD_S2) * x * x) / eta;
+           (GWF_L(PD_S1) * x + GWF_L(PD_S43) * (x * x13) + GWF_L(PD_S53) * (x * x23) + GWF_L(PD_S2) * x * x) * ieta;
   } else if (xv < rec[PD_FMRDJ * stride]) {
-    phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) / (3. * x * x * x) + GWF_L(PD_B2) * dlog(x)) / eta + GWF_L(PD_C1I) +
-           GWF_L(PD_C2I) * x;
+    const T ix = 1.0 / x;
+    phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) * (ix * ix * ix) * (1.0 / 3.0) + GWF_L(PD_B2) * dlog(x)) * ieta +
+           GWF_L(PD_C1I) + GWF_L(PD_C2I) * x;
   } else if (apply_fcut && !(xv < 0.2)) {
     phis = 0.0 * x;
   } else {
     const T x14 = dsqrt(dsqrt(x));
-    phis = (-(GWF_L(PD_A2) / x) + (4.0 / 3.0) * (GWF_L(PD_A3) * (x14 * x14 * x14)) + GWF_L(PD_A1) * x +
-            GWF_L(PD_A4) * datan((x - GWF_L(PD_A5) * GWF_L(PD_FRING)) / GWF_L(PD_FDAMP))) /
-               eta +
+    phis = (-(GWF_L(PD_A2) * (1.0 / x)) + (4.0 / 3.0) * (GWF_L(PD_A3) * (x14 * x14 * x14)) + GWF_L(PD_A1) * x +
+            GWF_L(PD_A4) * datan((x - GWF_L(PD_A5) * GWF_L(PD_FRING)) / GWF_L(PD_FDAMP))) *
+               ieta +
            GWF_L(PD_C1M) + GWF_L(PD_C2M) * x;
   }
   if (apply_fcut) {
@@ -587,8 +589,8 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, bool apply_f
   typedef D<N> T;
   const double xv = x.v;
   T amp;
+  const T x13 = dcbrt(x);
   if (xv < 0.014) {
-    const T x13 = dpow(x, 1. / 3.);
     const T x23 = x13 * x13;
     const T x43 = x23 * x23;
     const T x53 = x43 * x13;
@@ -612,7 +614,9 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, bool apply_f
     const T fdg3 = fdamp * g3;
     amp = dexp(-(x - fring) * g2 / fdg3) * (fdg3 * g1) / ((x - fring) * (x - fring) + fdg3 * fdg3);
   }
-  return ldd<N>(rec + PD_OAMP * stride) * dpow(x, -7. / 6.) * amp;
+  // x^(-7/6) = 1 / (x sqrt(x^(1/3)))
+  const T xm76 = 1.0 / (x * dsqrt(x13));
+  return ldd<N>(rec + PD_OAMP * stride) * xm76 * amp;
 }
 
 // Value-only IMRPhenomD amplitude (the SNR path): reads only .v of the record.
@@ -792,12 +796,12 @@ GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v) {
 // and eta as duals over (Mc, eta).
 template <class T>
 GWF_HD T tau_star(const T& MG, const T& eta, double f) {
-  const T v = dpow(PI * MG * f, 1. / 3.);
+  const T v = dcbrt(PI * MG * f);
   const T eta2 = eta * eta;
   const T v2 = v * v;
   const T v4 = v2 * v2;
   const T v8 = v4 * v4;
-  const T OverallFac = 5. / 256 * MG / (eta * v8);
+  const T OverallFac = (5. / 256) * MG / (eta * v8);
   const T t05 = 1. + (743. / 252. + 11. / 3. * eta) * v2 - 32. / 5. * PI * (v2 * v) +
                 (3058673. / 508032. + 5429. / 504. * eta + 617. / 72. * eta2) * v4 -
                 (7729. / 252. - 13. / 3. * eta) * PI * (v4 * v);

@@ -71,8 +71,12 @@ __global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const d
   setup_event<MODEL, NT>(w, net, par, ld, e, f0, rec + e * (int64_t)REC_DOUBLES);
 }
 
+#ifndef GWF_FISHER_MIN_BLOCKS
+#define GWF_FISHER_MIN_BLOCKS 1
+#endif
+
 template <int MODEL, int NT, int P>
-__global__ __launch_bounds__(256) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
+__global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
                                                 int arm_sel, double* __restrict__ F) {
   const int64_t e = wave_event();
   if (e >= n_ev) return;  // wave-uniform exit
