@@ -71,6 +71,34 @@ def cpu_baseline(n_sample=12, res=RES, seed=0):
                       'NumPy FP64 restatement of gwfast\'s algorithm, %.1f s' % (n_sample, res, dt)}
 
 
+def plain_path(net, rows, n, steps=3):
+    """The plain FP64 GPU path on the same workload: per detector the grid and
+    PSD (gwf_grid), per arm the materialised strain derivatives J (gwf_strain,
+    (11, N, N_f) complex128 in HBM) and the trapezoid Gram over J (gwf_gram),
+    summed over arms.  Returns (Fisher/s, Fisher tensor)."""
+    import torch
+    from gwfast_amd import signal as S
+    sigs = list(net.signals.values())
+    F = torch.zeros((11, 11, n), dtype=torch.float64, device=rows.device)
+    J = torch.empty((11, n, RES, 2), dtype=torch.float64, device=rows.device)
+
+    def once():
+        F.zero_()
+        for sig in sigs:
+            f, Sg = S.grid_device(sig, rows, RES)
+            for k in range(sig.n_arms):
+                S.strain_device_into(sig, rows, f, k, J)
+                S.gram_device(J, f, Sg, 11, out=F, accumulate=True)
+    once()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        once()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    return n / dt, F
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -79,6 +107,7 @@ def main():
     ap.add_argument('--n-events', type=int, default=N_EV)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample', type=int, default=512)
+    ap.add_argument('--no-plain', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (events shard with no '
@@ -139,6 +168,14 @@ def main():
                      'kernel': 'k_fisher<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
                      'note': 'FP64 dense peak; algorithmic flops = Gram 4 P^2 N_f N_ifo per Fisher'},
     }
+    if not args.no_plain:
+        pv, Fp = plain_path(net, rows, n)
+        Fp = Fp.cpu().numpy()
+        d = np.sqrt(np.abs(np.einsum('iin->in', Fp)))
+        err = float(np.max(np.abs(Fh - Fp) / (d[:, None, :] * d[None, :, :])))
+        out['plain_fp64'] = {'value': pv, 'unit': 'Fisher matr/s', 'speedup_fused_vs_plain': value / pv,
+                             'max_norm_err_fused_vs_plain': err,
+                             'path': 'gwf_grid + per-arm gwf_strain (J in HBM) + gwf_gram, FP64'}
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)

@@ -22,13 +22,15 @@ constexpr double INV_DAY = 1.0 / (3600. * 24.);
 // -------------------------------------------------------------- event record
 // [waveform slots: REC_SLOTS * SLOT][event constants][site constants]
 enum EvConst {
-  EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_COUNT
+  EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_CRA, EV_SRA, EV_COUNT
 };
 enum SiteConst {
   SC_U1 = 0, SC_U2, SC_U3, SC_U1T, SC_U2T, SC_U3T,
   SC_E1, SC_E2, SC_E3, SC_E4, SC_E5, SC_G1, SC_G2, SC_G3, SC_G4,
   SC_E1T, SC_E2T, SC_E3T, SC_E4T, SC_E5T, SC_G1T, SC_G2T, SC_G3T, SC_G4T,
-  SC_P1, SC_P2, SC_COUNT
+  SC_P1, SC_P2,
+  SC_CL0, SC_SL0,  // cos/sin(lon + 2 pi t0), t0 = tcoal (fixed or rotating Earth) or 0 (noMotion)
+  SC_COUNT
 };
 constexpr int REC_EV = REC_SLOTS * SLOT;
 constexpr int REC_SITE = REC_EV + EV_COUNT;
@@ -98,6 +100,52 @@ GWF_HD double trap_w(double a, double b, int i, int div, double fi) {
   return 0.5 * (fr - fl);
 }
 
+// Walks a geometric grid in chunks of nlanes consecutive points (point
+// base + lane in each lane) and yields each point's frequency and trapezoid
+// weight.  On the device the neighbours come from the adjacent lanes
+// (one exp10 per point instead of three); lane 0 takes the previous chunk's
+// last point, lane 63 the next chunk's first point.
+struct GridWalk {
+  double a, b;
+  int div, lane, nlanes;
+  double f_cur, f_next, lin_cur, lin_next, prev_last;
+  GWF_HD void init(double a_, double b_, int res, int lane_, int nlanes_) {
+    a = a_;
+    b = b_;
+    div = res - 1;
+    lane = lane_;
+    nlanes = nlanes_;
+    lin_cur = grid_lin(a, b, lane < res ? lane : 0, div);
+    f_cur = exp10d(lin_cur);
+    prev_last = 0.0;
+  }
+  // Call once per chunk with base = 0, nlanes, 2 nlanes, ... (all lanes);
+  // returns f, its trapezoid weight and ln f of point base + lane.
+  GWF_HD void step(int base, int res, double* f, double* w, double* lnf) {
+    const int i = base + lane;
+    const int inext = i + nlanes;
+    lin_next = grid_lin(a, b, inext < res ? inext : 0, div);
+    f_next = exp10d(lin_next);
+#if defined(__HIP_DEVICE_COMPILE__)
+    double fl = __shfl_up(f_cur, 1, 64);
+    double fr = __shfl_down(f_cur, 1, 64);
+    const double first_next = __shfl(f_next, 0, 64);
+    if (lane == 0) fl = prev_last;
+    if (lane == nlanes - 1) fr = first_next;
+    prev_last = __shfl(f_cur, nlanes - 1, 64);
+#else
+    const double fl = (i > 0) ? grid_f(a, b, i - 1, div) : 0.0;
+    const double fr = (i < div) ? grid_f(a, b, i + 1, div) : 0.0;
+#endif
+    const double fi = f_cur;
+    *f = fi;
+    *w = (div <= 0) ? 0.0 : 0.5 * (((i < div) ? fr : fi) - ((i > 0) ? fl : fi));
+    *lnf = lin_cur * 2.302585092994045684;
+    f_cur = f_next;
+    lin_cur = lin_next;
+  }
+};
+
 // jnp.interp(f, xp, fp, left=1, right=1) on the PSD table; lnf = log(f).
 GWF_HD double psd_at(const PsdTab& t, double f, double lnf) {
   const double* x = t.f;
@@ -143,6 +191,8 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   ev[EV_TC] = tc;
   ev[EV_PHIC] = par[8 * ld + e];
   ev[EV_RA] = phi;
+  ev[EV_CRA] = cos(phi);
+  ev[EV_SRA] = sin(phi);
   ev[EV_FCUT] = rec[(MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
   // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
   const double ra = phi, dec = 0.5 * PI - theta;
@@ -182,6 +232,9 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
     sc[SC_G4T] = 0.5 * S.s2lat * sd;
     sc[SC_P1] = S.sinzeta * c2p;
     sc[SC_P2] = S.sinzeta * s2p;
+    const double t0 = (S.motion == MOTION_NONE) ? 0.0 : tc;
+    sc[SC_CL0] = cos(S.lon + 2. * PI * t0);
+    sc[SC_SL0] = sin(S.lon + 2. * PI * t0);
   }
 }
 
@@ -203,16 +256,32 @@ GWF_HD void sincos_d(double x, double* s, double* c) {
 #endif
 }
 
+// Site geometry at one frequency.  The Earth-rotation angle
+// L = lon + 2 pi t_noloc is the per-event angle L0 = lon + 2 pi t0 rotated by
+// -delta, delta = 2 pi tau / 86400 (the same for every site, so one sincos
+// per frequency serves all sites: cd, sd = cos, sin delta); the pattern
+// angle alpha = ra - L - eps, eps = 2 pi Delta t / 86400 (|eps| < 1.6e-6), is
+// (ra - L) rotated by -eps with cos eps = 1 - eps^2/2, sin eps = eps - eps^3/6
+// (truncation < 1e-24).  Angle-addition rounding is at the 1e-16 level.
 template <bool WITH_DERIV>
-GWF_HD void site_geo(const SiteK& S, const double* sc, double tn, double ra, Geo& g) {
+GWF_HD void site_geo(const SiteK& S, const double* sc, const double* ev, double tn, double cdel, double sdel, Geo& g) {
   g.tn = tn;
-  double sL, cL;
-  sincos_d(S.lon + 2. * PI * tn, &sL, &cL);
+  double cL = sc[SC_CL0], sL = sc[SC_SL0];
+  if (S.motion == MOTION_EARTH) {
+    const double c0 = cL, s0 = sL;
+    cL = c0 * cdel + s0 * sdel;
+    sL = s0 * cdel - c0 * sdel;
+  }
   g.dt = sc[SC_U1] * cL + sc[SC_U2] * sL + sc[SC_U3];
-  const double t = tn + g.dt * INV_DAY;
-  const double al = ra - S.lon - 2. * PI * t;
-  double sa, ca;
-  sincos_d(al, &sa, &ca);
+  // beta = ra - L
+  const double cra = ev[EV_CRA], sra = ev[EV_SRA];
+  const double cb = cra * cL + sra * sL;
+  const double sb = sra * cL - cra * sL;
+  const double eps = (2. * PI * INV_DAY) * g.dt;
+  const double ce = 1.0 - 0.5 * eps * eps;
+  const double se = eps - eps * eps * eps * (1.0 / 6.0);
+  const double ca = cb * ce + sb * se;
+  const double sa = sb * ce - cb * se;
   const double c2a = ca * ca - sa * sa;
   const double s2a = 2. * sa * ca;
   g.Sa = sc[SC_E1] * c2a + sc[SC_E3] * ca + sc[SC_E5];
@@ -234,6 +303,16 @@ GWF_HD void site_geo(const SiteK& S, const double* sc, double tn, double ra, Geo
   }
 }
 
+// sincos of the site-independent Earth-rotation increment 2 pi tau / 86400.
+GWF_HD void earth_delta(bool any_earth, double tau, double* c, double* s) {
+  if (any_earth) {
+    sincos_d((2. * PI * INV_DAY) * tau, s, c);
+  } else {
+    *c = 1.0;
+    *s = 0.0;
+  }
+}
+
 // t without location delay for a site (signal.py:444-453).
 GWF_HD double site_tn(int motion, double tc, double tau) {
   if (motion == MOTION_EARTH) return tc - tau * INV_DAY;
@@ -249,26 +328,44 @@ struct WfPoint {
   D<2> tau;   // tau_star over (Mc, eta)
 };
 
+template <int NT>
+GWF_HD D<2> first2(const D<NT>& x) {
+  D<2> r;
+  r.v = x.v;
+  r.d[0] = x.d[0];
+  r.d[1] = x.d[1];
+  return r;
+}
+
+// Waveform quantities at one frequency.  The powers and logarithm of the
+// scaled frequency are computed once and shared by phase, amplitude and tau:
+// IMRPhenomD: x = M G f, tau's v = (pi x)^(1/3) = pi^(1/3) x^(1/3);
+// TaylorF2:   v = (pi M G f)^(1/3).
 template <int MODEL, int NT>
 GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o) {
-  const D<NT> MG = ldd<NT>(rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG) * SLOT);
+  const int smg = MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG;
+  const int seta = MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA;
+  const D<NT> MG = ldd<NT>(rec + smg * SLOT);
+  D<NT> v, lv;
   if (MODEL == GWF_MODEL_IMRPHENOMD) {
     const D<NT> x = MG * f;
-    o.A = pd_amp_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
-    o.Phi = pd_phi_at<NT>(rec, SLOT, x, w.apply_fcut != 0);
+    const D<NT> x13 = dcbrt(x);
+    const D<NT> lx = dlog(x);
+    o.A = pd_amp_at<NT>(rec, SLOT, x, x13, w.apply_fcut != 0);
+    o.Phi = pd_phi_at<NT>(rec, SLOT, x, x13, lx, w.apply_fcut != 0);
+    if (need_tau) {
+      v = x13 * cbrt(PI);
+      lv = (lx + log(PI)) * (1.0 / 3.0);
+    }
   } else {
-    const D<NT> v = dcbrt(PI * MG * f);
-    o.Phi = tf2_phi_at<NT>(rec, SLOT, v);
+    v = dcbrt(PI * MG * f);
+    lv = dlog(v);
+    o.Phi = tf2_phi_at<NT>(rec, SLOT, v, lv);
     const double f13 = cbrt(f);
     o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
   }
   if (need_tau) {
-    D<2> mg2, eta2;
-    const double* pm = rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG) * SLOT;
-    const double* pe = rec + (MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA) * SLOT;
-    mg2.v = pm[0]; mg2.d[0] = pm[1]; mg2.d[1] = pm[2];
-    eta2.v = pe[0]; eta2.d[0] = pe[1]; eta2.d[1] = pe[2];
-    o.tau = tau_star(mg2, eta2, f);
+    o.tau = tau_star_v(first2(MG), first2(ldd<NT>(rec + seta * SLOT)), first2(v), first2(lv));
   } else {
     o.tau = mkc<2>(0.0);
   }
@@ -399,28 +496,31 @@ GWF_HD void gram_update(double* G, const double* re, const double* im, double w)
 // Accumulates the packed Fisher of event e over points lane, lane+nlanes, ...
 // arm_sel < 0: the network sum (over net.sum_arm); else only that physical arm.
 template <int MODEL, int NT, int P>
-GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int lane, int nlanes, int arm_sel,
-                         double* G) {
+GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int res, int lane, int nlanes,
+                         int arm_sel, double* G) {
   const double* ev = rec + REC_EV;
-  const int res = net.res;
-  const int div = res - 1;
   const double a = log10(net.fmin);
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
   const bool sum_mode = arm_sel < 0;
   const int narm = sum_mode ? net.n_sum_arms : 1;
-  for (int i = lane; i < res; i += nlanes) {
+  GridWalk gw;
+  gw.init(a, b, res, lane, nlanes);
+  for (int base = 0; base < res; base += nlanes) {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
     // keep the per-event record in memory (scalar loads) instead of letting
     // loop-invariant loads occupy vector registers for the whole loop
     __asm__ volatile("" ::: "memory");
 #endif
-    const double lin = grid_lin(a, b, i, div);
-    const double f = exp10d(lin);
-    const double wt = 4.0 * trap_w(a, b, i, div, f);
-    const double lnf = lin * 2.302585092994045684;
+    const int i = base + lane;
+    double f, wt, lnf;
+    gw.step(base, res, &f, &wt, &lnf);
+    if (i >= res) continue;
+    wt *= 4.0;
     WfPoint<MODEL, NT> wp;
     wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
-    const double tc = ev[EV_TC], ra = ev[EV_RA];
+    const double tc = ev[EV_TC];
+    double cdel, sdel;
+    earth_delta(net.any_earth != 0, wp.tau.v, &cdel, &sdel);
     int cur_site = -1, cur_psd = -1;
     double Sf = 1.0;
     Geo g;
@@ -434,7 +534,7 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
           Sf = psd_at(net.psd[S.psd], f, lnf);
           cur_psd = S.psd;
         }
-        site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
+        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
         cur_site = s;
       }
       double re[P], im[P], Ap, Ac;
@@ -448,26 +548,28 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
 // Per-arm trapz((Ap^2 + Ac^2)/S, f) of site s; arms of the site accumulate
 // into acc[0..2].
 template <int MODEL>
-GWF_HD void snr_site(const gwf_waveform& w, const NetK& net, const double* rec, int s, int arm0, int nA, int lane,
-                     int nlanes, double acc[3]) {
+GWF_HD void snr_site(const gwf_waveform& w, const NetK& net, const double* rec, int res, int s, int arm0, int nA,
+                     int lane, int nlanes, double acc[3]) {
   const double* ev = rec + REC_EV;
   const double* sc = rec + REC_SITE + s * SC_COUNT;
   const SiteK& S = net.site[s];
-  const int res = net.res;
-  const int div = res - 1;
   const double a = log10(net.fmin);
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
-  const double tc = ev[EV_TC], ra = ev[EV_RA], idL = ev[EV_IDL], cp = ev[EV_CP], cc = ev[EV_CC];
+  const double tc = ev[EV_TC], idL = ev[EV_IDL], cp = ev[EV_CP], cc = ev[EV_CC];
   const double P1 = sc[SC_P1], P2 = sc[SC_P2];
-  for (int i = lane; i < res; i += nlanes) {
-    const double lin = grid_lin(a, b, i, div);
-    const double f = exp10d(lin);
-    const double wt = trap_w(a, b, i, div, f);
-    const double lnf = lin * 2.302585092994045684;
+  GridWalk gw;
+  gw.init(a, b, res, lane, nlanes);
+  for (int base = 0; base < res; base += nlanes) {
+    const int i = base + lane;
+    double f, wt, lnf;
+    gw.step(base, res, &f, &wt, &lnf);
+    if (i >= res) continue;
     const double Av = wf_amp_val<MODEL>(w, rec, f) * idL;
     const double tau = (S.motion == MOTION_EARTH) ? tau_val(rec, MODEL, f) : 0.0;
+    double cdel, sdel;
+    earth_delta(S.motion == MOTION_EARTH, tau, &cdel, &sdel);
     Geo g;
-    site_geo<false>(S, sc, site_tn(S.motion, tc, tau), ra, g);
+    site_geo<false>(S, sc, ev, site_tn(S.motion, tc, tau), cdel, sdel, g);
     const double wS = wt / psd_at(net.psd[S.psd], f, lnf);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -497,13 +599,15 @@ GWF_HD void strain_event(const gwf_waveform& w, const NetK& net, const double* r
   const int s = arm.site;
   const SiteK& S = net.site[s];
   const double* sc = rec + REC_SITE + s * SC_COUNT;
-  const double tc = ev[EV_TC], ra = ev[EV_RA], phic = ev[EV_PHIC];
+  const double tc = ev[EV_TC], phic = ev[EV_PHIC];
   for (int64_t i = lane; i < nf; i += nlanes) {
     const double fi = f[i * fs];
     WfPoint<MODEL, NT> wp;
     wf_point<MODEL, NT>(w, rec, fi, S.motion == MOTION_EARTH, wp);
+    double cdel, sdel;
+    earth_delta(S.motion == MOTION_EARTH, wp.tau.v, &cdel, &sdel);
     Geo g;
-    site_geo<true>(S, sc, site_tn(S.motion, tc, wp.tau.v), ra, g);
+    site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
     double re[P], im[P], Ap, Ac;
     arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, fi, re, im, &Ap, &Ac);
     const double psi = (2. * PI * fi * (tc * 3600. * 24.) - phic - wp.Phi.v) + 0. + (2. * PI * fi) * g.dt;

@@ -546,15 +546,14 @@ GWF_HD PdPhase<D<N>> pd_load_phase(const double* rec, int stride) {
 // Per-frequency IMRPhenomD phase Phi(f) as a dual (waveforms.py:1139-1147).
 // Only the coefficients of the branch that x falls in are read.
 template <int N>
-GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, bool apply_fcut) {
+GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, const D<N>& lx, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
 #define GWF_L(slot) ldd<N>(rec + (slot) * stride)
   const T ieta = GWF_L(PD_IETA);
   T phis;
   if (xv < 0.018) {
-    const T lpx3 = dlog(PI * x) * (1.0 / 3.0);
-    const T x13 = dcbrt(x);
+    const T lpx3 = (lx + log(PI)) * (1.0 / 3.0);
     const T x23 = x13 * x13;
     const T ix13 = 1.0 / x13;
     const T ix23 = ix13 * ix13;
@@ -565,7 +564,7 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, bool apply_f
            (GWF_L(PD_S1) * x + GWF_L(PD_S43) * (x * x13) + GWF_L(PD_S53) * (x * x23) + GWF_L(PD_S2) * x * x) * ieta;
   } else if (xv < rec[PD_FMRDJ * stride]) {
     const T ix = 1.0 / x;
-    phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) * (ix * ix * ix) * (1.0 / 3.0) + GWF_L(PD_B2) * dlog(x)) * ieta +
+    phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) * (ix * ix * ix) * (1.0 / 3.0) + GWF_L(PD_B2) * lx) * ieta +
            GWF_L(PD_C1I) + GWF_L(PD_C2I) * x;
   } else if (apply_fcut && !(xv < 0.2)) {
     phis = 0.0 * x;
@@ -585,11 +584,10 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, bool apply_f
 }
 
 template <int N>
-GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, bool apply_fcut) {
+GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
   T amp;
-  const T x13 = dcbrt(x);
   if (xv < 0.014) {
     const T x23 = x13 * x13;
     const T x43 = x23 * x23;
@@ -775,9 +773,8 @@ GWF_HD void tf2_setup(const gwf_waveform& w, const D<N>& Mc, const D<N>& eta, co
 
 // Per-frequency TaylorF2 phase.  v is (pi MG f)^(1/3) as a dual.
 template <int N>
-GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v) {
+GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v, const D<N>& lv) {
   typedef D<N> T;
-  const T lv = dlog(v);
   const T iv = 1.0 / v;
   const T iv2 = iv * iv;
   const T iv3 = iv2 * iv;
@@ -795,8 +792,7 @@ GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v) {
 // 3.5PN time to coalescence (waveforms.py:878-901 = 1299-1322); MG = Mtot*G
 // and eta as duals over (Mc, eta).
 template <class T>
-GWF_HD T tau_star(const T& MG, const T& eta, double f) {
-  const T v = dcbrt(PI * MG * f);
+GWF_HD T tau_star_v(const T& MG, const T& eta, const T& v, const T& lv) {
   const T eta2 = eta * eta;
   const T v2 = v * v;
   const T v4 = v2 * v2;
@@ -807,10 +803,16 @@ GWF_HD T tau_star(const T& MG, const T& eta, double f) {
                 (7729. / 252. - 13. / 3. * eta) * PI * (v4 * v);
   const T t6 = (-10052469856691. / 23471078400. + 128. / 3. * PI * PI + 6848. / 105. * EULER_GAMMA +
                 (3147553127. / 3048192. - 451. / 12. * PI * PI) * eta - 15211. / 1728. * eta2 + 25565. / 1296. * eta2 * eta +
-                3424. / 105. * dlog(16. * v2)) *
+                3424. / 105. * (log(16.) + 2.0 * lv)) *
                (v4 * v2);
   const T t7 = (-15419335. / 127008. - 75703. / 756. * eta + 14809. / 378. * eta2) * PI * (v4 * v2 * v);
   return OverallFac * (t05 + t6 + t7);
+}
+
+template <class T>
+GWF_HD T tau_star(const T& MG, const T& eta, double f) {
+  const T v = dcbrt(PI * MG * f);
+  return tau_star_v(MG, eta, v, dlog(v));
 }
 
 }  // namespace gwf

@@ -11,6 +11,7 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -75,17 +76,9 @@ __global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const d
 #define GWF_FISHER_MIN_BLOCKS 1
 #endif
 
-template <int MODEL, int NT, int P>
-__global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
-                                                int arm_sel, double* __restrict__ F) {
-  const int64_t e = wave_event();
-  if (e >= n_ev) return;  // wave-uniform exit
-  const int lane = threadIdx.x & 63;
-  constexpr int NG = P * (P + 1) / 2;
-  double G[NG];
-#pragma unroll
-  for (int k = 0; k < NG; ++k) G[k] = 0.0;
-  fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, lane, 64, arm_sel, G);
+// Wave-reduces the packed Gram and writes the symmetric (P, P, n_ev) output.
+template <int P>
+__device__ __forceinline__ void store_packed(const double* G, double* __restrict__ F, int64_t n_ev, int64_t e, int lane) {
   int k = 0;
 #pragma unroll
   for (int a = 0; a < P; ++a) {
@@ -101,9 +94,52 @@ __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_wavef
   }
 }
 
+template <int MODEL, int NT, int P>
+__global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
+                                                int arm_sel, double* __restrict__ F) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;  // wave-uniform exit
+  const int lane = threadIdx.x & 63;
+  constexpr int NG = P * (P + 1) / 2;
+  double G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) G[k] = 0.0;
+  fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, arm_sel, G);
+  store_packed<P>(G, F, n_ev, e, lane);
+}
+
+// Ragged grids: one persistent launch; each wavefront takes events from an
+// atomic counter (lane 0, vector-memory atomic) until the batch is exhausted,
+// so every wave reaches the exit; res_ev[e] is event e's grid length.
+template <int MODEL, int NT, int P>
+__global__ __launch_bounds__(256) void k_fisher_ragged(gwf_waveform w, NetK net, const double* __restrict__ rec,
+                                                       int64_t n_ev, const int* __restrict__ res_ev, int arm_sel,
+                                                       unsigned int* counter, double* __restrict__ F) {
+  const int lane = threadIdx.x & 63;
+  constexpr int NG = P * (P + 1) / 2;
+  for (;;) {
+    unsigned int t = 0;
+    if (lane == 0) t = atomicAdd(counter, 1u);
+    const int64_t e = (int64_t)__builtin_amdgcn_readfirstlane((int)__shfl(t, 0, 64));
+    if (e >= n_ev) break;
+    double G[NG];
+#pragma unroll
+    for (int k = 0; k < NG; ++k) G[k] = 0.0;
+    int res = res_ev[e];
+    res = res < 2 ? 2 : res;
+    fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, res, lane, 64, arm_sel, G);
+    store_packed<P>(G, F, n_ev, e, lane);
+  }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_snr(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
-                                             double* __restrict__ out) {
+                                             const int* __restrict__ res_ev, double* __restrict__ out) {
   const int64_t e = wave_event();
   if (e >= n_ev) return;
   const int lane = threadIdx.x & 63;
@@ -113,7 +149,7 @@ __global__ __launch_bounds__(256) void k_snr(gwf_waveform w, NetK net, const dou
     int nA = 0;
     while (arm + nA < net.n_arms && net.arm[arm + nA].site == s) ++nA;
     double acc[3] = {0.0, 0.0, 0.0};
-    snr_site<MODEL>(w, net, r, s, arm, nA, lane, 64, acc);
+    snr_site<MODEL>(w, net, r, res_ev ? (res_ev[e] < 2 ? 2 : res_ev[e]) : net.res, s, arm, nA, lane, 64, acc);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const double v = wave_sum(acc[k]);
@@ -489,7 +525,10 @@ int gwf_abi_version(void) { return GWF_ABI_VERSION; }
 
 const char* gwf_last_error(void) { return g_err.c_str(); }
 
-size_t gwf_workspace_bytes(int64_t n_ev) { return (size_t)(n_ev < 0 ? 0 : n_ev) * REC_DOUBLES * sizeof(double); }
+size_t gwf_workspace_bytes(int64_t n_ev) {
+  // event records + 256 bytes of counters (ragged launches)
+  return (size_t)(n_ev < 0 ? 0 : n_ev) * REC_DOUBLES * sizeof(double) + 256;
+}
 
 int gwf_num_arms(const gwf_detector* dets, int32_t n_det) {
   if (!dets || n_det < 0) return fail(-EINVAL, "invalid detector list");
@@ -568,8 +607,8 @@ int gwf_prepare(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det,
   return launch_setup(wf, net, params, ld, n_ev, nullptr, 0, (double*)workspace, (hipStream_t)stream);
 }
 
-int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
-                     double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream) {
+static int snr_prepared_impl(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                      const int32_t* res_ev, double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream) {
   int rc = check_wf(wf);
   if (rc) return rc;
   if (res < 2) return fail(-EINVAL, "res must be >= 2");
@@ -585,10 +624,15 @@ int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n
   const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
   const double* rec = (const double*)workspace;
   if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_snr<GWF_MODEL_IMRPHENOMD>), grid, block, 0, st, *wf, net, rec, n_ev, snrsq_out);
+    hipLaunchKernelGGL((k_snr<GWF_MODEL_IMRPHENOMD>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev, snrsq_out);
   else
-    hipLaunchKernelGGL((k_snr<GWF_MODEL_TAYLORF2>), grid, block, 0, st, *wf, net, rec, n_ev, snrsq_out);
+    hipLaunchKernelGGL((k_snr<GWF_MODEL_TAYLORF2>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev, snrsq_out);
   return hip_check(hipGetLastError(), "gwf_snr launch");
+}
+
+int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
+                     double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream) {
+  return snr_prepared_impl(wf, dets, n_det, n_ev, res, nullptr, snrsq_out, workspace, ws_bytes, stream);
 }
 
 int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
@@ -633,6 +677,49 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
   int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
   if (rc) return rc;
   return gwf_fisher_prepared(wf, dets, n_det, n_ev, res, arm, fisher_out, workspace, ws_bytes, stream);
+}
+
+int gwf_snr_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                   int64_t n_ev, const int32_t* res_ev, double* snrsq_out, void* workspace, size_t ws_bytes,
+                   void* stream) {
+  if (n_ev > 0 && !res_ev) return fail(-EINVAL, "res_ev is NULL");
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, 2, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  return snr_prepared_impl(wf, dets, n_det, n_ev, 2, res_ev, snrsq_out, workspace, ws_bytes, stream);
+}
+
+int gwf_fisher_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                      int64_t n_ev, const int32_t* res_ev, int32_t arm, double* fisher_out, void* workspace,
+                      size_t ws_bytes, void* stream) {
+  if (n_ev > 0 && !res_ev) return fail(-EINVAL, "res_ev is NULL");
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, 2, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  if (ws_bytes < gwf_workspace_bytes(n_ev)) return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+  NetK net;
+  rc = build_net_checked(dets, n_det, 2, net);
+  if (rc) return rc;
+  if (arm >= net.n_arms) return fail(-EINVAL, "arm index out of range");
+  if (n_ev == 0) return 0;
+  if (!fisher_out) return fail(-EINVAL, "fisher_out is NULL");
+  hipStream_t st = (hipStream_t)stream;
+  const double* rec = (const double*)workspace;
+  unsigned int* counter = (unsigned int*)((char*)workspace + (size_t)n_ev * REC_DOUBLES * sizeof(double));
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 1);
+  rc = hip_check(hipGetLastError(), "counter reset launch");
+  if (rc) return rc;
+  // persistent grid: enough waves to fill every SIMD several times, never more than the events
+  int64_t blocks = std::min<int64_t>((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK, 256 * 8);
+  const dim3 grid((unsigned)blocks), block(BLOCK);
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
+                       (int)arm, counter, fisher_out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
+                       (int)arm, counter, fisher_out);
+  else
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
+                       (int)arm, counter, fisher_out);
+  return hip_check(hipGetLastError(), "gwf_fisher_ragged launch");
 }
 
 int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,

@@ -44,7 +44,7 @@ class DetNet(object):
         for g in self._groups():
             sigs = [s for _, s in g]
             rows = sigs[0]._rows(evParams)
-            sq = S.snr_sq_device(sigs, rows, int(res)).cpu().numpy()
+            sq = S.snr_sq_device(sigs, rows, res).cpu().numpy()
             k = 0
             for name, s in g:
                 part = sq[k:k + s.n_arms]
@@ -64,7 +64,8 @@ class DetNet(object):
                    **kwargs):
         """network.py:84-152: sum of the detectors' Fisher matrices."""
         self._prepare(evParams)
-        res = int(kwargs.pop('res', 1000))
+        res = kwargs.pop('res', 1000)
+        res = int(res) if np.ndim(res) == 0 else np.asarray(res)
         S._check_fisher_options(res, kwargs.get('df'), kwargs.get('spacing', 'geom'), kwargs.get('use_m1m2', False),
                                 kwargs.get('use_chi1chi2', True), kwargs.get('computeDerivFinDiff', False))
         if return_derivatives or return_SNR_derivatives:

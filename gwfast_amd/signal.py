@@ -97,7 +97,7 @@ class GWSignal(object):
     def SNRInteg(self, evParams, res=1000, return_all=False):
         """signal.py:658-777."""
         self._complete(evParams)
-        snrsq = snr_sq_device([self], self._rows(evParams), int(res)).cpu().numpy()
+        snrsq = snr_sq_device([self], self._rows(evParams), res).cpu().numpy()
         if return_all:
             return 2 * np.sqrt(snrsq) if self.detector_shape == 'T' else 2 * np.sqrt(snrsq[0])
         return 2 * np.sqrt(snrsq.sum(axis=0))
@@ -112,10 +112,12 @@ class GWSignal(object):
         self._complete(evParams)
         rows = self._rows(evParams)
         if return_derivatives or return_SNR_derivatives:
+            if np.ndim(res) != 0:
+                raise NotImplementedError('derivative outputs need one grid length for all events')
             return self._fisher_with_derivs(evParams, rows, int(res), return_derivatives)
         if return_all:
-            return [fisher_device([self], self.wf_model, rows, int(res), arm=k).cpu().numpy() for k in range(self.n_arms)]
-        return fisher_device([self], self.wf_model, rows, int(res)).cpu().numpy()
+            return [fisher_device([self], self.wf_model, rows, res, arm=k).cpu().numpy() for k in range(self.n_arms)]
+        return fisher_device([self], self.wf_model, rows, res).cpu().numpy()
 
     def _fisher_with_derivs(self, evParams, rows, res, return_derivatives):
         n = rows.shape[1]
@@ -175,7 +177,7 @@ class GWSignal(object):
 
 
 def _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff):
-    if res is None:
+    if res is None or (np.ndim(res) == 0 and int(res) < 2):
         raise NotImplementedError('df-spaced grids are not supported: pass res')
     if spacing != 'geom':
         raise NotImplementedError("only spacing='geom' is supported")
@@ -191,17 +193,34 @@ def _dets(signals):
     return arr
 
 
+def _res_arg(res, n, device):
+    """int -> (int, None); per-event array -> (0, device int32 tensor)."""
+    if np.ndim(res) == 0:
+        return int(res), None
+    t = E.torch()
+    r = t.as_tensor(np.asarray(res), dtype=t.int32, device=device).contiguous()
+    if r.numel() != n:
+        raise ValueError('res must be an int or have one entry per event')
+    return 0, r
+
+
 def snr_sq_device(signals, rows, res):
     """Per-arm trapz((Ap^2+Ac^2)/S, f) for a list of GWSignal sharing one model,
-    fmin and fmax: (n_arms, N_ev) device tensor."""
+    fmin and fmax: (n_arms, N_ev) device tensor.  res: int, or one grid length
+    per event (ragged)."""
     t = E.torch()
     wf = signals[0].wf_model
     n = rows.shape[1]
     narms = sum(s.n_arms for s in signals)
     out = t.empty((narms, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    E.call('gwf_snr', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, res, N.ptr(out),
-           N.ptr(ws), nb, N.stream_ptr())
+    r, r_ev = _res_arg(res, n, rows.device)
+    if r_ev is None:
+        E.call('gwf_snr', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, r, N.ptr(out),
+               N.ptr(ws), nb, N.stream_ptr())
+    else:
+        E.call('gwf_snr_ragged', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n,
+               N.ptr(r_ev), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
 
@@ -213,8 +232,13 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None):
     if out is None:
         out = t.empty((P, P, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    E.call('gwf_fisher', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, res, arm,
-           N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    r, r_ev = _res_arg(res, n, rows.device)
+    if r_ev is None:
+        E.call('gwf_fisher', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, r, arm,
+               N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    else:
+        E.call('gwf_fisher_ragged', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n,
+               N.ptr(r_ev), arm, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
 
@@ -244,6 +268,16 @@ def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True)
     E.call('gwf_strain', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
            N.ptr(h), N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
     return h, J
+
+
+def strain_device_into(sig, rows, f, arm, J):
+    """Derivative rows of one arm into a caller buffer J (P, N_ev, N_f, 2)."""
+    n = rows.shape[1]
+    nf = f.shape[0]
+    ws, nb = E.workspace(n)
+    E.call('gwf_strain', sig.wf_model.c_struct(), ctypes_byref(sig.c_struct()), arm, N.ptr(rows), rows.stride(0), n,
+           N.ptr(f), nf, None, N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
+    return J
 
 
 def gram_device(J, f, S, P, out=None, accumulate=False):

@@ -125,6 +125,19 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
                int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes,
                void* stream);
 
+/* Ragged per-event grids (no reference counterpart: gwfast uses one res for
+ * all events, signal.py:890-898): event e uses jnp.geomspace(fmin, fcut_e,
+ * res_ev[e]) with res_ev a DEVICE int32 array (values < 2 are treated as 2).
+ * gwf_fisher_ragged runs one persistent launch whose wavefronts pull events
+ * from an atomic counter kept in the workspace, so uneven grids balance
+ * across the GPU.  Outputs as gwf_snr / gwf_fisher. */
+int gwf_snr_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                   int64_t n_ev, const int32_t* res_ev, double* snrsq_out, void* workspace, size_t ws_bytes,
+                   void* stream);
+int gwf_fisher_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,
+                      int64_t n_ev, const int32_t* res_ev, int32_t arm, double* fisher_out, void* workspace,
+                      size_t ws_bytes, void* stream);
+
 /* The two halves of gwf_snr / gwf_fisher, for callers that reuse the
  * per-event records (e.g. SNR then Fisher of the same events) or time the
  * integral kernels alone.  gwf_prepare fills the workspace with the event

@@ -44,7 +44,7 @@ def test_library_exports_every_declared_symbol(so_path):
 def test_library_binds_without_gpu(so_path):
     L = N.load(so_path)
     assert L.gwf_abi_version() == 1
-    assert L.gwf_workspace_bytes(10) > 0 and L.gwf_workspace_bytes(10) % 80 == 0
+    assert L.gwf_workspace_bytes(10) > L.gwf_workspace_bytes(0) > 0
     d = (N.GwfDetector * 3)()
     d[0].shape, d[1].shape, d[2].shape = N.SHAPE_T, N.SHAPE_L, N.SHAPE_L
     assert L.gwf_num_arms(d, 3) == 5

@@ -217,3 +217,23 @@ def test_full_size_properties(gpu):
     assert np.max(np.abs(F[2, 2] * dL ** 2 / snr ** 2 - 1.0)) <= 1e-10
     assert np.max(np.abs(F[8, 8] / snr ** 2 - 1.0)) <= 1e-10
     assert np.max(np.abs(F[2, 8]) / np.sqrt(F[2, 2] * F[8, 8])) <= 1e-12
+
+
+def test_ragged_grids_match_uniform(gpu):
+    """Per-event grid lengths in one persistent launch give, for every event,
+    the Fisher and SNR of a uniform launch at that event's length (same code
+    path per event: bitwise up to the reduction, <= 1e-15 normalised)."""
+    wf, _ = _models('TF2', is_tidal=True, which_ISCO='Kerr')
+    ev = bns(24, seed=31)
+    rng = np.random.default_rng(0)
+    res = rng.integers(500, 3000, size=24).astype(np.int32)
+    net = hip_net(wf, NET_C1, True)
+    F = net.FisherMatr(dict(ev), res=res)
+    snr = net.SNR(dict(ev), res=res)
+    for r in np.unique(res)[:6]:
+        sel = np.where(res == r)[0]
+        sub = {k: v[sel] for k, v in ev.items()}
+        Fu = net.FisherMatr(dict(sub), res=int(r))
+        su = net.SNR(dict(sub), res=int(r))
+        assert norm_err(F[:, :, sel], Fu) <= 1e-15
+        assert np.max(np.abs(snr[sel] - su) / su) <= 1e-15
