@@ -254,6 +254,21 @@ __global__ __launch_bounds__(256) void k_gram(const double* __restrict__ J, cons
   }
 }
 
+// VJP of the strain: one wave per (parameter, event) pair.
+__global__ __launch_bounds__(256) void k_vjp(const double* __restrict__ J, const double* __restrict__ gh, int P,
+                                             int64_t n_ev, int64_t nf, double tc_scale, double* __restrict__ g) {
+  const int64_t pe = wave_event();
+  if (pe >= (int64_t)P * n_ev) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t p = pe / n_ev, e = pe % n_ev;
+  const double* Jr = J + 2 * (p * n_ev + e) * nf;
+  double acc = 0.0;
+  for (int64_t i = lane; i < nf; i += 64)
+    acc += gh[2 * (i * n_ev + e)] * Jr[2 * i] + gh[2 * (i * n_ev + e) + 1] * Jr[2 * i + 1];
+  acc = wave_sum(acc);
+  if (lane == 0) g[p * n_ev + e] = (p == 7) ? acc * tc_scale : acc;
+}
+
 // ------------------------------------------------- covariance (per thread)
 constexpr int PMAX = 16;
 
@@ -748,6 +763,18 @@ int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, con
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
                        h_out, J_out);
   return hip_check(hipGetLastError(), "gwf_strain launch");
+}
+
+int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, int64_t n_f, double tcoal_scale,
+                   double* g_out, void* stream) {
+  if (P < 1 || P > 16 || n_ev < 0 || n_f < 0) return fail(-EINVAL, "invalid sizes");
+  if ((int64_t)P * n_ev == 0) return 0;
+  if (!J || !gh || !g_out) return fail(-EINVAL, "NULL array");
+  if ((int64_t)P * n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "too many events for one call");
+  const int64_t waves = (int64_t)P * n_ev;
+  hipLaunchKernelGGL(k_vjp, dim3((unsigned)((waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), dim3(BLOCK), 0,
+                     (hipStream_t)stream, J, gh, (int)P, n_ev, n_f, tcoal_scale, g_out);
+  return hip_check(hipGetLastError(), "gwf_strain_vjp launch");
 }
 
 int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,

@@ -161,6 +161,15 @@ int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, con
                int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
                size_t ws_bytes, void* stream);
 
+/* Vector-Jacobian product of the strain for reverse-mode callers (no
+ * reference counterpart; JAX does this inside jacrev, signal.py:1178-1187):
+ * g_out[p][e] = sum_f (gh[f][e].re J[p][e][f].re + gh[f][e].im J[p][e][f].im)
+ * for J (P, n_ev, n_f) complex128 as written by gwf_strain and gh (n_f, n_ev)
+ * complex128.  tcoal_scale multiplies row 7 (86400 gives d/dtcoal in days
+ * from the per-second row). g_out: (P, n_ev). */
+int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, int64_t n_f, double tcoal_scale,
+                   double* g_out, void* stream);
+
 /* The reference's plain Fisher integral from materialised derivatives
  * (signal.py:922-931): F_ab = 4 trapz(Re(conj(J_a) J_b)/S, f) for J
  * (P, n_ev, n_f) complex128, f and S (n_f, n_ev).  fisher_out (P, P, n_ev);
