@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, 'lib', 'libgwfast_hip.so')
 
 c_int32, c_int64, c_double, c_void_p, c_size_t = (ctypes.c_int32, ctypes.c_int64, ctypes.c_double,
                                                   ctypes.c_void_p, ctypes.c_size_t)
+c_float = ctypes.c_float
 
 MODEL_TAYLORF2 = 0
 MODEL_IMRPHENOMD = 1
@@ -72,6 +73,28 @@ SIGNATURES = {
                                       c_void_p]),
     'gwf_inversion_error': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
     'gwf_check_fisher': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    # learned SNR screen
+    'gwf_gemm_bf16': (ctypes.c_int, [c_int32, c_int32, c_int32, c_float, c_void_p, c_int64, c_int64, c_void_p,
+                                     c_int64, c_int64, c_float, c_void_p, c_int64, c_int32, c_void_p]),
+    'gwf_bias_silu': (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
+    'gwf_silu_backward': (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    'gwf_colsum': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int32, c_void_p, c_int32, c_void_p]),
+    'gwf_head_forward': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p]),
+    'gwf_head_backward': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p,
+                                         c_void_p]),
+    'gwf_mse_grad': (ctypes.c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
+    'gwf_adam': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float, c_float, c_float, c_float,
+                                c_int32, c_void_p, c_void_p]),
+    'gwf_to_bf16': (ctypes.c_int, [c_void_p, c_int64, c_void_p, c_void_p]),
+    'gwf_snr_features': (ctypes.c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    'gwf_snr_screen': (ctypes.c_int, [c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float,
+                                      c_float, c_void_p, c_void_p, c_void_p]),
+    'gwf_select_workspace_bytes': (c_size_t, [c_int64]),
+    'gwf_select': (ctypes.c_int, [c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'gwf_gather_events': (ctypes.c_int, [c_void_p, c_int64, c_int32, c_void_p, c_int64, c_void_p, c_int64,
+                                         c_void_p]),
 }
 
 _lib = None

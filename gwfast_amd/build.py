@@ -6,7 +6,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, 'csrc', 'gwfast_hip.hip')
+SRCS = [os.path.join(HERE, 'csrc', 'gwfast_hip.hip'), os.path.join(HERE, 'csrc', 'gwf_mlp.hip')]
 OUT_DIR = os.path.join(HERE, 'lib')
 OUT = os.path.join(OUT_DIR, 'libgwfast_hip.so')
 ARCH = os.environ.get('GWF_OFFLOAD_ARCH', 'gfx950')
@@ -24,7 +24,7 @@ def command(out=OUT):
             '-I', os.path.join(ROOT, 'include'), '-I', os.path.join(HERE, 'csrc'),
             # no packed-FP32 instructions anywhere in the device code
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-            SRC, '-o', out]
+            *SRCS, '-o', out]
 
 
 def build(verbose=True):

@@ -533,6 +533,11 @@ int check_common(const gwf_waveform* wf, const double* par, int64_t ld, int64_t 
 
 }  // namespace
 
+// the screen kernels (gwf_mlp.hip) report through the same gwf_last_error
+namespace gwf_mlp {
+void set_error(const std::string& s) { g_err = s; }
+}  // namespace gwf_mlp
+
 // ====================================================================== ABI
 extern "C" {
 

@@ -201,6 +201,82 @@ int gwf_inversion_error(const double* F, const double* C, int32_t P, int64_t n_e
 int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out, double* evecs_out, double* cond_out,
                      void* stream);
 
+/* ------------------------------------------------------------------------
+ * Learned SNR screen (no reference counterpart: gwfast evaluates every SNR
+ * with GWSignal.SNRInteg, signal.py:658-777).  An 11 -> 256 -> 256 -> 256 -> 1
+ * SiLU MLP with bf16 weights and activations and fp32 accumulation predicts
+ * the network log SNR; events whose prediction falls inside a band around
+ * the detection threshold are recomputed exactly.  bf16 arrays are passed
+ * as void* (raw 16-bit patterns, round-to-nearest-even).
+ * ---------------------------------------------------------------------- */
+
+/* C (M x N, row stride ldc, fp32) = alpha * sum_k A(m, k) B(k, n) + beta * C,
+ * A(m, k) = A[m*sam + k*sak], B(k, n) = B[k*sbk + n*sbn] (bf16), on MFMA.
+ * split_k > 1 splits K over workgroups and adds the partial products
+ * atomically (C pre-scaled by beta). */
+int gwf_gemm_bf16(int32_t M, int32_t N, int32_t K, float alpha, const void* A, int64_t sam, int64_t sak,
+                  const void* B, int64_t sbk, int64_t sbn, float beta, float* C, int64_t ldc, int32_t split_k,
+                  void* stream);
+
+/* Z = C + bias (per column), H = bf16(SiLU(Z)); C, Z (M, N) fp32, Z_out may be NULL. */
+int gwf_bias_silu(const float* C, const float* bias, int64_t M, int32_t N, float* Z_out, void* H_out, void* stream);
+
+/* dZ = dH * SiLU'(Z) over n elements; either output may be NULL (not both). */
+int gwf_silu_backward(const float* dH, const float* Z, int64_t n, float* dZ_out, void* dZ_bf16_out, void* stream);
+
+/* out[n] (+)= sum_m w[m] X[m, n] for X (M, N) given as fp32 X or bf16 X_bf16
+ * (exactly one non-NULL); w NULL means weight 1; accumulate = 0 overwrites. */
+int gwf_colsum(const float* X, const void* X_bf16, const float* w, int64_t M, int32_t N, float* out,
+               int32_t accumulate, void* stream);
+
+/* Output layer: y[m] = sum_k H[m, k] w[k] + b[0], H (M, 256) bf16, w fp32. */
+int gwf_head_forward(const void* H, const float* w, const float* b, int64_t M, float* y, void* stream);
+
+/* Output-layer backward: dH = g w^T (M, 256), dw = H^T g, db = sum g. */
+int gwf_head_backward(const void* H, const float* w, const float* g, int64_t M, float* dH_out, float* dw_out,
+                      float* db_out, void* stream);
+
+/* Mean squared error: g = 2 (y - t) / n and *loss_accum += mean((y - t)^2). */
+int gwf_mse_grad(const float* y, const float* t, int64_t n, float* g_out, float* loss_accum, void* stream);
+
+/* Adam step `step` (>= 1, bias-corrected) on fp32 master weights w with
+ * moments m, v; w_bf16_out (may be NULL) receives the bf16 copy. */
+int gwf_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float beta1, float beta2, float eps,
+             int32_t step, void* w_bf16_out, void* stream);
+
+/* y = bf16(x) over n elements. */
+int gwf_to_bf16(const float* x, int64_t n, void* y, void* stream);
+
+/* Screen features of events given as FP64 parameter rows (>= 11 rows of
+ * stride ld, Fisher order): log Mc, eta, log dL, cos theta,
+ * cos/sin(phi - 2 pi tcoal), cos iota, cos/sin 2psi, chi1z, chi2z.
+ * raw_out (n_ev, 11) fp32; X_out (n_ev, 16) bf16 standardised with
+ * (f - mean) * istd (mean, istd NULL: unstandardised), columns 11..15 zero. */
+int gwf_snr_features(const double* params, int64_t ld, int64_t n_ev, const float* mean, const float* istd,
+                     float* raw_out, void* X_out, void* stream);
+
+/* One-launch screen: features + MLP + decision.  W1 (256, 16), W2, W3
+ * (256, 256) bf16 row-major (out, in), 16-byte aligned; b1..b3 (256,),
+ * w4 (256,), b4 (1,) fp32.  log SNR = y * y_scale + y_shift;
+ * class_out: 0 if SNR < snr_lo, 2 if SNR > snr_hi, else 1 (recompute).
+ * Either output may be NULL (not both). */
+int gwf_snr_screen(const double* params, int64_t ld, int64_t n_ev, const float* mean, const float* istd,
+                   const void* W1, const float* b1, const void* W2, const float* b2, const void* W3, const float* b3,
+                   const float* w4, const float* b4, float y_scale, float y_shift, float snr_lo, float snr_hi,
+                   float* logsnr_out, int8_t* class_out, void* stream);
+
+/* Bytes of workspace gwf_select needs for n events. */
+size_t gwf_select_workspace_bytes(int64_t n);
+
+/* Indices (ascending) of the events with cls[i] == want -> idx_out (room for
+ * n), their number -> count_out[0] (device). */
+int gwf_select(const int8_t* cls, int64_t n, int32_t want, int64_t* idx_out, int64_t* count_out, void* workspace,
+               size_t workspace_bytes, void* stream);
+
+/* out[r * ld_out + j] = params[r * ld + idx[j]] for r < n_rows, j < n_idx. */
+int gwf_gather_events(const double* params, int64_t ld, int32_t n_rows, const int64_t* idx, int64_t n_idx,
+                      double* out, int64_t ld_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
