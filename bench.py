@@ -71,11 +71,12 @@ def cpu_baseline(n_sample=12, res=RES, seed=0):
                       'NumPy FP64 restatement of gwfast\'s algorithm, %.1f s' % (n_sample, res, dt)}
 
 
-def plain_path(net, rows, n, steps=3):
+def plain_path(net, rows, n, steps=3, mode='fp64'):
     """The plain FP64 GPU path on the same workload: per detector the grid and
     PSD (gwf_grid), per arm the materialised strain derivatives J (gwf_strain,
-    (11, N, N_f) complex128 in HBM) and the trapezoid Gram over J (gwf_gram),
-    summed over arms.  Returns (Fisher/s, Fisher tensor)."""
+    (11, N, N_f) complex128 in HBM) and the trapezoid Gram over J (gwf_gram;
+    mode 'split_fp32': gwf_gram_split, FP32 MFMA on hi/lo operands), summed
+    over arms.  Returns (Fisher/s, Fisher tensor)."""
     import torch
     from gwfast_amd import signal as S
     sigs = list(net.signals.values())
@@ -88,7 +89,7 @@ def plain_path(net, rows, n, steps=3):
             f, Sg = S.grid_device(sig, rows, RES)
             for k in range(sig.n_arms):
                 S.strain_device_into(sig, rows, f, k, J)
-                S.gram_device(J, f, Sg, 11, out=F, accumulate=True)
+                S.gram_device(J, f, Sg, 11, out=F, accumulate=True, mode=mode)
     once()
     torch.cuda.synchronize()
     t = time.perf_counter()
@@ -176,6 +177,13 @@ def main():
         out['plain_fp64'] = {'value': pv, 'unit': 'Fisher matr/s', 'speedup_fused_vs_plain': value / pv,
                              'max_norm_err_fused_vs_plain': err,
                              'path': 'gwf_grid + per-arm gwf_strain (J in HBM) + gwf_gram, FP64'}
+        sv, Fs = plain_path(net, rows, n, mode='split_fp32')
+        Fs = Fs.cpu().numpy()
+        out['plain_split_fp32_gram'] = {
+            'value': sv, 'unit': 'Fisher matr/s', 'speedup_vs_plain_fp64': sv / pv,
+            'max_norm_err_vs_plain_fp64': float(np.max(np.abs(Fs - Fp) / (d[:, None, :] * d[None, :, :]))),
+            'path': 'as plain_fp64 with gwf_gram_split (FP32 MFMA 16x16x4, hi/lo operands, FP64 reduction)'}
+        del Fs
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)

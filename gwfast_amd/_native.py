@@ -69,10 +69,15 @@ SIGNATURES = {
     'gwf_strain_vjp': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_int64, c_double, c_void_p, c_void_p]),
     'gwf_gram': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p,
                                 c_void_p]),
+    'gwf_gram_split': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p,
+                                      c_void_p]),
     'gwf_cov_invert': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_void_p]),
     'gwf_inversion_error': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
     'gwf_check_fisher': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'gwf_fix_params': (ctypes.c_int, [c_void_p, c_int32, c_int64, ctypes.c_uint32, c_void_p, c_void_p, c_void_p]),
+    'gwf_sigma_sky_area': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_int32, c_int32, c_double, c_int32,
+                                          c_void_p, c_void_p, c_void_p]),
     # learned SNR screen
     'gwf_gemm_bf16': (ctypes.c_int, [c_int32, c_int32, c_int32, c_float, c_void_p, c_int64, c_int64, c_void_p,
                                      c_int64, c_int64, c_float, c_void_p, c_int64, c_int32, c_void_p]),

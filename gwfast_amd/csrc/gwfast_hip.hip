@@ -254,6 +254,82 @@ __global__ __launch_bounds__(256) void k_gram(const double* __restrict__ J, cons
   }
 }
 
+// The same Fisher integral on FP32 MFMA (v_mfma_f32_16x16x4_f32) with split
+// operands and FP64 reduction.  Each weighted derivative x = J sqrt(w/S)
+// (FP64) is split into hi = fp32(x) and lo = fp32(x - hi), so x = hi + lo to
+// ~2^-48; F = hi^T hi + hi^T lo + (hi^T lo)^T (the lo^T lo term, ~2^-48
+// relative, is dropped).  One wave per event; per step the wave takes 4
+// frequencies, lane l holding parameter l%16 of frequency l/16, which is the
+// A (row l%16, k l/16) and B (k l/16, column l%16) operand layout of the
+// 16x16x4 MFMA, so A = B^T comes from the same register.  The real and
+// imaginary parts are two k-steps.  The fp32 accumulators (8 samples each)
+// are added into FP64 accumulators after every step, so the fp32 rounding
+// is bounded by 8 * 2^-24 of sum_s |x_a x_b| <= sqrt(F_aa F_bb).
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+
+template <int P>
+__global__ __launch_bounds__(256) void k_gram_split(const double* __restrict__ J, const double* __restrict__ f,
+                                                    const double* __restrict__ S, int64_t n_ev, int64_t nf,
+                                                    int accumulate, double* __restrict__ F) {
+  __shared__ double red[WAVES_PER_BLOCK][2][16 * 16];
+  const int64_t e = wave_event();
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const bool live = e < n_ev;  // whole wave uniform
+  const int a = lane & 15, q = lane >> 4;
+  double hh[4] = {0., 0., 0., 0.}, hl[4] = {0., 0., 0., 0.};
+  if (live) {
+    const double* Ja = J + 2 * ((int64_t)(a < P ? a : 0) * n_ev + e) * nf;
+    for (int64_t i0 = 0; i0 < nf; i0 += 4) {
+      const int64_t i = i0 + q;
+      float hr = 0.f, lr = 0.f, hi_ = 0.f, li = 0.f;
+      if (a < P && i < nf) {
+        const double fi = f[i * n_ev + e];
+        const double fl = i > 0 ? f[(i - 1) * n_ev + e] : fi;
+        const double fr = i < nf - 1 ? f[(i + 1) * n_ev + e] : fi;
+        const double sw = sqrt(4.0 * 0.5 * (fr - fl) / S[i * n_ev + e]);
+        const double xr = Ja[2 * i] * sw, xi = Ja[2 * i + 1] * sw;
+        hr = (float)xr;
+        lr = (float)(xr - (double)hr);
+        hi_ = (float)xi;
+        li = (float)(xi - (double)hi_);
+      }
+      f32x4_t c_hh = {0.f, 0.f, 0.f, 0.f}, c_hl = {0.f, 0.f, 0.f, 0.f};
+      c_hh = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, hr, c_hh, 0, 0, 0);
+      c_hl = __builtin_amdgcn_mfma_f32_16x16x4f32(hr, lr, c_hl, 0, 0, 0);
+      c_hh = __builtin_amdgcn_mfma_f32_16x16x4f32(hi_, hi_, c_hh, 0, 0, 0);
+      c_hl = __builtin_amdgcn_mfma_f32_16x16x4f32(hi_, li, c_hl, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        hh[r] += (double)c_hh[r];
+        hl[r] += (double)c_hl[r];
+      }
+    }
+    // lane l holds rows 4(l/16)..+3 of column l%16
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      red[wv][0][(4 * q + r) * 16 + a] = hh[r];
+      red[wv][1][(4 * q + r) * 16 + a] = hl[r];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  constexpr int NG = P * (P + 1) / 2;
+  for (int k = lane; k < NG; k += 64) {
+    // k -> (r, c), r <= c, row-major over the upper triangle
+    int r = 0, kk = k;
+    while (kk >= P - r) {
+      kk -= P - r;
+      ++r;
+    }
+    const int c = r + kk;
+    const double s = red[wv][0][r * 16 + c] + red[wv][1][r * 16 + c] + red[wv][1][c * 16 + r];
+    const int64_t i1 = (int64_t)(r * P + c) * n_ev + e, i2 = (int64_t)(c * P + r) * n_ev + e;
+    F[i1] = accumulate ? F[i1] + s : s;
+    if (r != c) F[i2] = accumulate ? F[i2] + s : s;
+  }
+}
+
 // VJP of the strain: one wave per (parameter, event) pair.
 __global__ __launch_bounds__(256) void k_vjp(const double* __restrict__ J, const double* __restrict__ gh, int P,
                                              int64_t n_ev, int64_t nf, double tc_scale, double* __restrict__ g) {
@@ -492,6 +568,40 @@ __global__ void k_inv_err(const double* __restrict__ F, const double* __restrict
       mx = (d > mx || isnan(d)) ? d : mx;
     }
   eps[e] = mx;
+}
+
+// ------------------------------------------------------ post-Fisher batch
+struct IdxMap {
+  int idx[PMAX];
+};
+
+// out (Pn, Pn, n_ev) = rows/columns map.idx[0..Pn) of F (P, P, n_ev);
+// diag (P,) optional: added to the kept diagonal first (addPrior)
+__global__ void k_sub(const double* __restrict__ F, int P, int64_t n_ev, IdxMap map, int Pn,
+                      const double* __restrict__ diag, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)Pn * Pn * n_ev) return;
+  const int64_t e = t % n_ev;
+  const int ab = (int)(t / n_ev);
+  const int a = map.idx[ab / Pn], b = map.idx[ab % Pn];
+  double v = F[(int64_t)(a * P + b) * n_ev + e];
+  if (diag && a == b) v += diag[a];
+  out[t] = v;
+}
+
+// sigma (P, n_ev) = sqrt(Cov_aa) and, if area != NULL, the sky area
+// scale * 2 pi sqrt(C_thth C_phph - C_phth^2) |sin theta|
+__global__ void k_sigma_area(const double* __restrict__ C, int P, int64_t n_ev, const double* __restrict__ theta,
+                             int ith, int iph, double scale, double* __restrict__ sigma, double* __restrict__ area) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  if (sigma)
+    for (int a = 0; a < P; ++a) sigma[(int64_t)a * n_ev + e] = sqrt(C[(int64_t)(a * P + a) * n_ev + e]);
+  if (area) {
+    const double tt = C[(int64_t)(ith * P + ith) * n_ev + e], pp = C[(int64_t)(iph * P + iph) * n_ev + e];
+    const double tp = C[(int64_t)(iph * P + ith) * n_ev + e];
+    area[e] = scale * 2.0 * M_PI * sqrt(tt * pp - tp * tp) * fabs(sin(theta[e]));
+  }
 }
 
 // ---------------------------------------------------------------- host side
@@ -798,6 +908,22 @@ int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64
   return hip_check(hipGetLastError(), "gwf_gram launch");
 }
 
+int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
+                   int32_t accumulate, double* fisher_out, void* stream) {
+  if (P != 11 && P != 13) return fail(-EINVAL, "gwf_gram_split supports P = 11 or 13");
+  if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
+  if (n_ev == 0) return 0;
+  if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
+  if (n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "too many events for one call");
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  hipStream_t st = (hipStream_t)stream;
+  if (P == 11)
+    hipLaunchKernelGGL((k_gram_split<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else
+    hipLaunchKernelGGL((k_gram_split<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  return hip_check(hipGetLastError(), "gwf_gram_split launch");
+}
+
 int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
                    int32_t* method_out, void* stream) {
   if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
@@ -828,6 +954,42 @@ int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out
   hipLaunchKernelGGL(k_check, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
                      evals_out, evecs_out, cond_out);
   return hip_check(hipGetLastError(), "gwf_check_fisher launch");
+}
+
+int gwf_fix_params(const double* F, int32_t P, int64_t n_ev, uint32_t fix_mask, const double* prior_diag,
+                   double* out, void* stream) {
+  if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (fix_mask >> P) return fail(-EINVAL, "fix_mask names a parameter >= P");
+  IdxMap map;
+  int Pn = 0;
+  for (int a = 0; a < P; ++a)
+    if (!((fix_mask >> a) & 1u)) map.idx[Pn++] = a;
+  for (int a = Pn; a < PMAX; ++a) map.idx[a] = 0;
+  if (Pn == 0) return fail(-EINVAL, "every parameter fixed");
+  if (n_ev == 0) return 0;
+  if (!F || !out) return fail(-EINVAL, "NULL array");
+  const int64_t tot = (int64_t)Pn * Pn * n_ev;
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, F, (int)P, n_ev,
+                     map, Pn, prior_diag, out);
+  return hip_check(hipGetLastError(), "gwf_fix_params launch");
+}
+
+int gwf_sigma_sky_area(const double* C, int32_t P, int64_t n_ev, const double* theta, int32_t i_theta, int32_t i_phi,
+                       double perc_level, int32_t sq_deg, double* sigma_out, double* area_out, void* stream) {
+  if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (!sigma_out && !area_out) return fail(-EINVAL, "no output requested");
+  if (area_out && (!theta || i_theta < 0 || i_theta >= P || i_phi < 0 || i_phi >= P))
+    return fail(-EINVAL, "sky area needs theta and the theta / phi indices");
+  if (area_out && !(perc_level > 0.0 && perc_level < 100.0)) return fail(-EINVAL, "perc_level must be in (0, 100)");
+  if (n_ev == 0) return 0;
+  if (!C) return fail(-EINVAL, "NULL array");
+  double scale = area_out ? -log(1.0 - perc_level / 100.0) : 0.0;
+  if (sq_deg) scale *= (180.0 / M_PI) * (180.0 / M_PI);
+  hipLaunchKernelGGL(k_sigma_area, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, C, (int)P,
+                     n_ev, theta, (int)i_theta, (int)i_phi, scale, sigma_out, area_out);
+  return hip_check(hipGetLastError(), "gwf_sigma_sky_area launch");
 }
 
 }  // extern "C"

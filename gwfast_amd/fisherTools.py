@@ -83,3 +83,69 @@ def CheckFisher(FisherM, condNumbMax=1.0e15, use_mpmath=True, verbose=False):
         else:
             print('Condition number= %s . Ok. ' % cond)
     return evals, evecs, cond
+
+
+# ------------------------------------------------------------ post-Fisher
+def fix_params_device(Fd, fix_idx, prior_diag=None):
+    """(Pn, Pn, N) device tensor: Fd with the rows / columns fix_idx removed
+    (and prior_diag (P,) added to the diagonal first)."""
+    t = E.torch()
+    P, n = Fd.shape[0], Fd.shape[2]
+    mask = 0
+    for i in fix_idx:
+        mask |= 1 << int(i)
+    Pn = P - bin(mask).count('1')
+    out = t.empty((Pn, Pn, n), dtype=t.float64, device=Fd.device)
+    pd = None if prior_diag is None else E.to_dev(np.asarray(prior_diag, dtype=np.float64))
+    E.call('gwf_fix_params', N.ptr(Fd), P, n, mask, N.ptr(pd), N.ptr(out), N.stream_ptr())
+    return out
+
+
+def fixParams(MatrIn, ParNums_inp, ParMarg):
+    """fisherTools.py:340-375: remove the rows and columns of the parameters
+    ParMarg; returns (Fisher (Pn, Pn, N), ParNums renumbered)."""
+    ParNums = dict(ParNums_inp)
+    idx = sorted(ParNums[p] for p in ParMarg)
+    out = fix_params_device(_dev(MatrIn), idx).cpu().numpy()
+    for pm in ParMarg:
+        for k in ParNums.keys():
+            if ParNums[k] > ParNums[pm]:
+                ParNums[k] -= 1
+        ParNums.pop(pm, None)
+    return out, ParNums
+
+
+def addPrior(Matr, vals, ParNums, ParAdd):
+    """fisherTools.py:378-403: Gaussian priors vals added on the diagonal
+    entries of the parameters ParAdd."""
+    M = np.asarray(Matr, dtype=np.float64)
+    two_d = M.ndim == 2
+    Fd = _dev(M[:, :, None] if two_d else M)
+    diag = np.zeros(Fd.shape[0])
+    diag[np.sort(np.array([ParNums[p] for p in ParAdd]))] = vals
+    out = fix_params_device(Fd, [], prior_diag=diag).cpu().numpy()
+    return out[:, :, 0] if two_d else out
+
+
+def sigma_sky_area_device(Cd, thetas, i_theta=3, i_phi=4, perc_level=90, units='SqDeg'):
+    """(sigma (P, N), sky area (N,)) device tensors from a device covariance."""
+    t = E.torch()
+    P, n = Cd.shape[0], Cd.shape[2]
+    sig = t.empty((P, n), dtype=t.float64, device=Cd.device)
+    area = t.empty(n, dtype=t.float64, device=Cd.device)
+    th = thetas if isinstance(thetas, t.Tensor) else E.to_dev(np.asarray(thetas, dtype=np.float64))
+    th = th.to(dtype=t.float64).contiguous()
+    E.call('gwf_sigma_sky_area', N.ptr(Cd), P, n, N.ptr(th), int(i_theta), int(i_phi), float(perc_level),
+           int(units == 'SqDeg'), N.ptr(sig), N.ptr(area), N.stream_ptr())
+    return sig, area
+
+
+def compute_localization_region(Cov, parNum, thFid, perc_level=90, units='SqDeg'):
+    """fisherTools.py:830-858 (Barack & Cutler 2004): the perc_level sky
+    area, in deg^2 ('SqDeg') or sr ('Sterad')."""
+    if units not in ('SqDeg', 'Sterad'):
+        return None
+    Cd = _dev(Cov)
+    th = np.broadcast_to(np.asarray(thFid, dtype=np.float64), (Cd.shape[2],))
+    _, area = sigma_sky_area_device(Cd, th, parNum['theta'], parNum['phi'], perc_level, units)
+    return area.cpu().numpy()

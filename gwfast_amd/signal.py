@@ -280,13 +280,19 @@ def strain_device_into(sig, rows, f, arm, J):
     return J
 
 
-def gram_device(J, f, S, P, out=None, accumulate=False):
+def gram_device(J, f, S, P, out=None, accumulate=False, mode='fp64'):
+    """Fisher integral of materialised derivatives J (P, n_ev, n_f, 2):
+    mode 'fp64' (FP64 VALU) or 'split_fp32' (FP32 MFMA on hi/lo split
+    operands with FP64 reduction)."""
     t = E.torch()
     n = J.shape[1]
     nf = J.shape[2]
+    if mode not in ('fp64', 'split_fp32'):
+        raise ValueError("mode must be 'fp64' or 'split_fp32'")
     if out is None:
         out = t.empty((P, P, n), dtype=t.float64, device=J.device)
-    E.call('gwf_gram', N.ptr(J), N.ptr(f), N.ptr(S), P, n, nf, int(accumulate), N.ptr(out), N.stream_ptr())
+    E.call('gwf_gram' if mode == 'fp64' else 'gwf_gram_split', N.ptr(J), N.ptr(f), N.ptr(S), P, n, nf,
+           int(accumulate), N.ptr(out), N.stream_ptr())
     return out
 
 

@@ -177,6 +177,13 @@ int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, i
 int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
              int32_t accumulate, double* fisher_out, void* stream);
 
+/* The same integral as gwf_gram (signal.py:922-931) on FP32 MFMA with split
+ * operands: x = J sqrt(w/S) is split into fp32 hi + lo, F = hi^T hi +
+ * hi^T lo + lo^T hi, each 8-sample fp32 partial added into FP64.  Error
+ * bound per element: ~8 * 2^-24 * sqrt(F_aa F_bb). */
+int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
+                   int32_t accumulate, double* fisher_out, void* stream);
+
 /* fisherTools.CovMatr(F, invMethodIn='cho', alt_method='svd') and
  * compute_inversion_error (fisherTools.py:32-210): per event, Jacobi scaling
  * ws = diag(1/sqrt(F_ii)), eigenvalues of the scaled matrix, Cholesky
@@ -276,6 +283,26 @@ int gwf_select(const int8_t* cls, int64_t n, int32_t want, int64_t* idx_out, int
 /* out[r * ld_out + j] = params[r * ld + idx[j]] for r < n_rows, j < n_idx. */
 int gwf_gather_events(const double* params, int64_t ld, int32_t n_rows, const int64_t* idx, int64_t n_idx,
                       double* out, int64_t ld_out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Post-Fisher batch (the catalogue driver's consumers of CovMatr,
+ * calculate_forecasts_from_catalog.py:573-630, 775).
+ * ---------------------------------------------------------------------- */
+
+/* fisherTools.fixParams (fisherTools.py:340-375) and addPrior (378-403):
+ * out = F with the rows / columns whose bit is set in fix_mask removed,
+ * (Pn, Pn, n_ev), Pn = P - popcount(fix_mask), kept order.  prior_diag
+ * (device, (P,), may be NULL) is added to the diagonal first. */
+int gwf_fix_params(const double* F, int32_t P, int64_t n_ev, uint32_t fix_mask, const double* prior_diag,
+                   double* out, void* stream);
+
+/* sigma_out (P, n_ev) = sqrt(diag Cov) (calculate_forecasts_from_catalog.py:775)
+ * and area_out (n_ev,) = compute_localization_region (fisherTools.py:830-858):
+ * -ln(1 - perc/100) 2 pi sqrt(C_thth C_phph - C_phth^2) |sin theta|,
+ * in deg^2 when sq_deg != 0, else sr.  theta: (n_ev,) device.  Either
+ * output may be NULL (not both). */
+int gwf_sigma_sky_area(const double* C, int32_t P, int64_t n_ev, const double* theta, int32_t i_theta, int32_t i_phi,
+                       double perc_level, int32_t sq_deg, double* sigma_out, double* area_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -83,3 +83,34 @@ def check_fisher(F):
         evecs[k] = np.array(ER.tolist(), dtype=float)
     cond = np.abs(evals).max(axis=1) / np.abs(evals).min(axis=1)
     return evals, evecs, cond
+
+
+def fix_params(F, par_nums, par_marg):
+    """fisherTools.py:340-375: delete the rows/columns of par_marg, renumber."""
+    idx = np.sort(np.array([par_nums[p] for p in par_marg], dtype=int))
+    out = np.delete(np.delete(np.asarray(F), idx, axis=0), idx, axis=1)
+    new = dict(par_nums)
+    for pm in par_marg:
+        for k in new.keys():
+            if new[k] > new[pm]:
+                new[k] -= 1
+        new.pop(pm, None)
+    return out, new
+
+
+def add_prior(F, vals, par_nums, par_add):
+    """fisherTools.py:378-403."""
+    d = np.zeros(F.shape[0])
+    d[np.sort(np.array([par_nums[p] for p in par_add]))] = vals
+    pp = np.diag(d)
+    return pp + F if F.ndim == 2 else pp[:, :, None] + F
+
+
+def localization_region(C, par_nums, theta, perc_level=90, units='SqDeg'):
+    """fisherTools.py:830-858 (Barack & Cutler, PRD 69, 082005)."""
+    tt = C[par_nums['theta'], par_nums['theta']]
+    pp = C[par_nums['phi'], par_nums['phi']]
+    tp = C[par_nums['phi'], par_nums['theta']]
+    base = 2 * np.pi * np.sqrt(tt * pp - tp ** 2) * np.abs(np.sin(theta))
+    sr = -base * np.log(1 - perc_level / 100)
+    return sr if units == 'Sterad' else (180 / np.pi) ** 2 * sr

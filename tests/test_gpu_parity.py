@@ -237,3 +237,28 @@ def test_ragged_grids_match_uniform(gpu):
         su = net.SNR(dict(sub), res=int(r))
         assert norm_err(F[:, :, sel], Fu) <= 1e-15
         assert np.max(np.abs(snr[sel] - su) / su) <= 1e-15
+
+
+# ------------------------------------------------------------ post-Fisher
+def test_post_fisher_batch_vs_oracle(gpu):
+    """fixParams / addPrior (bit-exact gathers), sigma and the 90% sky area
+    against the oracle on a real network Fisher."""
+    from gwfast_amd import fisherTools as FT
+    from gwfast_amd.waveforms import IMRPhenomD
+    from oracle import covariance as OC
+    ev = bbh(16, seed=71)
+    wf = IMRPhenomD()
+    net = hip_net(wf, NET_C1, True)
+    F = net.FisherMatr(dict(ev), res=1000)
+    Fr, nums = FT.fixParams(F, wf.ParNums, ['Phicoal', 'psi'])
+    Fo, nums_o = OC.fix_params(F, wf.ParNums, ['Phicoal', 'psi'])
+    assert nums == nums_o and np.array_equal(Fr, Fo)
+    Fp = FT.addPrior(F, [1.0, 4.0], wf.ParNums, ['chi2z', 'chi1z'])
+    assert np.array_equal(Fp, OC.add_prior(F, [1.0, 4.0], wf.ParNums, ['chi2z', 'chi1z']))
+    C, _ = FT.CovMatr(Fr)
+    area = FT.compute_localization_region(C, nums, ev['theta'])
+    area_o = OC.localization_region(C, nums, ev['theta'])
+    assert np.max(np.abs(area / area_o - 1)) < 1e-13
+    sig, _ = FT.sigma_sky_area_device(FT._dev(C), ev['theta'], nums['theta'], nums['phi'])
+    assert np.allclose(sig.cpu().numpy(), np.sqrt(np.einsum('iin->in', C)), rtol=1e-15, atol=0)
+    assert FT.compute_localization_region(C, nums, ev['theta'], units='bogus') is None

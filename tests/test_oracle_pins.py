@@ -372,3 +372,53 @@ def test_jnp_interp_semantics():
     assert np.allclose(y.v, [1., 1., 2., 3., 5., 7., 7.])
     # slope of the right-hand segment at a knot, of the last segment at the end, 0 outside
     assert np.allclose(y.d[0], [0., 2., 2., 2., 2., 2., 0.])
+
+
+# ------------------------------------------------------------ post-Fisher
+PARNUMS11 = {'Mc': 0, 'eta': 1, 'dL': 2, 'theta': 3, 'phi': 4, 'iota': 5, 'psi': 6, 'tcoal': 7, 'Phicoal': 8,
+             'chi1z': 9, 'chi2z': 10}
+
+
+def _spd(n, P, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(0, 1, (n, P, P))
+    F = np.einsum('nij,nkj->ikn', A, A) + 0.5 * np.eye(P)[:, :, None]
+    return F
+
+
+def test_fix_params_is_conditional_covariance():
+    """Fixing parameters = conditioning: inv(F_kept) = C_kk - C_kf C_ff^-1 C_fk."""
+    from oracle.covariance import fix_params
+    F = _spd(5, 11, 1)
+    Fr, nums = fix_params(F, PARNUMS11, ['psi', 'Phicoal', 'tcoal'])
+    keep = [i for i in range(11) if i not in (6, 7, 8)]
+    assert nums == {'Mc': 0, 'eta': 1, 'dL': 2, 'theta': 3, 'phi': 4, 'iota': 5, 'chi1z': 6, 'chi2z': 7}
+    for e in range(5):
+        C = np.linalg.inv(F[:, :, e])
+        f = [6, 7, 8]
+        cond = C[np.ix_(keep, keep)] - C[np.ix_(keep, f)] @ np.linalg.solve(C[np.ix_(f, f)], C[np.ix_(f, keep)])
+        assert np.allclose(np.linalg.inv(Fr[:, :, e]), cond, rtol=1e-9, atol=1e-12)
+
+
+def test_add_prior_adds_diagonal_only():
+    """Values land on the SORTED parameter indices (fisherTools.py:395-398
+    sorts the indices, not the values)."""
+    from oracle.covariance import add_prior
+    F = _spd(3, 11, 2)
+    G = add_prior(F, [1.0, 2.0], PARNUMS11, ['iota', 'eta'])
+    D = G - F
+    assert np.allclose(D[1, 1], 1.0) and np.allclose(D[5, 5], 2.0)
+    D[1, 1] = D[5, 5] = 0
+    assert np.all(D == 0)
+
+
+def test_localization_region_closed_form():
+    """Uncorrelated sky errors: area = -ln(1-p) 2 pi s_th s_ph |sin th|."""
+    from oracle.covariance import localization_region
+    C = np.zeros((11, 11, 2))
+    C[3, 3] = [1e-4, 4e-6]
+    C[4, 4] = [9e-4, 1e-6]
+    th = np.array([0.3, 2.0])
+    a = localization_region(C, PARNUMS11, th, 90, 'Sterad')
+    assert np.allclose(a, np.log(10) * 2 * np.pi * np.sqrt(C[3, 3] * C[4, 4]) * np.abs(np.sin(th)), rtol=1e-14)
+    assert np.allclose(localization_region(C, PARNUMS11, th), a * (180 / np.pi) ** 2, rtol=1e-14)
