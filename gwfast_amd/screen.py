@@ -14,7 +14,7 @@ Every network SNR is also available exactly from its Fisher matrix:
 dh/d dL = -h/dL, so F[dL, dL] = SNR^2 / dL^2 and SNR = dL sqrt(F[dL, dL]).
 Events the screen sends to the Fisher step above the band are therefore
 re-checked against the threshold at no extra cost; the only error the
-screen can make is a false negative (a true SNR >= threshold predicted
+screen can make is a false negative (a true SNR > threshold predicted
 below the band), which ``audit`` counts against the exact SNR of every
 event.
 
@@ -344,7 +344,8 @@ def snr_from_fisher(F, rows):
 
 
 def screened_fisher(net, screen, rows, threshold=12.0, band=(8.0, 16.0), res=2000):
-    """Fisher matrices of the events with network SNR >= threshold, screened.
+    """Fisher matrices of the events with network SNR > threshold (the
+    catalogue driver's cut, calculate_forecasts_from_catalog.py:474), screened.
 
     Returns a dict: ``idx`` (ascending event indices), ``F`` (P, P, len(idx)),
     ``snr`` (exact, from F), class counts, the number of exact SNR
@@ -354,7 +355,7 @@ def screened_fisher(net, screen, rows, threshold=12.0, band=(8.0, 16.0), res=200
     band_idx = select(cls, 1)
     above_idx = select(cls, 2)
     snr_band = net.snr_device(gather(rows, band_idx), res) if band_idx.numel() else band_idx.to(T.float64)
-    cand = T.cat([above_idx, band_idx[snr_band >= threshold]]).sort().values
+    cand = T.cat([above_idx, band_idx[snr_band > threshold]]).sort().values
     n_above, n_band = int(above_idx.numel()), int(band_idx.numel())
     counts = {'n_below': int(cls.numel()) - n_above - n_band, 'n_band': n_band, 'n_above': n_above,
               'n_exact_snr': n_band, 'n_fisher': int(cand.numel())}
@@ -365,29 +366,29 @@ def screened_fisher(net, screen, rows, threshold=12.0, band=(8.0, 16.0), res=200
     cand_rows = gather(rows, cand)
     F = net.fisher_device(cand_rows, res)
     snr = snr_from_fisher(F, cand_rows)
-    keep = snr >= threshold
+    keep = snr > threshold
     return dict(counts, idx=cand[keep], F=F[:, :, keep], snr=snr[keep],
                 n_dropped_after_fisher=int((~keep).sum().item()))
 
 
 def plain_fisher(net, rows, threshold=12.0, res=2000):
     """The same result without the screen: exact SNR of every event, then
-    the Fisher matrices of those at or above the threshold."""
+    the Fisher matrices of those above the threshold."""
     snr = net.snr_device(rows, res)
-    idx = (snr >= threshold).nonzero().flatten()
+    idx = (snr > threshold).nonzero().flatten()
     F = net.fisher_device(gather(rows, idx), res)
     return {'idx': idx, 'F': F, 'snr': snr[idx], 'snr_all': snr, 'n_exact_snr': int(snr.numel()),
             'n_fisher': int(idx.numel())}
 
 
 def audit(screen, rows, snr_exact, threshold=12.0, band=(8.0, 16.0)):
-    """False negatives (exact SNR >= threshold, predicted below the band),
+    """False negatives (exact SNR > threshold, predicted below the band),
     false positives sent to Fisher (exact < threshold, predicted above the
     band), and the RMSE of the predicted log SNR."""
     T = torch()
     ls = screen.predict_log_snr(rows).to(T.float64)
     cls = screen.classify(rows, band[0], band[1])
-    above = snr_exact >= threshold
+    above = snr_exact > threshold
     fn = int((above & (cls == 0)).sum().item())
     fp = int((~above & (cls == 2)).sum().item())
     d = ls - snr_exact.to(T.float64).log()

@@ -262,3 +262,41 @@ def test_post_fisher_batch_vs_oracle(gpu):
     sig, _ = FT.sigma_sky_area_device(FT._dev(C), ev['theta'], nums['theta'], nums['phi'])
     assert np.allclose(sig.cpu().numpy(), np.sqrt(np.einsum('iin->in', C)), rtol=1e-15, atol=0)
     assert FT.compute_localization_region(C, nums, ev['theta'], units='bogus') is None
+
+
+def test_catalogue_forecast_files_and_resume(gpu, tmp_path):
+    """The driver's chain and file set (calculate_forecasts_from_catalog.py:
+    222-247, 421-630, 738-741) on 300 events in 3 chunks, against the same
+    chain called directly."""
+    from gwfast_amd import forecast as FC, fisherTools as FT
+    from gwfast_amd.waveforms import IMRPhenomD
+    ev = bbh(300, seed=81)
+    ev['dL'] = ev['dL'] * 4.0   # push part of the catalogue below SNR 12
+    wf = IMRPhenomD()
+    net = hip_net(wf, NET_C1, True)
+    suffs = FC.run_catalog(ev, net, str(tmp_path), batch_size=100, params_fix=['Phicoal'], res=500, verbose=False)
+    assert suffs == ['_0_to_100', '_100_to_200', '_200_to_300']
+    snr = net.SNR(dict(ev), res=500)
+    det = np.nonzero(snr > 12.0)[0]
+    assert 0 < len(det) < 300
+    have = [s for s in suffs if (tmp_path / ('fishers%s.npy' % s)).exists()]
+    got_idx = np.concatenate([np.atleast_1d(np.loadtxt(tmp_path / ('idxs_det%s.txt' % s))) for s in have])
+    assert np.array_equal(got_idx.astype(int), det)
+    snrs = np.concatenate([np.loadtxt(tmp_path / ('snrs%s.txt' % s)) for s in suffs])
+    assert np.allclose(snrs, snr, rtol=1e-14, atol=0)
+    F = np.concatenate([np.load(tmp_path / ('fishers%s.npy' % s)) for s in have], axis=2)
+    Fr, nums = FT.fixParams(net.FisherMatr({k: v[det] for k, v in ev.items()}, res=500), wf.ParNums, ['Phicoal'])
+    assert F.shape == (10, 10, len(det)) and np.array_equal(F, Fr)
+    C = np.concatenate([np.load(tmp_path / ('covs%s.npy' % s)) for s in have], axis=2)
+    Cr, eps = FT.CovMatr(Fr)
+    assert np.array_equal(C, Cr)
+    err = np.concatenate([np.loadtxt(tmp_path / ('errors%s.txt' % s)).reshape(10, -1) for s in have], axis=1)
+    assert np.allclose(err, np.sqrt(np.einsum('iin->in', Cr)), rtol=1e-15)
+    area = np.concatenate([np.atleast_1d(np.loadtxt(tmp_path / ('sky_area%s.txt' % s))) for s in have])
+    assert np.allclose(area, FT.compute_localization_region(Cr, nums, ev['theta'][det]), rtol=1e-14)
+    # resume: an existing chunk is skipped (its file is left untouched)
+    p = tmp_path / 'snrs_100_to_200.txt'
+    p.write_text('sentinel\n')
+    FC.run_catalog(ev, net, str(tmp_path), batch_size=100, params_fix=['Phicoal'], res=500, resume=True,
+                   verbose=False)
+    assert p.read_text() == 'sentinel\n'
