@@ -100,6 +100,62 @@ def plain_path(net, rows, n, steps=3, mode='fp64'):
     return n / dt, F
 
 
+def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, threshold=12.0, band=(8.0, 16.0)):
+    """The learned SNR screen on a 10^6-event catalogue of the same network:
+    train the MLP on exact SNRs of an independent catalogue (seed 1001), then
+    screen the catalogue (seed 7) at SNR 12 with the [8, 16] recompute band and
+    compute the confirmed events' Fisher matrices; compare with the plain
+    pipeline (exact SNR of every event, Fisher of those above 12)."""
+    import torch
+    from gwfast_amd import screen as SC
+    t0 = time.perf_counter()
+    rows_t, snr_t = SC.training_set(net, n_train, seed=1001, res=res)
+    vrows, vsnr = SC.training_set(net, 50_000, seed=1002, res=res)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t0
+    sc = SC.SNRScreen(seed=7)
+    hist = sc.train(rows_t, snr_t, epochs=epochs, batch=4096, lr=2e-3, seed=3)
+    del rows_t, snr_t
+    rows, _ = SC.training_set(net, 1000, seed=7, res=res)    # warm-up of both pipelines
+    SC.screened_fisher(net, sc, rows, threshold, band, res)
+    SC.plain_fisher(net, rows, threshold, res)
+    from gwfast_amd import catalog, _engine as E
+    rows = E.to_dev(E.param_rows(catalog.draw_bbh(n_cat, seed=7), False))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    a = SC.screened_fisher(net, sc, rows, threshold, band, res)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    b = SC.plain_fisher(net, rows, threshold, res)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    aud = SC.audit(sc, rows, b["snr_all"], threshold, band)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    sc.classify(rows, band[0], band[1])
+    ev1.record()
+    torch.cuda.synchronize()
+    t_mlp = ev0.elapsed_time(ev1) * 1e-3
+    same = bool(torch.equal(a['idx'], b['idx']) and torch.equal(a['F'], b['F']))
+    ts, tp = t2 - t1, t3 - t2
+    mlp_tflops = SC.FLOP_PER_INFERENCE * n_cat / t_mlp / 1e12
+    return {'n_catalogue': n_cat, 'threshold': threshold, 'band': list(band),
+            'train': {'n_events': n_train, 'epochs': epochs, 'steps': hist['steps'], 'time_s': hist['time_s'],
+                      'snr_generation_s': t_gen, 'final_loss': hist['loss'][-1]},
+            'rmse_log_snr': aud['rmse_log_snr'], 'max_abs_err_log_snr': aud['max_abs_err_log_snr'],
+            'false_negatives': aud['false_negatives'], 'false_positives_to_fisher': aud['false_positives_to_fisher'],
+            'n_true_above': aud['n_true_above'], 'n_below_band': aud['n_below_band'], 'n_band': aud['n_band'],
+            'n_above_band': aud['n_above_band'],
+            'screened': {'time_s': ts, 'n_exact_snr': a['n_exact_snr'], 'n_fisher': a['n_fisher'],
+                         'confirmed_fisher_per_s': int(a['idx'].numel()) / ts},
+            'plain': {'time_s': tp, 'n_exact_snr': b['n_exact_snr'], 'n_fisher': b['n_fisher'],
+                      'confirmed_fisher_per_s': int(b['idx'].numel()) / tp},
+            'gain_end_to_end': tp / ts, 'fisher_identical_to_plain': same,
+            'mlp_inference': {'kernel_s': t_mlp, 'events_per_s': n_cat / t_mlp, 'bf16_tflops': mlp_tflops,
+                              'frac_of_bf16_dense_peak': mlp_tflops / 2500.0}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -109,6 +165,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-sample', type=int, default=512)
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
+    ap.add_argument('--copy-calibration-mib', type=int, default=0,
+                    help='first run a device copy of this many MiB (a known-size kernel for PMC byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (events shard with no '
@@ -120,6 +179,13 @@ def main():
     from gwfast_amd import signal as S
 
     torch.cuda.set_device(0)
+    if args.copy_calibration_mib:
+        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     wf, net = build_net()
     n = args.n_events
     ev = catalog.draw_bbh(n, seed=0)
@@ -184,6 +250,9 @@ def main():
             'max_norm_err_vs_plain_fp64': float(np.max(np.abs(Fs - Fp) / (d[:, None, :] * d[None, :, :]))),
             'path': 'as plain_fp64 with gwf_gram_split (FP32 MFMA 16x16x4, hi/lo operands, FP64 reduction)'}
         del Fs
+    if not args.no_screen:
+        del F, ws
+        out['screen_1e6'] = screen_leg(net)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)

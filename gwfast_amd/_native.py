@@ -19,6 +19,7 @@ c_float = ctypes.c_float
 
 MODEL_TAYLORF2 = 0
 MODEL_IMRPHENOMD = 1
+MODEL_IMRPHENOMD_NRTIDALV2 = 2
 SHAPE_L = 0
 SHAPE_T = 1
 MAX_DETECTORS = 8

@@ -161,6 +161,11 @@ GWF_HD double psd_at(const PsdTab& t, double f, double lnf) {
 
 GWF_HD double fcut_eff(double fcut, double fmax) { return (fmax > 0.0 && fcut > fmax) ? fmax : fcut; }
 
+// IMRPhenomD and IMRPhenomD_NRTidalv2 share the IMRPhenomD record layout.
+GWF_HD constexpr bool is_pd(int model) {
+  return model == GWF_MODEL_IMRPHENOMD || model == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
+}
+
 // -------------------------------------------------------------- setup body
 // Fills the event record: waveform coefficients (duals) + projection constants.
 template <int MODEL, int NT>
@@ -173,7 +178,17 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   if (MODEL == GWF_MODEL_IMRPHENOMD) {
     const double f0u = (w.fref > 0.0) ? w.fref : f0;
     pd_setup<NT>(mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3), f0u,
-                 w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT);
+                 w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT, mkc<NT>(1.0), mkc<NT>(1.0));
+  } else if (MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2) {
+    // Lambda1, Lambda2 from the (LambdaTilde, deltaLambda) rows (signal.py:552-559)
+    const double f0u = (w.fref > 0.0) ? w.fref : f0;
+    const D<NT> Mcs = mkseed<NT>(Mc, 0), etas = mkseed<NT>(eta, 1), c1s = mkseed<NT>(chi1, 2), c2s = mkseed<NT>(chi2, 3);
+    D<NT> L1, L2;
+    lam12_from_lamt(mkseed<NT>(par[11 * ld + e], 4), mkseed<NT>(par[12 * ld + e], 5), etas, L1, L2);
+    const D<NT> QM1 = quadmon(L1), QM2 = quadmon(L2);
+    pd_setup<NT>(Mcs, etas, c1s, c2s, f0u, w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT,
+                 QM1, QM2);
+    nrt_setup<NT>(Mcs, etas, c1s, c2s, L1, L2, QM1, QM2, rec, SLOT);
   } else {
     const double Lt = w.is_tidal ? par[11 * ld + e] : 0.0;
     const double dLm = w.is_tidal ? par[12 * ld + e] : 0.0;
@@ -193,7 +208,7 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   ev[EV_RA] = phi;
   ev[EV_CRA] = cos(phi);
   ev[EV_SRA] = sin(phi);
-  ev[EV_FCUT] = rec[(MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
+  ev[EV_FCUT] = rec[(is_pd(MODEL) ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
   // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
   const double ra = phi, dec = 0.5 * PI - theta;
   const double cd = cos(dec), sd = sin(dec), cr = cos(ra), sr = sin(ra);
@@ -343,16 +358,17 @@ GWF_HD D<2> first2(const D<NT>& x) {
 // TaylorF2:   v = (pi M G f)^(1/3).
 template <int MODEL, int NT>
 GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o) {
-  const int smg = MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG;
-  const int seta = MODEL == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA;
+  const int smg = is_pd(MODEL) ? (int)PD_MG : (int)TF_MG;
+  const int seta = is_pd(MODEL) ? (int)PD_ETA : (int)TF_ETA;
   const D<NT> MG = ldd<NT>(rec + smg * SLOT);
   D<NT> v, lv;
-  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+  if (is_pd(MODEL)) {
+    constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
     const D<NT> x = MG * f;
     const D<NT> x13 = dcbrt(x);
     const D<NT> lx = dlog(x);
-    o.A = pd_amp_at<NT>(rec, SLOT, x, x13, w.apply_fcut != 0);
-    o.Phi = pd_phi_at<NT>(rec, SLOT, x, x13, lx, w.apply_fcut != 0);
+    o.A = pd_amp_at<NT, NRT>(rec, SLOT, x, x13, w.apply_fcut != 0 || NRT);
+    o.Phi = pd_phi_at<NT, NRT>(rec, SLOT, x, x13, lx, w.apply_fcut != 0);
     if (need_tau) {
       v = x13 * cbrt(PI);
       lv = (lx + log(PI)) * (1.0 / 3.0);
@@ -374,17 +390,18 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
 // Value-only amplitude without 1/dL (SNR path).
 template <int MODEL>
 GWF_HD double wf_amp_val(const gwf_waveform& w, const double* rec, double f) {
-  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+  if (is_pd(MODEL)) {
+    constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
     const double x = rec[PD_MG * SLOT] * f;
-    return pd_amp_val(rec, SLOT, x, w.apply_fcut != 0);
+    return pd_amp_val(rec, SLOT, x, w.apply_fcut != 0 || NRT, NRT);
   }
   const double f13 = cbrt(f);
   return rec[TF_AMP * SLOT] * (1.0 / (f * sqrt(f13)));
 }
 
 GWF_HD double tau_val(const double* rec, int model, double f) {
-  const int sm = model == GWF_MODEL_IMRPHENOMD ? (int)PD_MG : (int)TF_MG;
-  const int se = model == GWF_MODEL_IMRPHENOMD ? (int)PD_ETA : (int)TF_ETA;
+  const int sm = is_pd(model) ? (int)PD_MG : (int)TF_MG;
+  const int se = is_pd(model) ? (int)PD_ETA : (int)TF_ETA;
   return tau_star(rec[sm * SLOT], rec[se * SLOT], f);
 }
 
@@ -471,8 +488,11 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
     GWF_SET(10, dA * Fp * cp, dA * Fc * cc, -wp.Phi.d[3]);
   }
   if (P > 11) {
-    GWF_SET(11, 0.0, 0.0, -wp.Phi.d[NT > 4 ? 4 : 0]);
-    GWF_SET(12, 0.0, 0.0, -wp.Phi.d[NT > 5 ? 5 : 0]);
+    // tidal parameters: phase (TaylorF2, NRTidalv2) and amplitude (NRTidalv2
+    // tidal amplitude and Planck taper; zero tangent for TaylorF2)
+    const double dA4 = wp.A.d[NT > 4 ? 4 : 0] * idL, dA5 = wp.A.d[NT > 5 ? 5 : 0] * idL;
+    GWF_SET(11, dA4 * Fp * cp, dA4 * Fc * cc, -wp.Phi.d[NT > 4 ? 4 : 0]);
+    GWF_SET(12, dA5 * Fp * cp, dA5 * Fc * cc, -wp.Phi.d[NT > 5 ? 5 : 0]);
   }
 #undef GWF_SET
 }

@@ -41,6 +41,8 @@ enum PdSlot {
   PD_AM23, PD_AM43, PD_AM53, PD_AM73, PD_AM83, PD_AM1, PD_AM2, PD_AM3,
   PD_DL0, PD_DL1, PD_DL2, PD_DL3, PD_DL4,
   PD_G1, PD_G2, PD_G3, PD_FPEAKA, PD_OAMP, PD_FCUT, PD_IETA,
+  // IMRPhenomD_NRTidalv2 only
+  PD_TPH, PD_SSS, PD_TAMP, PD_FMERG,
   PD_COUNT
 };
 // TaylorF2 event record: duals of 4 (6 when tidal) tangents.
@@ -255,7 +257,7 @@ GWF_HD X pd_phi_piecewise(const PdPhase<T>& c, const X& x, bool apply_fcut) {
 template <int N>
 GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2, double f0,
                      bool apply_fcut, const double* qa, const double* qfr, const double* qfd, int nq,
-                     double* rec, int stride) {
+                     double* rec, int stride, const D<N>& QM1, const D<N>& QM2) {
   typedef D<N> T;
   const T M = Mc / dpow(eta, 3. / 5.);
   const T MG = M * GMSUN_C3;
@@ -285,8 +287,9 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   const T alpha4 = fit_eval(FIT_ALPHA4, eta, eta2, xi);
   const T alpha5 = fit_eval(FIT_ALPHA5, eta, eta2, xi);
 
-  const T one = mkc<N>(1.0);
-  Tf2Coeffs<T> tc = tf2_coeffs(eta, Seta, chi1, chi2, one, one, false);
+  // spin-induced quadrupole moments: 1 for IMRPhenomD; from Lambda for
+  // NRTidalv2 (waveforms.py:1388-1389, 1452-1453)
+  Tf2Coeffs<T> tc = tf2_coeffs(eta, Seta, chi1, chi2, QM1, QM2, false);
   const T TF2OA = 3. / (128. * eta);
   const T five = (38645. * PI / 756. - 65. * PI * eta / 9. - tc.five_tmp);
   const T five_log = five * 3.;
@@ -543,9 +546,89 @@ GWF_HD PdPhase<D<N>> pd_load_phase(const double* rec, int stride) {
   return c;
 }
 
-// Per-frequency IMRPhenomD phase Phi(f) as a dual (waveforms.py:1139-1147).
-// Only the coefficients of the branch that x falls in are read.
+// ---- IMRPhenomD_NRTidalv2 per-point terms (waveforms.py:1582-1616, 1744-1799)
+// (pi x)^(2/3) from x^(1/3)
 template <int N>
+GWF_HD D<N> nrt_px23(const D<N>& x13) {
+  const D<N> p13 = x13 * 1.4645918875615231;  // pi^(1/3)
+  return p13 * p13;
+}
+
+// tidal phase (arXiv:1905.06011 eq. 7) + 3.5PN spin-squared / spin-cubed terms
+template <int N>
+GWF_HD D<N> nrt_phase_extra(const double* rec, int stride, const D<N>& x, const D<N>& x13) {
+  typedef D<N> T;
+  const T px23 = nrt_px23(x13);
+  const T px = PI * x;
+  const T px43 = px * (x13 * 1.4645918875615231);
+  const T px53 = px * px23;
+  const T num = 1.0 + (-12.615214237993088 * px23) + (19.0537346970349 * px) + (-21.166863146081035 * px43) +
+                (90.55082156324926 * px53) + (-60.25357801943598 * px * px);
+  const T den = 1.0 + (-15.11120782773667 * px23) + (22.195327350624694 * px) + (8.064109635305156 * px43);
+  return ldd<N>(rec + PD_TPH * stride) * px53 * num / den + ldd<N>(rec + PD_SSS * stride) * px23;
+}
+
+// NaN -> 0, +-inf -> +-DBL_MAX (numpy.nan_to_num)
+GWF_HD double nan_to_num(double v) {
+  if (v != v) return 0.0;
+  if (v > 1.79769313486231570815e308) return 1.79769313486231570815e308;
+  if (v < -1.79769313486231570815e308) return -1.79769313486231570815e308;
+  return v;
+}
+
+// Planck taper ending at 1.2 y (waveforms.py:1777-1797): value and the
+// derivative with respect to y; the reference's custom JVP propagates no
+// tangent through x.  At exactly x == 1.2 y the reference's expression
+// evaluates to 1 (exp(+inf)) instead of its limit 0; the grid's last point
+// M G fcut lands there or a few ulp away depending on the rounding of log10
+// and pow, so the end point is taken as part of the cut here (value 0).
+GWF_HD void planck_taper(double x, double y, double& tv, double& dv) {
+  const double a = 1.2;
+  const double yp = a * y;
+  if (x < y) {
+    tv = 1.0;
+    dv = 0.0;
+    return;
+  }
+  if (!(x < yp)) {
+    tv = 0.0;
+    dv = 0.0;
+    return;
+  }
+  const double z = (yp - y) / (x - y) + (yp - y) / (x - yp);
+  const double ez = exp(z);
+  tv = nan_to_num(1. - 1. / (ez + 1.));
+  dv = nan_to_num(ez * ((-1. + a) / (x - y) + (-1. + a) / (x - yp) + (-y + yp) / ((x - y) * (x - y)) +
+                        1.2 * (-y + yp) / ((x - yp) * (x - yp))) /
+                  ((ez + 1.) * (ez + 1.)));
+}
+
+// tidal amplitude (arXiv:1905.06011 eq. 24) without 1/dL
+template <int N>
+GWF_HD D<N> nrt_amp_tidal(const double* rec, int stride, const D<N>& x13) {
+  typedef D<N> T;
+  const T xT = nrt_px23(x13);
+  const T xT2 = xT * xT;
+  const T xT325 = xT2 * xT * dsqrt(dsqrt(xT));
+  const T xT289 = dexp(2.89 * dlog(xT));
+  const T poly = (1.0 + 4.157407407407407 * xT + 2519.111111111111 * xT289) / (1. + 13477.8073677 * (xT2 * xT2));
+  return ldd<N>(rec + PD_TAMP * stride) * xT325 * poly;
+}
+
+GWF_HD double nrt_amp_tidal_val(const double* rec, int stride, double x13) {
+  const double p13 = x13 * 1.4645918875615231;
+  const double xT = p13 * p13;
+  const double xT2 = xT * xT;
+  const double xT325 = xT2 * xT * sqrt(sqrt(xT));
+  const double xT289 = exp(2.89 * log(xT));
+  const double poly = (1.0 + 4.157407407407407 * xT + 2519.111111111111 * xT289) / (1. + 13477.8073677 * (xT2 * xT2));
+  return rec[PD_TAMP * stride] * xT325 * poly;
+}
+
+// Per-frequency IMRPhenomD phase Phi(f) as a dual (waveforms.py:1139-1147).
+// Only the coefficients of the branch that x falls in are read.  NRT adds
+// the NRTidalv2 terms inside the Mf < 0.2 window (waveforms.py:1613-1616).
+template <int N, bool NRT = false>
 GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, const D<N>& lx, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
@@ -576,14 +659,19 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, const D<N>& 
            GWF_L(PD_C1M) + GWF_L(PD_C2M) * x;
   }
   if (apply_fcut) {
-    if (xv < 0.2) return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF));
+    if (xv < 0.2) {
+      if (NRT)
+        return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra(rec, stride, x, x13));
+      return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF));
+    }
     return phis + 0.0;
   }
+  if (NRT) return phis - GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra(rec, stride, x, x13);
   return phis - GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF);
 #undef GWF_L
 }
 
-template <int N>
+template <int N, bool NRT = false>
 GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
@@ -614,11 +702,21 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& 
   }
   // x^(-7/6) = 1 / (x sqrt(x^(1/3)))
   const T xm76 = 1.0 / (x * dsqrt(x13));
-  return ldd<N>(rec + PD_OAMP * stride) * xm76 * amp;
+  if (!NRT) return ldd<N>(rec + PD_OAMP * stride) * xm76 * amp;
+  // NRTidalv2: (Overallamp amp0 x^-7/6 A_IMR + 2 sqrt(pi/5) Overallamp A_tidal) * taper
+  const T A = ldd<N>(rec + PD_OAMP * stride) * xm76 * amp + nrt_amp_tidal(rec, stride, x13);
+  const T fm = ldd<N>(rec + PD_FMERG * stride);
+  double tv, dv;
+  planck_taper(xv, fm.v, tv, dv);
+  T r;
+  r.v = A.v * tv;
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = A.d[k] * tv + A.v * dv * fm.d[k];
+  return r;
 }
 
 // Value-only IMRPhenomD amplitude (the SNR path): reads only .v of the record.
-GWF_HD double pd_amp_val(const double* rec, int stride, double x, bool apply_fcut) {
+GWF_HD double pd_amp_val(const double* rec, int stride, double x, bool apply_fcut, bool nrt = false) {
   double amp;
   if (x < 0.014) {
     const double x13 = cbrt(x);
@@ -643,7 +741,11 @@ GWF_HD double pd_amp_val(const double* rec, int stride, double x, bool apply_fcu
   }
   const double x13 = cbrt(x);
   // x^(-7/6) = 1 / (x * x^(1/6)), x^(1/6) = sqrt(x^(1/3))
-  return rec[PD_OAMP * stride] * (1.0 / (x * sqrt(x13))) * amp;
+  if (!nrt) return rec[PD_OAMP * stride] * (1.0 / (x * sqrt(x13))) * amp;
+  const double A = rec[PD_OAMP * stride] * (1.0 / (x * sqrt(x13))) * amp + nrt_amp_tidal_val(rec, stride, x13);
+  double tv, dv;
+  planck_taper(x, rec[PD_FMERG * stride], tv, dv);
+  return A * tv;
 }
 
 // ------------------------------------------------------------- TaylorF2
@@ -813,6 +915,76 @@ template <class T>
 GWF_HD T tau_star(const T& MG, const T& eta, double f) {
   const T v = dcbrt(PI * MG * f);
   return tau_star_v(MG, eta, v, dlog(v));
+}
+
+// --------------------------------------------------- IMRPhenomD_NRTidalv2
+// f_merger of the Planck taper (waveforms.py:1760-1773, 1818-1830).
+template <class T>
+GWF_HD T nrt_fmerger(const T& eta, const T& Seta, const T& kappa2T) {
+  const T q = 0.5 * (1.0 + Seta - 2.0 * eta) / eta;
+  const T k2 = kappa2T * kappa2T;
+  const T numPT = 1.0 + 3.35411203e-2 * kappa2T + 4.31460284e-5 * k2;
+  const T denPT = 1.0 + 7.54224145e-2 * kappa2T + 2.23626859e-4 * k2;
+  const T Q0 = 0.3586 / dsqrt(q);
+  return Q0 * (numPT / denPT) / (2. * PI);
+}
+
+// kappa2T (arXiv:1905.06011 eq. 8; waveforms.py:1594, 1749, 1816)
+template <class T>
+GWF_HD T nrt_kappa2T(const T& Seta, const T& L1, const T& L2) {
+  const T Xa = 0.5 * (1.0 + Seta);
+  const T Xb = 0.5 * (1.0 - Seta);
+  // X**5 as pow (the reference's expression): the taper's end 1.2 f_merger and
+  // the grid's last point M G fcut are compared at ulp level
+  return (3.0 / 13.0) * ((1.0 + 12.0 * Xb / Xa) * dpow(Xa, 5.) * L1 + (1.0 + 12.0 * Xa / Xb) * dpow(Xb, 5.) * L2);
+}
+
+// fcut = 1.2 f_merger / (M G) (waveforms.py:1803-1835), value only, from
+// (LambdaTilde, deltaLambda).
+GWF_HD double nrt_fcut(double Mc, double eta, double Lt, double dLm) {
+  const double Seta = sqrt(eta < 0.25 ? 1.0 - 4.0 * eta : 0.0);
+  double L1, L2;
+  lam12_from_lamt(Lt, dLm, eta, L1, L2);
+  const double fm = nrt_fmerger(eta, Seta, nrt_kappa2T(Seta, L1, L2));
+  const double M = Mc / pow(eta, 3. / 5.);
+  return (1.2 * fm) / (M * GMSUN_C3);
+}
+
+// The NRTidalv2 slots of an event record (after pd_setup with the same
+// quadrupole moments): tidal-phase factor, 3.5PN SS + SSS coefficient,
+// tidal-amplitude factor, merger frequency, and the model's fcut.
+template <int N>
+GWF_HD void nrt_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2, const D<N>& L1,
+                      const D<N>& L2, const D<N>& QM1, const D<N>& QM2, double* rec, int stride) {
+  typedef D<N> T;
+  const T Seta = seta_of(eta);
+  const T m1ByM = 0.5 * (1.0 + Seta);
+  const T m2ByM = 0.5 * (1.0 - Seta);
+  const T chi12 = chi1 * chi1, chi22 = chi2 * chi2;
+  const T kappa2T = nrt_kappa2T(Seta, L1, L2);
+  const T TF2OA = 3. / (128. * eta);
+  // octupole moments minus the BBH baseline (waveforms.py:1607-1608)
+  auto octmon = [](const T& Q) {
+    const T lq = dlog(Q);
+    return -1. + dexp(0.003131 + 2.071 * lq - 0.7152 * lq * lq + 0.2458 * lq * lq * lq - 0.03309 * lq * lq * lq * lq);
+  };
+  const T Oct1 = octmon(QM1), Oct2 = octmon(QM2);
+  const T SS = -400. * PI * (QM1 - 1.) * chi12 * m1ByM * m1ByM - 400. * PI * (QM2 - 1.) * chi22 * m2ByM * m2ByM;
+  const T SSS = 10. * ((m1ByM * m1ByM + 308. / 3. * m1ByM) * chi1 + (m2ByM * m2ByM - 89. / 3. * m2ByM) * chi2) *
+                    (QM1 - 1.) * m1ByM * m1ByM * chi12 +
+                10. * ((m2ByM * m2ByM + 308. / 3. * m2ByM) * chi2 + (m1ByM * m1ByM - 89. / 3. * m1ByM) * chi1) *
+                    (QM2 - 1.) * m2ByM * m2ByM * chi22 -
+                440. * Oct1 * m1ByM * m1ByM * m1ByM * chi12 * chi1 - 440. * Oct2 * m2ByM * m2ByM * m2ByM * chi22 * chi2;
+  const T M = Mc / dpow(eta, 3. / 5.);
+  // Overallamp without 1/dL (waveforms.py:1248) times 2 sqrt(pi/5) and -9 kappa2T
+  const T OA = 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3;
+  const T fm = nrt_fmerger(eta, Seta, kappa2T);
+  auto put = [&](int slot, const T& x) { std_(rec + slot * stride, x); };
+  put(PD_TPH, -kappa2T * 2.4375 / (m1ByM * m2ByM));
+  put(PD_SSS, (SS + SSS) * TF2OA);
+  put(PD_TAMP, OA * (2. * sqrt(PI / 5.)) * (-9.0 * kappa2T));
+  put(PD_FMERG, fm);
+  put(PD_FCUT, mkc<N>((1.2 * fm.v) / (M.v * GMSUN_C3)));
 }
 
 }  // namespace gwf

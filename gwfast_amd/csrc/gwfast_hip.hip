@@ -206,6 +206,8 @@ __global__ void k_fcut(gwf_waveform w, const double* __restrict__ par, int64_t l
   double fc;
   if (w.model == GWF_MODEL_IMRPHENOMD)
     fc = w.fcut_par / (Mc * GMSUN_C3 / pow(eta, 3. / 5.));
+  else if (w.model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    fc = nrt_fcut(Mc, eta, par[11 * ld + e], par[12 * ld + e]);
   else
     fc = tf2_fcut(w, Mc, eta, par[9 * ld + e], par[10 * ld + e]);
   out[e] = fcut_eff(fc, fmax);
@@ -607,21 +609,29 @@ __global__ void k_sigma_area(const double* __restrict__ C, int P, int64_t n_ev, 
 // ---------------------------------------------------------------- host side
 int check_wf(const gwf_waveform* wf) {
   if (!wf) return fail(-EINVAL, "waveform description is NULL");
-  if (wf->model != GWF_MODEL_TAYLORF2 && wf->model != GWF_MODEL_IMRPHENOMD)
+  if (wf->model != GWF_MODEL_TAYLORF2 && wf->model != GWF_MODEL_IMRPHENOMD &&
+      wf->model != GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     return fail(-EINVAL, "unknown waveform model");
-  if (wf->model == GWF_MODEL_IMRPHENOMD && (!wf->qnm_a || !wf->qnm_fring || !wf->qnm_fdamp || wf->n_qnm < 2))
+  if (is_pd(wf->model) && (!wf->qnm_a || !wf->qnm_fring || !wf->qnm_fdamp || wf->n_qnm < 2))
     return fail(-EINVAL, "IMRPhenomD needs the QNM tables");
   if (wf->model == GWF_MODEL_IMRPHENOMD && wf->is_tidal) return fail(-EINVAL, "IMRPhenomD is not tidal");
+  if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2 && !wf->is_tidal)
+    return fail(-EINVAL, "IMRPhenomD_NRTidalv2 is tidal (13 parameters)");
   return 0;
 }
 
-int nt_of(const gwf_waveform* wf) { return (wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) ? 6 : 4; }
+int nt_of(const gwf_waveform* wf) {
+  return ((wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) || wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2) ? 6 : 4;
+}
 
 int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int64_t ld, int64_t n_ev,
                  const double* fgrid, int64_t nf, double* rec, hipStream_t st) {
   const dim3 grid((unsigned)((n_ev + 255) / 256)), block(256);
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_IMRPHENOMD, 4>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid,
+                       nf, rec);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else
@@ -716,6 +726,9 @@ int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, 
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_IMRPHENOMD, 4>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out,
                        phi_out, tau_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f,
+                       ampl_out, phi_out, tau_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
                        tau_out);
@@ -755,6 +768,9 @@ static int snr_prepared_impl(const gwf_waveform* wf, const gwf_detector* dets, i
   const double* rec = (const double*)workspace;
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_snr<GWF_MODEL_IMRPHENOMD>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev, snrsq_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_snr<GWF_MODEL_IMRPHENOMD_NRTIDALV2>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
+                       snrsq_out);
   else
     hipLaunchKernelGGL((k_snr<GWF_MODEL_TAYLORF2>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev, snrsq_out);
   return hip_check(hipGetLastError(), "gwf_snr launch");
@@ -785,6 +801,9 @@ int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
                        fisher_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       (int)arm, fisher_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
                        fisher_out);
@@ -843,6 +862,9 @@ int gwf_fisher_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
                        (int)arm, counter, fisher_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec,
+                       n_ev, res_ev, (int)arm, counter, fisher_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
                        (int)arm, counter, fisher_out);
@@ -871,6 +893,9 @@ int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, con
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f,
                        n_f, h_out, J_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       (int)arm, f, n_f, h_out, J_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
                        h_out, J_out);

@@ -1,8 +1,8 @@
 """Waveform models of the event-batched path, with gwfast's WaveFormModel API.
 
-``TaylorF2_RestrictedPN`` and ``IMRPhenomD`` keep the reference's
-constructors, flags, ``ParNums``/``nParams`` and methods
-(gwfast/waveforms.py:49-199, 697-953, 959-1333).  ``Phi``, ``Ampl``,
+``TaylorF2_RestrictedPN``, ``IMRPhenomD`` and ``IMRPhenomD_NRTidalv2`` keep
+the reference's constructors, flags, ``ParNums``/``nParams`` and methods
+(gwfast/waveforms.py:49-199, 697-953, 959-1333, 1339-1837).  ``Phi``, ``Ampl``,
 ``tau_star`` and ``fcut`` are evaluated by the HIP library
 (gwf_waveform_eval / gwf_fcut); inside SNR and Fisher calls the same device
 code runs fused with the projection and the integrals.
@@ -186,4 +186,32 @@ class IMRPhenomD(WaveFormModel):
         w.qnm_a = self._qnm_dev[0].data_ptr()
         w.qnm_fring = self._qnm_dev[1].data_ptr()
         w.qnm_fdamp = self._qnm_dev[2].data_ptr()
+        return w
+
+
+class IMRPhenomD_NRTidalv2(IMRPhenomD):
+    """IMRPhenomD_NRTidalv2 (waveforms.py:1339-1837): IMRPhenomD with the
+    NRTidalv2 tidal phase and amplitude, the spin-induced quadrupole (and
+    octupole) moments from Lambda1, Lambda2, and a Planck taper that ends the
+    waveform at 1.2 f_merger; 13 parameters (LambdaTilde, deltaLambda).  The
+    taper's end point itself is cut (value 0): the reference's expression
+    gives 1 there, at a frequency the grid's last point hits or misses by
+    ulp-level rounding."""
+
+    def __init__(self, fRef=None, **kwargs):
+        super().__init__(fRef=fRef, **kwargs)
+        self.objType = 'BNS'
+        self.is_tidal = True
+        self.ParNums = {'Mc': 0, 'eta': 1, 'dL': 2, 'theta': 3, 'phi': 4, 'iota': 5, 'psi': 6, 'tcoal': 7,
+                        'Phicoal': 8, 'chi1z': 9, 'chi2z': 10, 'LambdaTilde': 11, 'deltaLambda': 12}
+        if not self.is_chi1chi2:
+            self.ParNums['chiS'] = self.ParNums.pop('chi1z')
+            self.ParNums['chiA'] = self.ParNums.pop('chi2z')
+            self.ParNums = dict(sorted(self.ParNums.items(), key=lambda item: item[1]))
+        self.nParams = 13
+
+    def _desc(self):
+        w = super()._desc()
+        w.model = N.MODEL_IMRPHENOMD_NRTIDALV2
+        w.is_tidal = 1
         return w

@@ -39,13 +39,14 @@ extern "C" {
 
 #define GWF_ABI_VERSION 1
 
-enum { GWF_MODEL_TAYLORF2 = 0, GWF_MODEL_IMRPHENOMD = 1 };
+enum { GWF_MODEL_TAYLORF2 = 0, GWF_MODEL_IMRPHENOMD = 1, GWF_MODEL_IMRPHENOMD_NRTIDALV2 = 2 };
 enum { GWF_SHAPE_L = 0, GWF_SHAPE_T = 1 };
 #define GWF_MAX_DETECTORS 8
 #define GWF_MAX_ARMS 24
 
-/* Waveform model description: TaylorF2_RestrictedPN(...) (waveforms.py:726)
- * or IMRPhenomD(fRef=None) (waveforms.py:973). */
+/* Waveform model description: TaylorF2_RestrictedPN(...) (waveforms.py:726),
+ * IMRPhenomD(fRef=None) (waveforms.py:973) or IMRPhenomD_NRTidalv2(fRef=None)
+ * (waveforms.py:1339-1837; is_tidal = 1, QNM tables as IMRPhenomD). */
 typedef struct gwf_waveform {
   int32_t model;            /* GWF_MODEL_* */
   int32_t is_tidal;         /* TaylorF2 is_tidal: 13 parameters */

@@ -328,7 +328,8 @@ class IMRPhenomD:
         alpha4 = _fit('alpha4', eta, eta2, xi)
         alpha5 = _fit('alpha5', eta, eta2, xi)
 
-        c = _tf2_coeffs(eta, Seta, chi1, chi2, 1.0, 1.0, False)
+        QuadMon1, QuadMon2 = self._quadmons(ev)
+        c = _tf2_coeffs(eta, Seta, chi1, chi2, QuadMon1, QuadMon2, False)
         TF2OverallAmpl = 3. / (128. * eta)
         five = (38645. * np.pi / 756. - 65. * np.pi * eta / 9. - c['five_tmp'])
         five_log = five * 3.
@@ -399,9 +400,16 @@ class IMRPhenomD:
                   C1MRD=C1MRD, C2MRD=C2MRD, fMRDJoin=fMRDJoin)
         phiRef = self._phi_piecewise(fRef, co)
         phis = self._phi_piecewise(fgrid, co)
+        extra = self._phase_extra(fgrid, ev, eta, Seta, QuadMon1, QuadMon2, TF2OverallAmpl)
         if self.apply_fcut:
-            return phis + D.where(D.value(fgrid) < self.fcutPar, -t0 * (fgrid - fRef) - phiRef, 0.)
-        return phis - t0 * (fgrid - fRef) - phiRef
+            return phis + D.where(D.value(fgrid) < self.fcutPar, -t0 * (fgrid - fRef) - phiRef + extra, 0.)
+        return phis - t0 * (fgrid - fRef) - phiRef + extra
+
+    def _quadmons(self, ev):
+        return 1.0, 1.0
+
+    def _phase_extra(self, fgrid, ev, eta, Seta, QuadMon1, QuadMon2, TF2OverallAmpl):
+        return 0.0
 
     def _phi_piecewise(self, x, co):
         """The 3-branch phase at waveforms.py:1143-1147 (fcut applied when
@@ -501,6 +509,9 @@ class IMRPhenomD:
         if self.apply_fcut:
             mrd = D.where(xv < self.fcutPar, mrd, 0.)
         amplitudeIMR = D.where(xv < self.AMP_fJoin_INS, ins, D.where(xv < D.value(fpeak), inter, mrd))
+        return self._amp_final(Overallamp, amp0, fgrid, amplitudeIMR, ev, eta, Seta)
+
+    def _amp_final(self, Overallamp, amp0, fgrid, amplitudeIMR, ev, eta, Seta):
         return Overallamp * amp0 * (fgrid ** (-7. / 6.)) * amplitudeIMR
 
     def tau_star(self, f, ev):
@@ -579,3 +590,120 @@ def _phenomd_deltas(f1, f2, f3, v1, v2, v3, d1, d2):
                 + 3 * f1 * f32 * v1 - f33 * v1 - f13 * v2 + 3 * f12 * f3 * v2 - 3 * f1 * f32 * v2 + f33 * v2 + f13 * v3
                 - 3 * f1 * f22 * v3 + 2 * f23 * v3 - 3 * f12 * f3 * v3 + 6 * f1 * f2 * f3 * v3 - 3 * f22 * f3 * v3) / den)
     return delta0, delta1, delta2, delta3, delta4
+
+
+# ---------------------------------------------------------------- NRTidalv2
+def _kappa2T(Seta, L1, L2):
+    """Tidal coupling constant, waveforms.py:1594 / 1749 (arXiv:1905.06011 eq. 8)."""
+    Xa = 0.5 * (1.0 + Seta)
+    Xb = 0.5 * (1.0 - Seta)
+    return (3.0 / 13.0) * ((1.0 + 12.0 * Xb / Xa) * (Xa ** 5) * L1 + (1.0 + 12.0 * Xa / Xb) * (Xb ** 5) * L2)
+
+
+def _f_merger(eta, Seta, kappa2T):
+    """Dimensionless merger frequency of the Planck taper, waveforms.py:1760-1773."""
+    q = 0.5 * (1.0 + Seta - 2.0 * eta) / eta
+    k2 = kappa2T * kappa2T
+    numPT = 1.0 + 3.35411203e-2 * kappa2T + 4.31460284e-5 * k2
+    denPT = 1.0 + 7.54224145e-2 * kappa2T + 2.23626859e-4 * k2
+    return 0.3586 / D.sqrt(q) * (numPT / denPT) / (2. * np.pi)
+
+
+def planck_taper(x, y, end_inclusive=False):
+    """waveforms.py:1775-1797: the taper value with the reference's custom
+    JVP -- a tangent through y (the merger frequency) only; x's tangent is
+    dropped (defjvps(None, ...)); NaN -> 0 (nan_to_num) in value and tangent.
+    At exactly x == 1.2 y the reference's expression gives 1 (exp(+inf));
+    end_inclusive=True treats that point as cut (value 0), as the HIP
+    kernels do."""
+    xv, yv = np.real(D.value(x)), np.real(D.value(y))
+    a = 1.2
+    ypv = a * yv
+    with np.errstate(divide='ignore', invalid='ignore', over='ignore'):
+        z = (ypv - yv) / (xv - yv) + (ypv - yv) / (xv - ypv)
+        ez = np.exp(z)
+        cut = (xv >= ypv) if end_inclusive else (xv > ypv)
+        val = np.where(xv < yv, 1., np.where(cut, 0., 1. - 1. / (ez + 1.)))
+        der = np.where(xv < yv, 0., np.where(cut, 0., ez * ((-1. + a) / (xv - yv) + (-1. + a) / (xv - ypv)
+                                                                 + (-yv + ypv) / ((xv - yv) ** 2)
+                                                                 + 1.2 * (-yv + ypv) / ((xv - ypv) ** 2))
+                                 / ((ez + 1.) ** 2)))
+    val = np.nan_to_num(val)
+    der = np.nan_to_num(der)
+    if isinstance(y, D.Dual):
+        return D.Dual(val, D._expand(y.d, y.v.ndim, val.ndim) * der[None])
+    return val
+
+
+class IMRPhenomD_NRTidalv2(IMRPhenomD):
+    """IMRPhenomD_NRTidalv2 (waveforms.py:1339-1837): IMRPhenomD with the
+    spin-induced quadrupole in the 2PN and 3PN coefficients, the NRTidalv2
+    tidal phase and amplitude, the 3.5PN spin-squared / spin-cubed terms and
+    a Planck taper ending at 1.2 f_merger."""
+
+    name = 'IMRPhenomD_NRTidalv2'
+
+    def __init__(self, fRef=None, apply_fcut=True, taper_end_inclusive=False):
+        super().__init__(fRef=fRef, apply_fcut=apply_fcut)
+        self.taper_end_inclusive = taper_end_inclusive
+        self.is_tidal = True
+        self.objType = 'BNS'
+        self.nParams = 13
+
+    def _lambdas(self, ev):
+        if 'Lambda1' in ev:
+            return ev['Lambda1'], ev['Lambda2']
+        return 0.0 * ev['eta'], 0.0 * ev['eta']
+
+    def _quadmons(self, ev):
+        """waveforms.py:1388-1389 (arXiv:1608.02582 eq. 15)."""
+        if 'Lambda1' in ev:
+            return _quadmon(ev['Lambda1']), _quadmon(ev['Lambda2'])
+        return 1.0, 1.0
+
+    def _phase_extra(self, fgrid, ev, eta, Seta, QuadMon1, QuadMon2, TF2OverallAmpl):
+        """waveforms.py:1582-1616: tidal phase (arXiv:1905.06011 eq. 7),
+        3.5PN spin-squared and spin-cubed terms (eq. 27)."""
+        L1, L2 = self._lambdas(ev)
+        chi1, chi2 = ev['chi1z'], ev['chi2z']
+        chi12, chi22 = chi1 * chi1, chi2 * chi2
+        m1ByM = 0.5 * (1.0 + Seta)
+        m2ByM = 0.5 * (1.0 - Seta)
+        kappa2T = (3.0 / 13.0) * ((1.0 + 12.0 * m2ByM / m1ByM) * (m1ByM ** 5) * L1
+                                  + (1.0 + 12.0 * m1ByM / m2ByM) * (m2ByM ** 5) * L2)
+        px = np.pi * fgrid
+        px23 = px ** (2. / 3.)
+        numTidal = (1.0 + (-12.615214237993088 * px23) + (19.0537346970349 * px) + (-21.166863146081035 * (px ** (4. / 3.)))
+                    + (90.55082156324926 * (px ** (5. / 3.))) + (-60.25357801943598 * px * px))
+        denTidal = 1.0 + (-15.11120782773667 * px23) + (22.195327350624694 * px) + (8.064109635305156 * (px ** (4. / 3.)))
+        tidal_phase = -kappa2T * 2.4375 / (m1ByM * m2ByM) * (px ** (5. / 3.)) * numTidal / denTidal
+
+        def octmon(Q):
+            lq = D.log(Q)
+            return -1. + D.exp(0.003131 + 2.071 * lq - 0.7152 * lq * lq + 0.2458 * lq * lq * lq - 0.03309 * lq * lq * lq * lq)
+        Oct1, Oct2 = octmon(QuadMon1), octmon(QuadMon2)
+        SS = -400. * np.pi * (QuadMon1 - 1.) * chi12 * m1ByM * m1ByM - 400. * np.pi * (QuadMon2 - 1.) * chi22 * m2ByM * m2ByM
+        SSS = (10. * ((m1ByM * m1ByM + 308. / 3. * m1ByM) * chi1 + (m2ByM * m2ByM - 89. / 3. * m2ByM) * chi2) * (QuadMon1 - 1.) * m1ByM * m1ByM * chi12
+               + 10. * ((m2ByM * m2ByM + 308. / 3. * m2ByM) * chi2 + (m1ByM * m1ByM - 89. / 3. * m1ByM) * chi1) * (QuadMon2 - 1.) * m2ByM * m2ByM * chi22
+               - 440. * Oct1 * m1ByM * m1ByM * m1ByM * chi12 * chi1 - 440. * Oct2 * m2ByM * m2ByM * m2ByM * chi22 * chi2)
+        return tidal_phase + (SS + SSS) * TF2OverallAmpl * px23
+
+    def _amp_final(self, Overallamp, amp0, fgrid, amplitudeIMR, ev, eta, Seta):
+        """waveforms.py:1744-1799: + tidal amplitude (arXiv:1905.06011 eq. 24),
+        times the Planck taper."""
+        L1, L2 = self._lambdas(ev)
+        kappa2T = _kappa2T(Seta, L1, L2)
+        xT = (np.pi * fgrid) ** (2. / 3.)
+        polyTidal = (1.0 + 4.157407407407407 * xT + 2519.111111111111 * (xT ** 2.89)) / (1. + 13477.8073677 * (xT ** 4))
+        ampTidal = -9.0 * kappa2T * (xT ** 3.25) * polyTidal
+        taper = planck_taper(fgrid, _f_merger(eta, Seta, kappa2T), self.taper_end_inclusive)
+        return Overallamp * (amp0 * (fgrid ** (-7. / 6.)) * amplitudeIMR + 2 * np.sqrt(np.pi / 5.) * ampTidal) * taper
+
+    def fcut(self, ev):
+        """waveforms.py:1803-1835: 1.2 f_merger / (M G)."""
+        eta = np.asarray(ev['eta'], dtype=float)
+        M = ev['Mc'] / (eta ** (3. / 5.))
+        Seta = np.sqrt(np.where(eta < 0.25, 1.0 - 4.0 * eta, 0.))
+        L1, L2 = self._lambdas(ev)
+        fm = _f_merger(eta, Seta, _kappa2T(Seta, D.value(L1), D.value(L2)))
+        return 1.2 * fm / (M * glob.GMsun_over_c3)

@@ -18,6 +18,9 @@ def _models(name, **kw):
     from oracle import waveforms as OW
     if name == 'TF2':
         return W.TaylorF2_RestrictedPN(**kw), OW.TaylorF2(**kw)
+    if name == 'NRT':
+        # the kernels cut the taper's end point (see IMRPhenomD_NRTidalv2)
+        return W.IMRPhenomD_NRTidalv2(**kw), OW.IMRPhenomD_NRTidalv2(taper_end_inclusive=True, **kw)
     return W.IMRPhenomD(**kw), OW.IMRPhenomD(**kw)
 
 
@@ -28,6 +31,7 @@ CASES = [
     ('C1-PhenomD-net', 'PD', {}, NET_C1, True, 8, 2000, bbh),
     ('C1-PhenomD-net-static', 'PD', {}, NET_C1, False, 6, 700, bbh),
     ('C3-TF2-tidal-ET', 'TF2', {'is_tidal': True}, NET_C1[:1], True, 6, 1500, bns),
+    ('NRTidalv2-net', 'NRT', {}, NET_C1, True, 8, 1500, bns),
 ]
 
 
@@ -300,3 +304,46 @@ def test_catalogue_forecast_files_and_resume(gpu, tmp_path):
     FC.run_catalog(ev, net, str(tmp_path), batch_size=100, params_fix=['Phicoal'], res=500, resume=True,
                    verbose=False)
     assert p.read_text() == 'sentinel\n'
+
+
+def test_nrtidal_waveform_methods(gpu):
+    wf, owf = _models('NRT')
+    ev = bns(5, seed=13)
+    od = oracle_net(NET_C0, False)[0]
+    f, _ = od.grid(owf, ev, 256)
+    for meth in ('Ampl', 'tau_star'):
+        g = getattr(wf, meth)(f, **ev)
+        o = getattr(owf, meth)(f, ev)
+        nz = o != 0
+        assert np.all(g[~nz] == 0), meth
+        assert np.max(np.abs(g[nz] - o[nz]) / np.abs(o[nz])) <= 1e-11, meth
+    g = wf.Phi(f, **ev)
+    o = owf.Phi(f, ev)
+    assert np.max(np.abs(g - o)) <= 1e-12 * np.max(np.abs(o)) + 1e-9
+    assert np.allclose(wf.fcut(**ev), owf.fcut(ev), rtol=1e-14, atol=0)
+
+
+def test_nrtidal_gw170817_known_answers(gpu):
+    """gwfast_tutorial.ipynb cells 12 and 22 on the GPU: LIGO-Virgo O2 SNR of
+    GW170817 '33.16' and 90% sky area after fixing deltaLambda '19.9 deg^2'."""
+    import os
+    from gwfast_amd import fisherTools as FT
+    from gwfast_amd.network import DetNet
+    from gwfast_amd.signal import GWSignal
+    from gwfast_amd.waveforms import IMRPhenomD_NRTidalv2
+    from gwfast_amd import gwfastGlobals as glob
+    from tests.test_oracle_pins import GOLDEN_PSD, O2_PSDS, gw170817_tutorial
+    wf = IMRPhenomD_NRTidalv2()
+    sigs = {n: GWSignal(wf, psd_path=os.path.join(GOLDEN_PSD, O2_PSDS[n]), detector_shape=glob.detectors[n]['shape'],
+                        det_lat=glob.detectors[n]['lat'], det_long=glob.detectors[n]['long'],
+                        det_xax=glob.detectors[n]['xax'], useEarthMotion=False, fmin=10., verbose=False)
+            for n in ('L1', 'H1', 'Virgo')}
+    net = DetNet(sigs, verbose=False)
+    ev = gw170817_tutorial()
+    snr = net.SNR(dict(ev))
+    assert abs(snr[0] - 33.16) < 0.005
+    F = net.FisherMatr(dict(ev))
+    Ff, nums = FT.fixParams(F, wf.ParNums, ['deltaLambda'])
+    C, _ = FT.CovMatr(Ff)
+    area = FT.compute_localization_region(C, nums, ev['theta'])
+    assert abs(area[0] - 19.9) < 0.05

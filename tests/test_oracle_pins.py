@@ -422,3 +422,131 @@ def test_localization_region_closed_form():
     a = localization_region(C, PARNUMS11, th, 90, 'Sterad')
     assert np.allclose(a, np.log(10) * 2 * np.pi * np.sqrt(C[3, 3] * C[4, 4]) * np.abs(np.sin(th)), rtol=1e-14)
     assert np.allclose(localization_region(C, PARNUMS11, th), a * (180 / np.pi) ** 2, rtol=1e-14)
+
+
+# ------------------------------------------------------ IMRPhenomD_NRTidalv2
+GOLDEN_PSD = __import__('os').path.join(__import__('os').path.dirname(__file__), 'golden', 'psds')
+O2_PSDS = {'L1': '2017-08-06_DCH_C02_L1_O2_Sensitivity_strain_asd.txt',
+           'H1': '2017-06-10_DCH_C02_H1_O2_Sensitivity_strain_asd.txt',
+           'Virgo': 'Hrec_hoft_V1O2Repro2A_16384Hz.txt'}
+
+
+def gw170817_tutorial():
+    """notebooks/gwfast_tutorial.ipynb cell 10 (printed values): Planck18
+    dL(z=0.0098) = 0.04374755 Gpc and GMST(tGPS 1187008882.4) = 0.43432288 d."""
+    z = np.array([0.00980])
+    return {'Mc': np.array([1.1859]) * (1. + z), 'dL': np.array([0.04374755]),
+            'theta': np.array([np.pi / 2. + 0.4080839999999999]), 'phi': np.array([3.4461599999999994]),
+            'iota': np.array([2.545065595974997]), 'psi': np.array([0.]), 'tcoal': np.array([0.43432288]),
+            'eta': np.array([0.24786618323504223]), 'Phicoal': np.array([0.]),
+            'chi1z': np.array([0.005136138323169717]), 'chi2z': np.array([0.003235146993487445]),
+            'Lambda1': np.array([368.17802383555687]), 'Lambda2': np.array([586.5487031450857])}
+
+
+def _lv_o2_oracle():
+    import os
+    return [OD.Detector(os.path.join(GOLDEN_PSD, O2_PSDS[n]), glob.detectors[n]['shape'], glob.detectors[n]['lat'],
+                        glob.detectors[n]['long'], glob.detectors[n]['xax'], useEarthMotion=False, fmin=10.)
+            for n in ('L1', 'H1', 'Virgo')]
+
+
+def test_nrtidal_gw170817_known_answers():
+    """gwfast_tutorial.ipynb cells 12 and 22: LIGO-Virgo O2 network SNR of
+    GW170817 with IMRPhenomD_NRTidalv2 '33.16', and its 90% sky area after
+    fixing deltaLambda '19.9 deg^2'."""
+    from oracle import covariance as OC
+    ev = gw170817_tutorial()
+    dets = _lv_o2_oracle()
+    wf = OW.IMRPhenomD_NRTidalv2()
+    snr = OD.network_snr(dets, wf, ev, 1000)
+    assert abs(snr[0] - 33.16) < 0.005
+    F = OD.network_fisher(dets, wf, ev, 1000, mode='faithful')
+    PN = {'Mc': 0, 'eta': 1, 'dL': 2, 'theta': 3, 'phi': 4, 'iota': 5, 'psi': 6, 'tcoal': 7, 'Phicoal': 8,
+          'chi1z': 9, 'chi2z': 10, 'LambdaTilde': 11, 'deltaLambda': 12}
+    Ff, nums = OC.fix_params(F, PN, ['deltaLambda'])
+    C, eps, _ = OC.cov_matr(Ff)
+    area = OC.localization_region(C, nums, ev['theta'])
+    assert abs(area[0] - 19.9) < 0.05
+
+
+def test_nrtidal_reduces_to_phenomd_without_tides_and_spins():
+    """Lambda = 0, chi = 0: NRTidalv2's phase is IMRPhenomD's and its
+    amplitude is IMRPhenomD's below the taper (quadrupole moment 1, no tidal
+    or spin-cubed terms)."""
+    ev = _pd_ev(3, 5)
+    ev['chi1z'] = ev['chi2z'] = np.zeros(3)
+    ev['Lambda1'] = ev['Lambda2'] = np.zeros(3)
+    nrt, pd = OW.IMRPhenomD_NRTidalv2(fRef=10.), OW.IMRPhenomD(fRef=10.)
+    fm = nrt.fcut(ev) / 1.2
+    f = np.geomspace(1e-2, 1., 50)[:, None] * fm[None, :] * 0.9
+    assert np.allclose(nrt.Phi(f, ev), pd.Phi(f, ev), rtol=1e-13, atol=1e-9)
+    assert np.allclose(nrt.Ampl(f, ev), pd.Ampl(f, ev), rtol=1e-14, atol=0)
+
+
+def test_nrtidal_leading_tidal_phase_is_5pn_term():
+    """At low frequency the tidal phase tends to the 5PN term
+    -(117/256) (LambdaTilde/eta) v^5 (Flanagan & Hinderer 2008), whatever the
+    mass ratio: kappa2T = 3/16 LambdaTilde."""
+    from gwfast_amd import catalog
+    ev = catalog.draw_bns(3, seed=4)
+    ev['chi1z'] = ev['chi2z'] = np.zeros(3)  # quadrupole terms enter with chi^2
+    wf = OW.IMRPhenomD_NRTidalv2()
+    ev0 = dict(ev)
+    ev0['Lambda1'] = ev0['Lambda2'] = np.zeros(3)
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    x = 1e-6
+    f = np.array([[x]]) / M[None, :]
+    d = wf.Phi(f, ev) - wf.Phi(f, ev0)
+    Lt, _ = OW.lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
+    lead = -(117. / 256.) * Lt / ev['eta'] * (np.pi * x) ** (5. / 3.)
+    # the phase's fRef reference point carries no tidal term; the Pade's
+    # relative corrections are O((pi x)^(2/3)) ~ 2e-4 here
+    assert np.allclose(d[0], lead, rtol=2e-3)
+
+
+def test_planck_taper_values_and_y_derivative():
+    y = np.array([0.05])
+    xs = y * np.array([0.5, 1.0, 1.05, 1.1, 1.15, 1.2, 1.3])
+    v = OW.planck_taper(xs, y)
+    assert v[0] == 1.0 and v[1] == 1.0 and v[-1] == 0.0
+    assert np.all(np.diff(v[1:5]) < 0)
+    assert v[5] == 1.0  # the reference's expression at exactly 1.2 y (exp(+inf))
+    assert OW.planck_taper(xs, y, end_inclusive=True)[5] == 0.0
+    yd = D.Dual.seed(y, 0, 1)
+    for x in (1.05, 1.1, 1.15):
+        h = 1e-8 * y
+        fd = (OW.planck_taper(x * y, y + h) - OW.planck_taper(x * y, y - h)) / (2 * h)
+        an = OW.planck_taper(np.array([x * y[0]]), yd).d[0]
+        assert np.allclose(an, fd, rtol=1e-6)
+
+
+def test_nrtidal_quadrupole_continuous_and_octupole_baseline():
+    lo = OW._quadmon(np.array([1.0 - 1e-12]))
+    hi = OW._quadmon(np.array([1.0]))
+    assert abs(lo[0] - hi[0]) < 1e-6
+    assert abs(hi[0] - np.exp(0.1940)) < 1e-15
+
+
+def test_nrtidal_amplitude_derivatives_vs_finite_differences():
+    """Tidal amplitude and taper tangents (LambdaTilde, deltaLambda, eta)
+    against central differences of the amplitude; the taper's x-tangent is
+    dropped as in the reference's custom JVP, so the check sits below the
+    taper (pure tidal amplitude) and inside it for the Lambda rows only."""
+    from gwfast_amd import catalog
+    ev = catalog.draw_bns(2, seed=6)
+    wf = OW.IMRPhenomD_NRTidalv2()
+    Lt, dL = OW.lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
+    fm = wf.fcut(ev) / 1.2
+    f = np.array([0.3, 0.7, 1.05, 1.12])[:, None] * fm[None, :]
+
+    def amp(lt, dl):
+        e = dict(ev)
+        e['Lambda1'], e['Lambda2'] = OW.lam12_from_lamt_dellam(lt, dl, ev['eta'])
+        return wf.Ampl(f, e)
+    e = dict(ev)
+    L1, L2 = OW.lam12_from_lamt_dellam(D.Dual.seed(Lt, 0, 2), D.Dual.seed(dL, 1, 2), ev['eta'])
+    e['Lambda1'], e['Lambda2'] = L1, L2
+    A = wf.Ampl(f, e)
+    for k, (lt, dl) in enumerate([(Lt * 1e-6, 0 * dL), (0 * Lt, np.maximum(np.abs(dL), 1.) * 1e-6)]):
+        fd = (amp(Lt + lt, dL + dl) - amp(Lt - lt, dL - dl)) / (2 * (lt + dl))
+        assert np.allclose(A.d[k], fd, rtol=1e-5, atol=1e-9 * np.abs(A.v).max())
