@@ -151,10 +151,22 @@ GWF_HD double psd_at(const PsdTab& t, double f, double lnf) {
   const double* x = t.f;
   const int n = t.n;
   if (f < x[0] || f > x[n - 1]) return 1.0;
+  // segment = largest j <= n-2 with x[j] <= f (searchsorted right, clipped):
+  // a guess from log-uniform spacing, a short walk, then bisection for
+  // tables that are not log-spaced (e.g. 0.5 Hz linear ASD tables)
   int j = (int)((lnf - t.lx0) * t.idl);
   j = j < 0 ? 0 : (j > n - 2 ? n - 2 : j);
-  while (j < n - 2 && x[j + 1] <= f) ++j;
-  while (j > 0 && x[j] > f) --j;
+  for (int s = 0; s < 4 && j < n - 2 && x[j + 1] <= f; ++s) ++j;
+  for (int s = 0; s < 4 && j > 0 && x[j] > f; ++s) --j;
+  if (x[j] > f || (j < n - 2 && x[j + 1] <= f)) {
+    int lo = 0, hi = n - 2;  // x[0] <= f holds here
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (x[mid] <= f) lo = mid;
+      else hi = mid - 1;
+    }
+    j = lo;
+  }
   const int i = j + 1;
   return t.S[i - 1] + ((f - x[i - 1]) / (x[i] - x[i - 1])) * (t.S[i] - t.S[i - 1]);
 }
