@@ -30,6 +30,11 @@ RES = 2000
 SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
 GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per k_fisher launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
+# (37,879 KiB, x2.0: a 256 MiB device copy reads 131,085 KiB by that counter)
+# and WRITE_SIZE (9,454 KiB, x1.0) runs of this script; summary in
+# profiles/k_fisher_imrphenomd_net_10k_2000f.md
+K_FISHER_TRAFFIC_10K = (37879.4 * 2.0 + 9453.6) * 1024
 
 
 def build_net():
@@ -110,7 +115,6 @@ def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, thres
     from gwfast_amd import screen as SC
     t0 = time.perf_counter()
     rows_t, snr_t = SC.training_set(net, n_train, seed=1001, res=res)
-    vrows, vsnr = SC.training_set(net, 50_000, seed=1002, res=res)
     torch.cuda.synchronize()
     t_gen = time.perf_counter() - t0
     sc = SC.SNRScreen(seed=7)
@@ -166,6 +170,10 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=512)
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
+    ap.add_argument('--screen-only', action='store_true', help='run only the learned-screen leg')
+    ap.add_argument('--screen-events', type=int, default=1_000_000)
+    ap.add_argument('--screen-train', type=int, default=1_000_000)
+    ap.add_argument('--screen-epochs', type=int, default=8)
     ap.add_argument('--copy-calibration-mib', type=int, default=0,
                     help='first run a device copy of this many MiB (a known-size kernel for PMC byte counters)')
     args = ap.parse_args()
@@ -179,6 +187,11 @@ def main():
     from gwfast_amd import signal as S
 
     torch.cuda.set_device(0)
+    if args.screen_only:
+        _, net = build_net()
+        print(json.dumps({'screen': screen_leg(net, n_cat=args.screen_events, n_train=args.screen_train,
+                                                epochs=args.screen_epochs)}), flush=True)
+        return
     if args.copy_calibration_mib:
         src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device='cuda')
         dst = torch.empty_like(src)
@@ -231,7 +244,8 @@ def main():
                                '11 params, %d events x %d freqs, FP64 fused plain path' % (n, RES),
                    'n_events': n, 'n_freqs': RES, 'n_params': 11, 'n_ifo': 5, 'parallelism': 'single GPU'},
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP64_PEAK_TFLOPS, 'traffic': None,
+                     'frac': achieved / FP64_PEAK_TFLOPS,
+                     'traffic': K_FISHER_TRAFFIC_10K if n == N_EV else None,
                      'kernel': 'k_fisher<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
                      'note': 'FP64 dense peak; algorithmic flops = Gram 4 P^2 N_f N_ifo per Fisher'},
     }
@@ -252,7 +266,8 @@ def main():
         del Fs
     if not args.no_screen:
         del F, ws
-        out['screen_1e6'] = screen_leg(net)
+        out['screen_1e6'] = screen_leg(net, n_cat=args.screen_events, n_train=args.screen_train,
+                                       epochs=args.screen_epochs)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
     print(json.dumps(out), flush=True)
