@@ -42,6 +42,8 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
         { err = "PSD frequency bounds must be positive and increasing"; return -EINVAL; }
       t.lx0 = std::log(g.psd_f_first);
       t.idl = (double)(g.n_psd - 1) / (std::log(g.psd_f_last) - t.lx0);
+      t.f_first = g.psd_f_first;
+      t.f_last = g.psd_f_last;
     }
     SiteK& S = net.site[d];
     const double lat = g.lat_deg * M_PI / 180., lon = g.long_deg * M_PI / 180., xax = g.xax_deg * M_PI / 180.;
@@ -105,6 +107,7 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
     na += narms;
   }
   net.n_arms = na;
+  net.n_psd = np;
   return 0;
 }
 

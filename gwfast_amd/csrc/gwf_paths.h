@@ -45,6 +45,7 @@ struct PsdTab {
   int n;
   double lx0;  // log(f[0])
   double idl;  // (n-1) / (log f[n-1] - log f[0])
+  double f_first, f_last;  // f[0], f[n-1]
 };
 
 struct SiteK {
@@ -67,7 +68,7 @@ struct ArmK {
 // triangle's three arms reduce to two weighted arms.
 struct NetK {
   int n_sites, n_arms, res, any_earth;
-  int n_sum_arms, pad_;
+  int n_sum_arms, n_psd;  // n_psd: distinct PSD tables
   double fmin, fmax;
   SiteK site[GWF_MAX_DETECTORS];
   ArmK arm[GWF_MAX_ARMS];
@@ -169,6 +170,32 @@ GWF_HD double psd_at(const PsdTab& t, double f, double lnf) {
   }
   const int i = j + 1;
   return t.S[i - 1] + ((f - x[i - 1]) / (x[i] - x[i - 1])) * (t.S[i] - t.S[i - 1]);
+}
+
+// The same interpolation split in two: psd_prefetch issues the loads of the
+// segment a log-spaced table predicts (independent of each other, so their
+// latency overlaps the waveform work that follows), psd_finish checks the
+// prediction and interpolates, falling back to psd_at when it missed.
+// Results are identical to psd_at.
+struct PsdPre {
+  int j;
+  double x0, x1, s0, s1;
+};
+
+GWF_HD void psd_prefetch(const PsdTab& t, double f, double lnf, PsdPre& p) {
+  int j = (int)((lnf - t.lx0) * t.idl);
+  j = j < 0 ? 0 : (j > t.n - 2 ? t.n - 2 : j);
+  p.j = j;
+  p.x0 = t.f[j];
+  p.x1 = t.f[j + 1];
+  p.s0 = t.S[j];
+  p.s1 = t.S[j + 1];
+}
+
+GWF_HD double psd_finish(const PsdTab& t, double f, double lnf, const PsdPre& p) {
+  if (f < t.f_first || f > t.f_last) return 1.0;
+  if (p.x0 <= f && (p.j == t.n - 2 || f < p.x1)) return p.s0 + ((f - p.x0) / (p.x1 - p.x0)) * (p.s1 - p.s0);
+  return psd_at(t, f, lnf);
 }
 
 GWF_HD double fcut_eff(double fcut, double fmax) { return (fmax > 0.0 && fcut > fmax) ? fmax : fcut; }
@@ -548,6 +575,12 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
     gw.step(base, res, &f, &wt, &lnf);
     if (i >= res) continue;
     wt *= 4.0;
+#ifndef GWF_NO_PSD_PREFETCH
+    // loads of the first two PSD tables' segments, consumed after the waveform
+    PsdPre pre0, pre1;
+    psd_prefetch(net.psd[0], f, lnf, pre0);
+    if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
+#endif
     WfPoint<MODEL, NT> wp;
     wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
     const double tc = ev[EV_TC];
@@ -563,7 +596,14 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (s != cur_site) {
         if (S.psd != cur_psd) {
-          Sf = psd_at(net.psd[S.psd], f, lnf);
+#ifndef GWF_NO_PSD_PREFETCH
+          if (S.psd == 0)
+            Sf = psd_finish(net.psd[0], f, lnf, pre0);
+          else if (S.psd == 1)
+            Sf = psd_finish(net.psd[1], f, lnf, pre1);
+          else
+#endif
+            Sf = psd_at(net.psd[S.psd], f, lnf);
           cur_psd = S.psd;
         }
         site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);

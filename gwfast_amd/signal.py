@@ -53,7 +53,7 @@ class GWSignal(object):
     # ------------------------------------------------------------------ ABI
     def c_struct(self):
         if self._psd_dev is None:
-            self._psd_dev = (E.to_dev(self.strainFreq), E.to_dev(self.noiseCurve))
+            self._psd_dev = _shared_psd(self.strainFreq, self.noiseCurve)
         d = N.GwfDetector()
         d.shape = N.SHAPE_L if self.detector_shape == 'L' else N.SHAPE_T
         d.earth_motion = int(self.useEarthMotion)
@@ -174,6 +174,21 @@ class GWSignal(object):
         """signal.py:468-483: 2 pi f tcoal - Phicoal - Phi_GW."""
         return (2. * np.pi * f * (evParams['tcoal'] * 3600. * 24.) - evParams['Phicoal']
                 - self.wf_model.Phi(f, **evParams))
+
+
+_PSD_DEV = {}
+
+
+def _shared_psd(f, S):
+    """One device copy per distinct PSD table and device: detectors with the
+    same noise curve (e.g. two CE sites) then share it, and a network kernel
+    interpolates each distinct table once per frequency."""
+    key = (E.device().index, f.shape[0], hash(f.tobytes()), hash(S.tobytes()))
+    hit = _PSD_DEV.get(key)
+    if hit is None or not (np.array_equal(hit[2], f) and np.array_equal(hit[3], S)):
+        hit = (E.to_dev(f), E.to_dev(S), f.copy(), S.copy())
+        _PSD_DEV[key] = hit
+    return hit[0], hit[1]
 
 
 def _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff):
