@@ -617,18 +617,18 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
 }
 
 // --------------------------------------------------------------- SNR body
-// Per-arm trapz((Ap^2 + Ac^2)/S, f) of site s; arms of the site accumulate
-// into acc[0..2].
-template <int MODEL>
-GWF_HD void snr_site(const gwf_waveform& w, const NetK& net, const double* rec, int res, int s, int arm0, int nA,
-                     int lane, int nlanes, double acc[3]) {
+// Per-arm trapz((Ap^2 + Ac^2)/S, f) of every arm of the network in one grid
+// walk: the amplitude, tau(f) and the PSD segments are evaluated once per
+// frequency and shared by the sites.  acc[a] accumulates physical arm a
+// (a < MAXA, the network's arm count rounded up to a compile-time bound).
+template <int MODEL, int MAXA = GWF_MAX_ARMS>
+GWF_HD void snr_event(const gwf_waveform& w, const NetK& net, const double* rec, int res, int lane, int nlanes,
+                      double acc[MAXA]) {
   const double* ev = rec + REC_EV;
-  const double* sc = rec + REC_SITE + s * SC_COUNT;
-  const SiteK& S = net.site[s];
   const double a = log10(net.fmin);
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
   const double tc = ev[EV_TC], idL = ev[EV_IDL], cp = ev[EV_CP], cc = ev[EV_CC];
-  const double P1 = sc[SC_P1], P2 = sc[SC_P2];
+  const bool earth = net.any_earth != 0;
   GridWalk gw;
   gw.init(a, b, res, lane, nlanes);
   for (int base = 0; base < res; base += nlanes) {
@@ -636,17 +636,40 @@ GWF_HD void snr_site(const gwf_waveform& w, const NetK& net, const double* rec, 
     double f, wt, lnf;
     gw.step(base, res, &f, &wt, &lnf);
     if (i >= res) continue;
+    PsdPre pre0, pre1;
+    psd_prefetch(net.psd[0], f, lnf, pre0);
+    if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
     const double Av = wf_amp_val<MODEL>(w, rec, f) * idL;
-    const double tau = (S.motion == MOTION_EARTH) ? tau_val(rec, MODEL, f) : 0.0;
+    const double tau = earth ? tau_val(rec, MODEL, f) : 0.0;
     double cdel, sdel;
-    earth_delta(S.motion == MOTION_EARTH, tau, &cdel, &sdel);
+    earth_delta(earth, tau, &cdel, &sdel);
+    int cur_site = -1, cur_psd = -1;
+    double wS = 0.0, P1 = 0.0, P2 = 0.0;
     Geo g;
-    site_geo<false>(S, sc, ev, site_tn(S.motion, tc, tau), cdel, sdel, g);
-    const double wS = wt / psd_at(net.psd[S.psd], f, lnf);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (k < nA) {
-        const ArmK& A = net.arm[arm0 + k];
+    for (int k = 0; k < MAXA; ++k) {
+      if (k < net.n_arms) {
+        const ArmK& A = net.arm[k];
+        const int s = A.site;
+        if (s != cur_site) {
+          const SiteK& S = net.site[s];
+          const double* sc = rec + REC_SITE + s * SC_COUNT;
+          if (S.psd != cur_psd) {
+            double Sf;
+            if (S.psd == 0)
+              Sf = psd_finish(net.psd[0], f, lnf, pre0);
+            else if (S.psd == 1)
+              Sf = psd_finish(net.psd[1], f, lnf, pre1);
+            else
+              Sf = psd_at(net.psd[S.psd], f, lnf);
+            wS = wt / Sf;
+            cur_psd = S.psd;
+          }
+          site_geo<false>(S, sc, ev, site_tn(S.motion, tc, tau), cdel, sdel, g);
+          P1 = sc[SC_P1];
+          P2 = sc[SC_P2];
+          cur_site = s;
+        }
         const double aa = A.s2x * g.Sa - A.c2x * g.Ca;
         const double bb = A.c2x * g.Cb + A.s2x * g.Sb;
         const double Ap = Av * (aa * P1 + bb * P2) * cp;
