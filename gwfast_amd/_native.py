@@ -76,6 +76,9 @@ SIGNATURES = {
                                       c_void_p]),
     'gwf_inversion_error': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
     'gwf_check_fisher': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'gwf_fisher_reparam': (ctypes.c_int, [c_void_p, c_int64, c_void_p, c_int32, c_int64, c_int32, c_int32, c_void_p]),
+    'gwf_fisher_block_transform': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32, c_void_p,
+                                                  c_void_p, c_void_p]),
     'gwf_fix_params': (ctypes.c_int, [c_void_p, c_int32, c_int64, ctypes.c_uint32, c_void_p, c_void_p, c_void_p]),
     'gwf_sigma_sky_area': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_int32, c_int32, c_double, c_int32,
                                           c_void_p, c_void_p, c_void_p]),

@@ -605,6 +605,85 @@ __global__ void k_sigma_area(const double* __restrict__ C, int P, int64_t n_ev, 
   }
 }
 
+// Fisher matrices in (m1, m2) instead of (Mc, eta) and / or (chiS, chiA)
+// instead of (chi1z, chi2z): F' = J^T F J with J the exact Jacobian of the
+// old parameters with respect to the new ones at the event (the reference
+// differentiates GWstrain in the new variables directly, signal.py:818-845;
+// the chain rule makes the two equal up to rounding).
+__global__ void k_reparam(const double* __restrict__ par, int64_t ld, double* __restrict__ F, int P, int64_t n_ev,
+                          int use_m1m2, int use_chiSA) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  // J restricted to the two 2x2 blocks: rows/cols (0, 1) and (9, 10)
+  double j00 = 1., j01 = 0., j10 = 0., j11 = 1.;
+  if (use_m1m2) {
+    const double Mc = par[e], eta = par[ld + e];
+    const double M = Mc / pow(eta, 3. / 5.);
+    const double Seta = sqrt(eta < 0.25 ? 1.0 - 4.0 * eta : 0.0);
+    const double m1 = 0.5 * M * (1.0 + Seta), m2 = 0.5 * M * (1.0 - Seta);
+    // d(Mc, eta) / d(m1, m2)
+    j00 = Mc * (3. / (5. * m1) - 1. / (5. * M));
+    j01 = Mc * (3. / (5. * m2) - 1. / (5. * M));
+    j10 = eta * (1. / m1 - 2. / M);
+    j11 = eta * (1. / m2 - 2. / M);
+  }
+  double s00 = 1., s01 = 0., s10 = 0., s11 = 1.;
+  if (use_chiSA) {  // chi1 = chiS + chiA, chi2 = chiS - chiA
+    s00 = 1.;
+    s01 = 1.;
+    s10 = 1.;
+    s11 = -1.;
+  }
+  auto at = [&](int a, int b) -> double& { return F[(int64_t)(a * P + b) * n_ev + e]; };
+  // right multiplication F J (columns), then left J^T (rows), block by block
+  for (int pass = 0; pass < 2; ++pass) {
+    const int c0 = pass == 0 ? 0 : 9;
+    const double a00 = pass == 0 ? j00 : s00, a01 = pass == 0 ? j01 : s01;
+    const double a10 = pass == 0 ? j10 : s10, a11 = pass == 0 ? j11 : s11;
+    if (a00 == 1. && a01 == 0. && a10 == 0. && a11 == 1.) continue;
+    for (int r = 0; r < P; ++r) {  // columns: new_c0 = F[r][c0] a00 + F[r][c0+1] a10, new_c1 = F[r][c0] a01 + F[r][c0+1] a11
+      const double x = at(r, c0), y = at(r, c0 + 1);
+      at(r, c0) = x * a00 + y * a10;
+      at(r, c0 + 1) = x * a01 + y * a11;
+    }
+    for (int c = 0; c < P; ++c) {  // rows
+      const double x = at(c0, c), y = at(c0 + 1, c);
+      at(c0, c) = a00 * x + a10 * y;
+      at(c0 + 1, c) = a01 * x + a11 * y;
+    }
+  }
+}
+
+// out = A F B per event, A and B the identity except the block on rows /
+// columns (i0, i1) (i1 < 0: the single entry i0), given per event as SoA
+// (4, n_ev): L = (A[i0][i0], A[i0][i1], A[i1][i0], A[i1][i1]), R likewise for B.
+// out may alias F: each thread owns one event.
+__global__ void k_block_transform(const double* __restrict__ F, double* out, int P, int64_t n_ev, int i0, int i1,
+                                  const double* __restrict__ L, const double* __restrict__ R) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_ev) return;
+  auto at = [&](int a, int b) -> double& { return out[(int64_t)(a * P + b) * n_ev + e]; };
+  if (F != out)
+    for (int k = 0; k < P * P; ++k) out[(int64_t)k * n_ev + e] = F[(int64_t)k * n_ev + e];
+  const double r00 = R[e], r01 = R[n_ev + e], r10 = R[2 * n_ev + e], r11 = R[3 * n_ev + e];
+  const double l00 = L[e], l01 = L[n_ev + e], l10 = L[2 * n_ev + e], l11 = L[3 * n_ev + e];
+  if (i1 < 0) {
+    for (int r = 0; r < P; ++r) at(r, i0) *= r00;
+    for (int c = 0; c < P; ++c) at(i0, c) *= l00;
+    return;
+  }
+  for (int r = 0; r < P; ++r) {  // G = F B
+    const double x = at(r, i0), y = at(r, i1);
+    at(r, i0) = x * r00 + y * r10;
+    at(r, i1) = x * r01 + y * r11;
+  }
+  for (int c = 0; c < P; ++c) {  // A G
+    const double x = at(i0, c), y = at(i1, c);
+    at(i0, c) = l00 * x + l01 * y;
+    at(i1, c) = l10 * x + l11 * y;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 int check_wf(const gwf_waveform* wf) {
   if (!wf) return fail(-EINVAL, "waveform description is NULL");
@@ -1017,6 +1096,29 @@ int gwf_sigma_sky_area(const double* C, int32_t P, int64_t n_ev, const double* t
   hipLaunchKernelGGL(k_sigma_area, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, C, (int)P,
                      n_ev, theta, (int)i_theta, (int)i_phi, scale, sigma_out, area_out);
   return hip_check(hipGetLastError(), "gwf_sigma_sky_area launch");
+}
+
+int gwf_fisher_block_transform(const double* F, double* out, int32_t P, int64_t n_ev, int32_t i0, int32_t i1,
+                               const double* L, const double* R, void* stream) {
+  if (P < 1 || P > 64) return fail(-EINVAL, "P must be in [1, 64]");
+  if (n_ev < 0) return fail(-EINVAL, "invalid sizes");
+  if (i0 < 0 || i0 >= P || i1 >= P || i1 == i0) return fail(-EINVAL, "block indices out of range");
+  if (n_ev == 0) return 0;
+  if (!F || !out || !L || !R) return fail(-EINVAL, "NULL array");
+  hipLaunchKernelGGL(k_block_transform, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, F, out,
+                     (int)P, n_ev, (int)i0, (int)(i1 < 0 ? -1 : i1), L, R);
+  return hip_check(hipGetLastError(), "gwf_fisher_block_transform launch");
+}
+
+int gwf_fisher_reparam(const double* params, int64_t ld, double* F, int32_t P, int64_t n_ev, int32_t use_m1m2,
+                       int32_t use_chiSA, void* stream) {
+  if (P != 11 && P != 13) return fail(-EINVAL, "P must be 11 or 13");
+  if (n_ev < 0 || ld < n_ev) return fail(-EINVAL, "invalid sizes");
+  if (n_ev == 0 || (!use_m1m2 && !use_chiSA)) return 0;
+  if (!params || !F) return fail(-EINVAL, "NULL array");
+  hipLaunchKernelGGL(k_reparam, dim3((unsigned)((n_ev + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params, ld, F,
+                     (int)P, n_ev, (int)use_m1m2, (int)use_chiSA);
+  return hip_check(hipGetLastError(), "gwf_fisher_reparam launch");
 }
 
 }  // extern "C"

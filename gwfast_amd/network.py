@@ -18,6 +18,21 @@ class DetNet(object):
         self.signals = signals
         self.verbose = verbose
 
+    def _clear_cache(self):
+        """network.py:31-33."""
+        for sig in self.signals.values():
+            sig._clear_cache()
+
+    def _update_all_seeds(self, seeds=[], verbose=True):
+        """network.py:36-51: reseed the duty-cycle draw of each detector
+        (None draws a fresh seed)."""
+        if seeds == []:
+            seeds = [None] * len(self.signals)
+        for i, (name, sig) in enumerate(self.signals.items()):
+            sig._update_seed(seed=seeds[i])
+            if verbose:
+                print('\nSeed for detector %s is %s' % (name, sig.seedUse))
+
     def _groups(self):
         """Detectors that can share one launch (same model object, fmin, fmax)."""
         groups = []
@@ -49,6 +64,9 @@ class DetNet(object):
             for name, s in g:
                 part = sq[k:k + s.n_arms]
                 k += s.n_arms
+                masks = s._duty_masks(part.shape[1])
+                if masks is not None:
+                    part = part * masks
                 if s.detector_shape == 'T' and return_all:
                     for i in range(3):
                         snrs[name + '_%s' % i] = 2 * np.sqrt(part[i])
@@ -66,13 +84,16 @@ class DetNet(object):
         self._prepare(evParams)
         res = kwargs.pop('res', 1000)
         res = int(res) if np.ndim(res) == 0 else np.asarray(res)
-        S._check_fisher_options(res, kwargs.get('df'), kwargs.get('spacing', 'geom'), kwargs.get('use_m1m2', False),
-                                kwargs.get('use_chi1chi2', True), kwargs.get('computeDerivFinDiff', False))
+        use_m1m2 = kwargs.get('use_m1m2', False)
+        use_chi1chi2 = kwargs.get('use_chi1chi2', True)
+        S._check_fisher_options(res, kwargs.get('df'), kwargs.get('spacing', 'geom'), use_m1m2, use_chi1chi2,
+                                kwargs.get('computeDerivFinDiff', False))
+        opts = {'use_m1m2': use_m1m2, 'use_chi1chi2': use_chi1chi2}
         if return_derivatives or return_SNR_derivatives:
             allF, allD = {}, {}
             for name, sig in self.signals.items():
                 F_, D_ = sig.FisherMatr(evParams, res=res, return_derivatives=return_derivatives,
-                                        return_SNR_derivatives=return_SNR_derivatives)
+                                        return_SNR_derivatives=return_SNR_derivatives, **opts)
                 if sig.detector_shape == 'T':
                     for i in range(3):
                         allF[name + '_%s' % i] = F_[i]
@@ -85,23 +106,24 @@ class DetNet(object):
                 allD['net'] = np.array([allD[k] for k in allD.keys()]).sum(axis=0) / SNRs_net
             allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
             return allF, allD
-        if return_all:
+        if return_all or any(sig.DutyFactor is not None for sig in self.signals.values()):
             allF = {}
             for name, sig in self.signals.items():
-                F_ = sig.FisherMatr(evParams, res=res, return_all=True)
+                F_ = sig.FisherMatr(evParams, res=res, return_all=True, **opts)
                 if sig.detector_shape == 'T':
                     for i in range(3):
                         allF[name + '_%s' % i] = F_[i]
                 else:
                     allF[name] = F_[0]
             allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
-            return allF
-        total = None
+            return allF if return_all else allF['net']
+        total, rows = None, None
         for g in self._groups():
             sigs = [s for _, s in g]
             rows = sigs[0]._rows(evParams)
             F = S.fisher_device(sigs, sigs[0].wf_model, rows, res)
             total = F if total is None else total + F
+        S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
         return total.cpu().numpy()
 
     # ------------------------------------------------------- device-resident

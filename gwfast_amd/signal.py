@@ -23,8 +23,8 @@ class GWSignal(object):
             raise ValueError('Enter valid detector configuration')
         if psd_path is None:
             raise ValueError('Enter a valid PSD or ASD path')
-        if DutyFactor is not None:
-            raise NotImplementedError('DutyFactor masks are not supported on this path')
+        if DutyFactor is not None and not (0. <= DutyFactor <= 1.):
+            raise ValueError('DutyFactor must be between 0 and 1')
         self.wf_model = wf_model
         self.verbose = verbose
         self.detector_shape = detector_shape
@@ -48,7 +48,31 @@ class GWSignal(object):
         self.angbtwArms = 0.5 * np.pi if detector_shape == 'L' else np.pi / 3.
         self.compute2arms = compute2arms
         self.jitCompileDerivs = jitCompileDerivs
+        # duty-cycle seed (signal.py:152-153): numpy's global generator, as the reference
+        np.random.seed(None)
+        self.seedUse = np.random.randint(2 ** 32 - 1, size=1)
         self._psd_dev = None
+
+    def _clear_cache(self):
+        """signal.py:216-219 rebuilds compiled derivative functions; here the device
+        copy of the PSD table is what is cached."""
+        self._psd_dev = None
+
+    def _update_seed(self, seed=None):
+        """signal.py:221-232."""
+        np.random.seed(None)
+        self.seedUse = np.random.randint(2 ** 32 - 1, size=1) if seed is None else seed
+
+    def _duty_masks(self, n):
+        """The reference's duty-cycle masks (signal.py:672-673, 729-761,
+        932-936): numpy's global generator reseeded with seedUse, then one
+        Bernoulli(DutyFactor) draw of n per arm, in arm order.  None without a
+        DutyFactor."""
+        if self.DutyFactor is None:
+            return None
+        np.random.seed(self.seedUse)
+        return np.array([np.random.choice([0, 1], n, p=[1. - self.DutyFactor, self.DutyFactor])
+                         for _ in range(self.n_arms)], dtype=np.float64)
 
     # ------------------------------------------------------------------ ABI
     def c_struct(self):
@@ -95,9 +119,12 @@ class GWSignal(object):
 
     # --------------------------------------------------------------- public
     def SNRInteg(self, evParams, res=1000, return_all=False):
-        """signal.py:658-777."""
+        """signal.py:658-777 (with the DutyFactor masks when set)."""
         self._complete(evParams)
         snrsq = snr_sq_device([self], self._rows(evParams), res).cpu().numpy()
+        masks = self._duty_masks(snrsq.shape[1])
+        if masks is not None:
+            snrsq = snrsq * masks
         if return_all:
             return 2 * np.sqrt(snrsq) if self.detector_shape == 'T' else 2 * np.sqrt(snrsq[0])
         return 2 * np.sqrt(snrsq.sum(axis=0))
@@ -111,13 +138,30 @@ class GWSignal(object):
         _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff)
         self._complete(evParams)
         rows = self._rows(evParams)
+        n = rows.shape[1]
+        masks = self._duty_masks(n)
+        P = self.wf_model.nParams
         if return_derivatives or return_SNR_derivatives:
             if np.ndim(res) != 0:
                 raise NotImplementedError('derivative outputs need one grid length for all events')
-            return self._fisher_with_derivs(evParams, rows, int(res), return_derivatives)
-        if return_all:
-            return [fisher_device([self], self.wf_model, rows, res, arm=k).cpu().numpy() for k in range(self.n_arms)]
-        return fisher_device([self], self.wf_model, rows, res).cpu().numpy()
+            allF, allD = self._fisher_with_derivs(evParams, rows, int(res), return_derivatives)
+            for k in range(len(allF)):
+                allF[k] = reparam_host(allF[k], evParams, use_m1m2, use_chi1chi2)
+                allD[k] = reparam_rows_host(allD[k], evParams, use_m1m2, use_chi1chi2)
+                if masks is not None:
+                    allF[k] = allF[k] * masks[k]
+                    allD[k] = allD[k] * (masks[k][None, :, None] if allD[k].ndim == 3 else masks[k][None, :])
+            return allF, allD
+        if return_all or masks is not None:
+            Fs = []
+            for k in range(self.n_arms):
+                F = fisher_device([self], self.wf_model, rows, res, arm=k)
+                reparam_device(F, rows, use_m1m2, use_chi1chi2)
+                Fs.append(F.cpu().numpy() * (masks[k] if masks is not None else 1.0))
+            return Fs if return_all else np.sum(Fs, axis=0)
+        F = fisher_device([self], self.wf_model, rows, res)
+        reparam_device(F, rows, use_m1m2, use_chi1chi2)
+        return F.cpu().numpy()
 
     def _fisher_with_derivs(self, evParams, rows, res, return_derivatives):
         n = rows.shape[1]
@@ -191,13 +235,68 @@ def _shared_psd(f, S):
     return hit[0], hit[1]
 
 
+def reparam_device(F, rows, use_m1m2, use_chi1chi2):
+    """In place: Fisher (P, P, N) device tensor into (m1, m2) and / or
+    (chiS, chiA) rows (signal.py:818-845), by the exact Jacobian."""
+    if not use_m1m2 and use_chi1chi2:
+        return F
+    E.call('gwf_fisher_reparam', N.ptr(rows), rows.stride(0), N.ptr(F), F.shape[0], F.shape[2], int(bool(use_m1m2)),
+           int(not use_chi1chi2), N.stream_ptr())
+    return F
+
+
+def _jac_blocks(evParams, use_m1m2, use_chi1chi2):
+    """Host copies of the 2x2 Jacobian blocks used by gwf_fisher_reparam."""
+    Mc = np.atleast_1d(np.asarray(evParams['Mc'], dtype=np.float64))
+    eta = np.atleast_1d(np.asarray(evParams['eta'], dtype=np.float64))
+    n = Mc.shape[0]
+    A = np.zeros((2, 2, n))
+    A[0, 0] = A[1, 1] = 1.
+    if use_m1m2:
+        M = Mc / eta ** (3. / 5.)
+        Seta = np.sqrt(np.where(eta < 0.25, 1.0 - 4.0 * eta, 0.))
+        m1, m2 = 0.5 * M * (1.0 + Seta), 0.5 * M * (1.0 - Seta)
+        A = np.array([[Mc * (3. / (5. * m1) - 1. / (5. * M)), Mc * (3. / (5. * m2) - 1. / (5. * M))],
+                      [eta * (1. / m1 - 2. / M), eta * (1. / m2 - 2. / M)]])
+    B = np.zeros((2, 2, n))
+    B[0, 0] = B[1, 1] = 1.
+    if not use_chi1chi2:
+        B[0, 0], B[0, 1], B[1, 0], B[1, 1] = 1., 1., 1., -1.
+    return A, B
+
+
+def reparam_host(F, evParams, use_m1m2, use_chi1chi2):
+    """The same change of variables on a host Fisher (P, P, N)."""
+    if not use_m1m2 and use_chi1chi2:
+        return F
+    A, B = _jac_blocks(evParams, use_m1m2, use_chi1chi2)
+    J = np.zeros((F.shape[0], F.shape[0], F.shape[2]))
+    for i in range(F.shape[0]):
+        J[i, i] = 1.
+    J[0:2, 0:2] = A
+    J[9:11, 9:11] = B
+    return np.einsum('kin,kln,ljn->ijn', J, F, J)
+
+
+def reparam_rows_host(D, evParams, use_m1m2, use_chi1chi2):
+    """Derivative rows (P, N[, N_f]) in the new variables: D'_j = sum_k J_kj D_k."""
+    if not use_m1m2 and use_chi1chi2:
+        return D
+    A, B = _jac_blocks(evParams, use_m1m2, use_chi1chi2)
+    out = np.array(D, copy=True)
+    ex = (lambda a: a[:, None]) if D.ndim == 3 else (lambda a: a)
+    for (r0, blk) in ((0, A), (9, B)):
+        d0, d1 = D[r0], D[r0 + 1]
+        out[r0] = ex(blk[0, 0]) * d0 + ex(blk[1, 0]) * d1
+        out[r0 + 1] = ex(blk[0, 1]) * d0 + ex(blk[1, 1]) * d1
+    return out
+
+
 def _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff):
     if res is None or (np.ndim(res) == 0 and int(res) < 2):
         raise NotImplementedError('df-spaced grids are not supported: pass res')
     if spacing != 'geom':
         raise NotImplementedError("only spacing='geom' is supported")
-    if use_m1m2 or not use_chi1chi2:
-        raise NotImplementedError('Fisher matrices are computed in (Mc, eta, chi1z, chi2z)')
     if computeDerivFinDiff:
         raise NotImplementedError('finite-difference derivatives are not provided (derivatives are exact)')
 

@@ -285,6 +285,24 @@ int gwf_select(const int8_t* cls, int64_t n, int32_t want, int64_t* idx_out, int
 int gwf_gather_events(const double* params, int64_t ld, int32_t n_rows, const int64_t* idx, int64_t n_idx,
                       double* out, int64_t ld_out, void* stream);
 
+/* GWSignal.FisherMatr(use_m1m2=True and / or use_chi1chi2=False)
+ * (signal.py:818-845): rows / columns 0, 1 of F (P, P, n_ev) turned from
+ * (Mc, eta) into the detector-frame (m1, m2), and 9, 10 from (chi1z, chi2z)
+ * into (chiS, chiA), in place: F' = J^T F J with the exact Jacobian at each
+ * event's params (Fisher-order rows of stride ld). */
+int gwf_fisher_reparam(const double* params, int64_t ld, double* F, int32_t P, int64_t n_ev, int32_t use_m1m2,
+                       int32_t use_chiSA, void* stream);
+
+/* The matrix variable changes of fisherTools.py:410-459 (log dL <-> dL) and
+ * 620-822 (Mc, eta <-> m1, m2; chi1z, chi2z -> chi_eff, Delta chi;
+ * chiS, chiA -> chi1z, chi2z), batched over events: out = A F B for
+ * F (P, P, n_ev), A and B the identity except on rows / columns (i0, i1),
+ * where they hold the per-event blocks L and R, SoA (4, n_ev) in the order
+ * [0][0], [0][1], [1][0], [1][1].  i1 = -1 scales row / column i0 alone by
+ * L[0] / R[0].  out may be F. */
+int gwf_fisher_block_transform(const double* F, double* out, int32_t P, int64_t n_ev, int32_t i0, int32_t i1,
+                               const double* L, const double* R, void* stream);
+
 /* ------------------------------------------------------------------------
  * Post-Fisher batch (the catalogue driver's consumers of CovMatr,
  * calculate_forecasts_from_catalog.py:573-630, 775).

@@ -114,3 +114,58 @@ def localization_region(C, par_nums, theta, perc_level=90, units='SqDeg'):
     base = 2 * np.pi * np.sqrt(tt * pp - tp ** 2) * np.abs(np.sin(theta))
     sr = -base * np.log(1 - perc_level / 100)
     return sr if units == 'Sterad' else (180 / np.pi) ** 2 * sr
+
+
+# ------------------------------------------------ matrix variable changes
+def jac_m1m2_mceta(Mc, eta):
+    """fisherTools.py:461-510, 566-576: d(m1, m2)/d(Mc, eta), (2, 2[, N])."""
+    s = np.sqrt(1 - 4 * eta)
+    return np.array([[(1 + s) * eta ** (-0.6) / 2, -Mc * (3 - 2 * eta + 3 * s) / (10 * s * eta ** 1.6)],
+                     [(1 - s) * eta ** (-0.6) / 2, -Mc * (-3 + 2 * eta + 3 * s) / (10 * s * eta ** 1.6)]])
+
+
+def jac_mceta_m1m2(m1, m2):
+    """fisherTools.py:513-562, 579-589: d(Mc, eta)/d(m1, m2), (2, 2[, N])."""
+    M = m1 + m2
+    dmc1 = m2 * (2 * m1 + 3 * m2) / (5 * (m1 * m2) ** 0.4 * M ** 1.2)
+    dmc2 = m1 * (2 * m2 + 3 * m1) / (5 * (m1 * m2) ** 0.4 * M ** 1.2)
+    return np.array([[dmc1, dmc2], [m2 * (m2 - m1) / M ** 3, m1 * (m1 - m2) / M ** 3]])
+
+
+def m1m2_from_mceta_unguarded(Mc, eta):
+    """fisherTools.py:592-603."""
+    d = np.sqrt(1 - 4 * eta)
+    return (1 + d) / 2 * Mc / eta ** 0.6, (1 - d) / 2 * Mc / eta ** 0.6
+
+
+def jac_chi1chi2_chieff_dchi(m1, m2):
+    """fisherTools.py:712-774: d(chi1z, chi2z)/d(chi_eff, Delta chi)."""
+    one = np.ones_like(np.asarray(m1 + m2, dtype=float))
+    return np.array([[one, m2 / (m1 + m2)], [one, -m1 / (m1 + m2)]])
+
+
+def sandwich(M, i0, i1, left, right):
+    """A M B for each event, A and B the identity with the 2x2 blocks left /
+    right (2, 2[, N]) on rows and columns (i0, i1): the matrix products of
+    fisherTools.py:620-822, written as explicit per-event matrices."""
+    M = np.asarray(M, dtype=float)
+    two_d = M.ndim == 2
+    M3 = M[:, :, None] if two_d else M
+    P, _, n = M3.shape
+    out = np.empty_like(M3)
+    for e in range(n):
+        A, B = np.eye(P), np.eye(P)
+        for (a, ia) in ((0, i0), (1, i1)):
+            for (b, ib) in ((0, i0), (1, i1)):
+                A[ia, ib] = np.broadcast_to(left[a][b], (n,))[e]
+                B[ia, ib] = np.broadcast_to(right[a][b], (n,))[e]
+        out[:, :, e] = A @ M3[:, :, e] @ B
+    return out[:, :, 0] if two_d else out
+
+
+def scale_row_col(M, i, s):
+    """fisherTools.py:410-458: row and column i times s (per event)."""
+    out = np.array(M, dtype=float, copy=True)
+    out[:, i] *= s
+    out[i, :] *= s
+    return out

@@ -146,17 +146,16 @@ class Detector:
         return np.squeeze(2 * np.sqrt(allSNRsq), axis=0)
 
     # -- derivatives -------------------------------------------------------------
-    def ad_derivatives(self, wf, ev, fgrids, rot=0.):
+    def ad_derivatives(self, wf, ev, fgrids, rot=0., use_m1m2=False, use_chi1chi2=True):
         """AD rows of signal.py:1143-1189 via forward-mode duals: derivatives
         of GWstrain w.r.t. Mc, eta, chi1z, chi2z (+ LambdaTilde, deltaLambda
-        for tidal models).  Returns complex array (n_AD, N_ev, N_f)."""
+        for tidal models), or w.r.t. m1, m2 (use_m1m2, signal.py:818-822,
+        522-524) and chiS, chiA (use_chi1chi2=False, signal.py:527-533).
+        Returns complex array (n_AD, N_ev, N_f)."""
         tidal = wf.is_tidal
         K = 6 if tidal else 4
         evd = dict(ev)
-        evd['Mc'] = D.Dual.seed(ev['Mc'], 0, K)
-        evd['eta'] = D.Dual.seed(ev['eta'], 1, K)
-        evd['chi1z'] = D.Dual.seed(ev['chi1z'], 2, K)
-        evd['chi2z'] = D.Dual.seed(ev['chi2z'], 3, K)
+        _seed_masses_spins(evd, ev, K, use_m1m2, use_chi1chi2)
         if tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             Ltd = D.Dual.seed(Lt, 4, K)
@@ -273,14 +272,14 @@ class Detector:
         Phicoal_deriv = -1j * (hp + hc)
         return dL_deriv, theta_deriv, phi_deriv, iota_deriv, psi_deriv, tc_deriv, Phicoal_deriv
 
-    def signal_derivatives(self, wf, ev, fgrids, rot=0.):
+    def signal_derivatives(self, wf, ev, fgrids, rot=0., use_m1m2=False, use_chi1chi2=True):
         """(P, N_ev, N_f) complex in ParNums order (signal.py:1283-1299)."""
-        ad = self.ad_derivatives(wf, ev, fgrids, rot=rot)
+        ad = self.ad_derivatives(wf, ev, fgrids, rot=rot, use_m1m2=use_m1m2, use_chi1chi2=use_chi1chi2)
         an = self.analytic_derivatives(wf, ev, fgrids, rot=rot)
         rows = [ad[0], ad[1]] + [np.asarray(a).T for a in an] + [ad[k] for k in range(2, ad.shape[0])]
         return np.array(rows)
 
-    def full_dual_derivatives(self, wf, ev, fgrids, rot=0.):
+    def full_dual_derivatives(self, wf, ev, fgrids, rot=0., use_m1m2=False, use_chi1chi2=True):
         """All P rows by forward-mode duals of GWstrain (no closed forms):
         used to pin the closed-form rows.  Returns (P, N_ev, N_f)."""
         P = wf.nParams
@@ -288,6 +287,7 @@ class Detector:
         evd = dict(ev)
         for k, key in enumerate(keys):
             evd[key] = D.Dual.seed(ev[key], k, P)
+        _seed_masses_spins(evd, ev, P, use_m1m2, use_chi1chi2, slots=(0, 1, 9, 10))
         if wf.is_tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             L1, L2 = lam12_from_lamt_dellam(D.Dual.seed(Lt, 11, P), D.Dual.seed(dLm, 12, P), evd['eta'])
@@ -306,7 +306,7 @@ class Detector:
                 F[b, a] = F[a, b]
         return F
 
-    def fisher_arms(self, wf, ev, res=1000, mode='faithful'):
+    def fisher_arms(self, wf, ev, res=1000, mode='faithful', use_m1m2=False, use_chi1chi2=True):
         """Per-arm Fisher matrices (signal.py:782-1099), list of (P, P, N_ev).
 
         mode='faithful' follows the reference's derivative construction (AD
@@ -314,7 +314,10 @@ class Detector:
         association); mode='dual' takes every row from duals of GWstrain."""
         fgrids, strainGrids = self.grid(wf, ev, res)
         P = wf.nParams
-        deriv = self.signal_derivatives if mode == 'faithful' else self.full_dual_derivatives
+        base = self.signal_derivatives if mode == 'faithful' else self.full_dual_derivatives
+
+        def deriv(wf_, ev_, fg_, rot=0.):
+            return base(wf_, ev_, fg_, rot=rot, use_m1m2=use_m1m2, use_chi1chi2=use_chi1chi2)
         if self.detector_shape == 'L':
             return [self._fisher_from_derivs(deriv(wf, ev, fgrids), strainGrids, fgrids, P)]
         if not self.compute2arms:
@@ -324,8 +327,34 @@ class Detector:
         d3 = -(d1 + d2)
         return [self._fisher_from_derivs(d, strainGrids, fgrids, P) for d in (d1, d2, d3)]
 
-    def fisher(self, wf, ev, res=1000, mode='faithful'):
-        return np.sum(self.fisher_arms(wf, ev, res, mode), axis=0)
+    def fisher(self, wf, ev, res=1000, mode='faithful', **kw):
+        return np.sum(self.fisher_arms(wf, ev, res, mode, **kw), axis=0)
+
+
+def _seed_masses_spins(evd, ev, K, use_m1m2, use_chi1chi2, slots=(0, 1, 2, 3)):
+    """Seed the dual variables of the mass and spin rows: (Mc, eta), or
+    (m1, m2) mapped to Mc, eta as gwfastUtils.py:450-479 does, and
+    (chi1z, chi2z), or (chiS, chiA) with chi1z = chiS + chiA,
+    chi2z = chiS - chiA (signal.py:527-533)."""
+    s0, s1, s2, s3 = slots
+    if use_m1m2:
+        eta = np.asarray(ev['eta'], dtype=float)
+        Seta = np.sqrt(np.where(eta < 0.25, 1.0 - 4.0 * eta, 0.))
+        m1 = 0.5 * (ev['Mc'] / (eta ** (3. / 5.))) * (1. + Seta)
+        m2 = 0.5 * (ev['Mc'] / (eta ** (3. / 5.))) * (1. - Seta)
+        m1d, m2d = D.Dual.seed(m1, s0, K), D.Dual.seed(m2, s1, K)
+        evd['Mc'] = ((m1d * m2d) ** (3. / 5.)) / ((m1d + m2d) ** (1. / 5.))
+        evd['eta'] = (m1d * m2d) / ((m1d + m2d) * (m1d + m2d))
+    else:
+        evd['Mc'] = D.Dual.seed(ev['Mc'], s0, K)
+        evd['eta'] = D.Dual.seed(ev['eta'], s1, K)
+    if use_chi1chi2:
+        evd['chi1z'] = D.Dual.seed(ev['chi1z'], s2, K)
+        evd['chi2z'] = D.Dual.seed(ev['chi2z'], s3, K)
+    else:
+        cS = D.Dual.seed(0.5 * (ev['chi1z'] + ev['chi2z']), s2, K)
+        cA = D.Dual.seed(0.5 * (ev['chi1z'] - ev['chi2z']), s3, K)
+        evd['chi1z'], evd['chi2z'] = cS + cA, cS - cA
 
 
 def geomspace_jax(start, stop, num):
@@ -348,6 +377,7 @@ def network_snr(dets, wf, ev, res=1000):
     return np.sqrt(np.array([s ** 2 for s in snrs]).sum(axis=0))
 
 
-def network_fisher(dets, wf, ev, res=1000, mode='faithful'):
-    """network.py:84-152: sum of detector Fishers, (P, P, N_ev)."""
-    return np.array([d.fisher(wf, ev, res, mode) for d in dets]).sum(axis=0)
+def network_fisher(dets, wf, ev, res=1000, mode='faithful', **kw):
+    """network.py:84-152: sum of detector Fishers, (P, P, N_ev); kw carries
+    use_m1m2 / use_chi1chi2 (signal.py:782-840)."""
+    return np.array([d.fisher(wf, ev, res, mode, **kw) for d in dets]).sum(axis=0)

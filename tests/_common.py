@@ -20,7 +20,7 @@ def oracle_net(sites, earth, fmin=2., fmax=None, compute2arms=True):
     return out
 
 
-def hip_net(wf, sites, earth, fmin=2., fmax=None, compute2arms=True):
+def hip_net(wf, sites, earth, fmin=2., fmax=None, compute2arms=True, DutyFactor=None):
     from gwfast_amd.network import DetNet
     from gwfast_amd.signal import GWSignal
     sigs = {}
@@ -28,7 +28,7 @@ def hip_net(wf, sites, earth, fmin=2., fmax=None, compute2arms=True):
         d = glob.detectors[name]
         sigs[name] = GWSignal(wf, psd_path=psd, detector_shape=d['shape'], det_lat=d['lat'], det_long=d['long'],
                               det_xax=d['xax'], useEarthMotion=earth, fmin=fmin, fmax=fmax,
-                              compute2arms=compute2arms, verbose=False)
+                              compute2arms=compute2arms, DutyFactor=DutyFactor, verbose=False)
     return DetNet(sigs, verbose=False)
 
 

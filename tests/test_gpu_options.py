@@ -1,0 +1,147 @@
+"""GPU parity of the FisherMatr variable changes and of the duty cycle.
+
+use_m1m2 / use_chi1chi2=False (gwfast/signal.py:818-845, 522-533): the
+device applies F' = J^T F J with the exact Jacobian of (m1, m2) -> (Mc, eta)
+and (chiS, chiA) -> (chi1z, chi2z); the oracle seeds its dual numbers directly
+on m1, m2, chiS, chiA, so the two agree to rounding (<= 1e-9 normalised).
+
+DutyFactor (signal.py:672-761, 809-1073; network.py:36-51): each detector
+reseeds numpy's global generator with its seedUse and draws one Bernoulli
+mask per arm, in arm order; a masked arm contributes nothing to SNR^2 or to
+the Fisher matrix.
+"""
+import numpy as np
+import pytest
+
+from tests._common import NET_C1, bbh, bns, hip_net, norm_err, oracle_net
+from tests.test_gpu_parity import _models
+
+pytestmark = pytest.mark.gpu
+
+VARIABLE_SETS = [(True, True), (False, False), (True, False)]
+
+
+@pytest.mark.parametrize('model', ['PD', 'NRT'])
+def test_fisher_variable_change_parity(gpu, model):
+    from oracle import detector as OD
+    wf, owf = _models(model)
+    ev = (bbh if model == 'PD' else bns)(5, seed=21)
+    net = hip_net(wf, NET_C1, True)
+    odets = oracle_net(NET_C1, True)
+    for use_m1m2, use_chi1chi2 in VARIABLE_SETS:
+        F = net.FisherMatr(dict(ev), res=800, use_m1m2=use_m1m2, use_chi1chi2=use_chi1chi2)
+        Fd = OD.network_fisher(odets, owf, ev, 800, mode='dual', use_m1m2=use_m1m2, use_chi1chi2=use_chi1chi2)
+        e = norm_err(F, Fd)
+        print(model, 'use_m1m2', use_m1m2, 'use_chi1chi2', use_chi1chi2, 'err %.3e' % e, flush=True)
+        assert F.shape == (wf.nParams, wf.nParams, 5)
+        assert e <= 1e-9
+
+
+def test_derivative_rows_in_m1m2_chiSA(gpu):
+    """FisherMatr(return_derivatives=True, use_m1m2=True, use_chi1chi2=False)
+    rows against the oracle's duals in the new variables."""
+    from oracle import detector as OD
+    wf, owf = _models('PD')
+    ev = bbh(4, seed=22)
+    sites = NET_C1[1:2]
+    sig = hip_net(wf, sites, True).signals['CE1Id']
+    od = oracle_net(sites, True)[0]
+    res = 300
+    fgrids, _ = od.grid(owf, ev, res)
+    Fs, Ds = sig.FisherMatr(dict(ev), res=res, return_derivatives=True, use_m1m2=True, use_chi1chi2=False)
+    Do = od.full_dual_derivatives(owf, ev, fgrids, use_m1m2=True, use_chi1chi2=False)
+    Do[7] /= 86400.
+    scale = np.max(np.abs(Do), axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(Ds[0] - Do) / scale) <= 1e-5
+    Fo = od.fisher(owf, ev, res, mode='dual', use_m1m2=True, use_chi1chi2=False)
+    assert norm_err(Fs[0], Fo) <= 1e-9
+
+
+def _masks(seed, n_arms, n, p):
+    np.random.seed(seed)
+    return [np.random.choice([0, 1], n, p=[1. - p, p]) for _ in range(n_arms)]
+
+
+def test_duty_factor_snr_and_fisher(gpu):
+    wf, _ = _models('PD')
+    n, res, p = 40, 400, 0.5
+    ev = bbh(n, seed=31)
+    base = hip_net(wf, NET_C1, True)
+    duty = hip_net(wf, NET_C1, True, DutyFactor=p)
+    seeds = [101, 202, 303]
+    duty._update_all_seeds(seeds, verbose=False)
+    arms = {'ETS': ['ETS_0', 'ETS_1', 'ETS_2'], 'CE1Id': ['CE1Id'], 'CE2NM': ['CE2NM']}
+    want_mask = {}
+    for (name, _), seed in zip(NET_C1, seeds):
+        for key, m in zip(arms[name], _masks(seed, len(arms[name]), n, p)):
+            want_mask[key] = m.astype(float)
+    allm = np.array(list(want_mask.values()))
+    assert 0 < allm.mean() < 1
+
+    s0 = base.SNR(dict(ev), res=res, return_all=True)
+    s1 = duty.SNR(dict(ev), res=res, return_all=True)
+    for key, m in want_mask.items():
+        assert np.array_equal(s1[key], s0[key] * m), key
+    want_net = np.sqrt(sum((s0[k] * want_mask[k]) ** 2 for k in want_mask))
+    assert np.max(np.abs(s1['net'] - want_net) / np.maximum(want_net, 1e-300)) <= 1e-14
+    again = duty.SNR(dict(ev), res=res)  # the same seeds, the same masks
+    assert np.max(np.abs(again - s1['net']) / np.maximum(s1['net'], 1e-300)) <= 1e-14
+
+    F0 = base.FisherMatr(dict(ev), res=res, return_all=True)
+    F1 = duty.FisherMatr(dict(ev), res=res)
+    want = sum(F0[k] * want_mask[k] for k in want_mask)
+    live = np.einsum('iin->n', want) > 0
+    assert np.all(F1[:, :, ~live] == 0)
+    assert norm_err(F1[:, :, live], want[:, :, live]) <= 1e-13
+
+    sig = duty.signals['ETS']
+    s_sig = sig.SNRInteg(dict(ev), res=res, return_all=True)
+    for i in range(3):
+        assert np.max(np.abs(s_sig[i] - s1['ETS_%d' % i])) <= 1e-13 * np.max(s1['ETS_%d' % i])
+
+
+def _spd(P, n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, P, P))
+    return np.transpose(A @ np.transpose(A, (0, 2, 1)) + P * np.eye(P), (1, 2, 0))
+
+
+def test_fishertools_variable_changes(gpu):
+    """fisherTools.py:410-458, 620-822 on the device against the oracle's
+    per-event matrix products (batched (P, P, N) and single (P, P) with
+    scalar parameters)."""
+    from gwfast_amd import fisherTools as FT
+    from gwfast_amd.waveforms import IMRPhenomD
+    from oracle import covariance as OC
+    ev = bbh(7, seed=41)
+    pn = IMRPhenomD().ParNums
+    M = _spd(11, 7, 3)
+    Mc, eta = ev['Mc'], ev['eta']
+    m1, m2 = OC.m1m2_from_mceta_unguarded(Mc, eta)
+    Jme, Jmm = OC.jac_m1m2_mceta(Mc, eta), OC.jac_mceta_m1m2(m1, m2)
+    Jch = OC.jac_chi1chi2_chieff_dchi(m1, m2)
+    T = lambda J: np.swapaxes(J, 0, 1)
+    pn_s = dict(pn)
+    pn_s['chiS'], pn_s['chiA'] = pn_s.pop('chi1z'), pn_s.pop('chi2z')
+    half = np.array([[.5, .5], [.5, -.5]])
+    cases = [
+        (FT.m1m2_to_Mceta_fish, pn, OC.sandwich(M, 0, 1, Jme, Jme)),
+        (FT.m1m2_to_Mceta_cov, pn, OC.sandwich(M, 0, 1, Jmm, Jmm)),
+        (FT.Mceta_to_m1m2_fish, pn, OC.sandwich(M, 0, 1, T(Jmm), Jmm)),
+        (FT.Mceta_to_m1m2_cov, pn, OC.sandwich(M, 0, 1, T(Jme), Jme)),
+        (FT.chi1chi2_to_chieffDeltachi_fish, pn, OC.sandwich(M, 9, 10, T(Jch), Jch)),
+        (FT.chiSchiA_to_chi1chi2_fish, pn_s, OC.sandwich(M, 9, 10, half, half)),
+        (FT.log_dL_to_dL_derivative_fish, pn, OC.scale_row_col(M, 2, 1. / ev['dL'])),
+        (FT.log_dL_to_dL_derivative_cov, pn, OC.scale_row_col(M, 2, ev['dL'])),
+    ]
+    for fn, parnums, want in cases:
+        got = fn(M, parnums, ev)
+        err = np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300))
+        print(fn.__name__, 'max rel err %.3e' % err, flush=True)
+        assert err <= 1e-13, fn.__name__
+        one = {k: float(v[2]) for k, v in ev.items()}
+        got1 = fn(M[:, :, 2], parnums, one)
+        assert got1.shape == (11, 11)
+        assert np.max(np.abs(got1 - want[:, :, 2]) / np.maximum(np.abs(want[:, :, 2]), 1e-300)) <= 1e-13, fn.__name__
+    assert np.allclose(FT.J_m1m2_Mceta(Mc, eta), Jme, rtol=1e-14, atol=0)
+    assert np.allclose(FT.J_Mceta_m1m2(m1, m2), Jmm, rtol=1e-14, atol=1e-300)
