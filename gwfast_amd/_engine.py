@@ -24,8 +24,13 @@ def device():
 
 
 def to_dev(a, dtype=None):
+    """Device copy of a host array: float64 unless dtype is given; integer
+    and boolean arrays keep their own dtype."""
     t = torch()
-    return t.as_tensor(np.ascontiguousarray(a), dtype=dtype or t.float64, device=device())
+    a = np.ascontiguousarray(a)
+    if dtype is None and a.dtype.kind not in 'iub':
+        dtype = t.float64
+    return t.as_tensor(a, dtype=dtype, device=device())
 
 
 class Workspace:

@@ -498,6 +498,10 @@ __global__ void k_gather(const double* __restrict__ par, int64_t ld, int n_rows,
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_idx) return;
   const int64_t e = idx[j];
+  if (e < 0 || e >= ld) {  // outside every row: a NaN column instead of a stray read
+    for (int r = 0; r < n_rows; ++r) out[r * ld_out + j] = __builtin_nan("");
+    return;
+  }
   for (int r = 0; r < n_rows; ++r) out[r * ld_out + j] = par[r * ld + e];
 }
 

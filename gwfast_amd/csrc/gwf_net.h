@@ -58,6 +58,9 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
     if (S.motion == MOTION_EARTH) net.any_earth = 1;
     const int narms = g.shape == GWF_SHAPE_L ? 1 : 3;
     if (na + narms > GWF_MAX_ARMS) { err = "too many arms"; return -EINVAL; }
+    S.arm0 = na;
+    S.narm = narms;
+    S.sum0 = net.n_sum_arms;
     double s2[3], c2[3];
     for (int k = 0; k < 3; ++k) {
       const double rot = (k * 60.) * M_PI / 180.;
@@ -104,6 +107,7 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
       if (l1 != 0.0) net.sum_arm[net.n_sum_arms++] = A1;
       if (l2 != 0.0) net.sum_arm[net.n_sum_arms++] = A2;
     }
+    S.nsum = net.n_sum_arms - S.sum0;
     na += narms;
   }
   net.n_arms = na;

@@ -52,6 +52,8 @@ struct SiteK {
   double clat, slat, s2lat, c2lat, lon, sinzeta;
   int motion;
   int psd;
+  int arm0, narm;  // this site's physical arms: arm[arm0 .. arm0 + narm)
+  int sum0, nsum;  // and its network-sum arms: sum_arm[sum0 .. sum0 + nsum)
 };
 
 struct ArmK {
@@ -554,9 +556,11 @@ GWF_HD void gram_update(double* G, const double* re, const double* im, double w)
 // ------------------------------------------------------------ Fisher body
 // Accumulates the packed Fisher of event e over points lane, lane+nlanes, ...
 // arm_sel < 0: the network sum (over net.sum_arm); else only that physical arm.
-template <int MODEL, int NT, int P>
+// MASKED: the network sum over the physical arms whose bit is set in mask
+// (bit a = physical arm a); a site with all its arms set uses its sum arms.
+template <int MODEL, int NT, int P, bool MASKED = false>
 GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int res, int lane, int nlanes,
-                         int arm_sel, double* G) {
+                         int arm_sel, double* G, uint32_t mask = 0u) {
   const double* ev = rec + REC_EV;
   const double a = log10(net.fmin);
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
@@ -589,12 +593,14 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
     int cur_site = -1, cur_psd = -1;
     double Sf = 1.0;
     Geo g;
-    for (int k = 0; k < narm; ++k) {
-      const ArmK& arm = sum_mode ? net.sum_arm[k] : net.arm[arm_sel];
-      const int s = arm.site;
-      const SiteK& S = net.site[s];
-      const double* sc = rec + REC_SITE + s * SC_COUNT;
-      if (s != cur_site) {
+    if constexpr (MASKED) {
+      (void)narm;
+      for (int s = 0; s < net.n_sites; ++s) {
+        const SiteK& S = net.site[s];
+        const uint32_t all = (1u << S.narm) - 1u;
+        const uint32_t bits = (mask >> S.arm0) & all;
+        if (!bits) continue;
+        const double* sc = rec + REC_SITE + s * SC_COUNT;
         if (S.psd != cur_psd) {
 #ifndef GWF_NO_PSD_PREFETCH
           if (S.psd == 0)
@@ -607,11 +613,41 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
           cur_psd = S.psd;
         }
         site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
-        cur_site = s;
+        const bool full = bits == all;
+        const int cnt = full ? S.nsum : S.narm;
+        for (int j = 0; j < cnt; ++j) {
+          if (!full && !((bits >> j) & 1u)) continue;
+          const ArmK& arm = full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j];
+          double re[P], im[P], Ap, Ac;
+          arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+          gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+        }
       }
-      double re[P], im[P], Ap, Ac;
-      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
-      gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+    } else {
+      for (int k = 0; k < narm; ++k) {
+        const ArmK& arm = sum_mode ? net.sum_arm[k] : net.arm[arm_sel];
+        const int s = arm.site;
+        const SiteK& S = net.site[s];
+        const double* sc = rec + REC_SITE + s * SC_COUNT;
+        if (s != cur_site) {
+          if (S.psd != cur_psd) {
+#ifndef GWF_NO_PSD_PREFETCH
+            if (S.psd == 0)
+              Sf = psd_finish(net.psd[0], f, lnf, pre0);
+            else if (S.psd == 1)
+              Sf = psd_finish(net.psd[1], f, lnf, pre1);
+            else
+#endif
+              Sf = psd_at(net.psd[S.psd], f, lnf);
+            cur_psd = S.psd;
+          }
+          site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
+          cur_site = s;
+        }
+        double re[P], im[P], Ap, Ac;
+        arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+        gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+      }
     }
   }
 }

@@ -108,6 +108,26 @@ __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_wavef
   store_packed<P>(G, F, n_ev, e, lane);
 }
 
+// Network Fisher with per-event arm masks (duty cycle): bit a of arm_mask[e]
+// switches physical arm a of the network on for event e.
+template <int MODEL, int NT, int P>
+__global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher_masked(gwf_waveform w, NetK net,
+                                                                             const double* __restrict__ rec,
+                                                                             int64_t n_ev,
+                                                                             const uint32_t* __restrict__ arm_mask,
+                                                                             double* __restrict__ F) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;  // wave-uniform exit
+  const int lane = threadIdx.x & 63;
+  const uint32_t mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
+  constexpr int NG = P * (P + 1) / 2;
+  double G[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) G[k] = 0.0;
+  fisher_event<MODEL, NT, P, true>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, -1, G, mask);
+  store_packed<P>(G, F, n_ev, e, lane);
+}
+
 // Ragged grids: one persistent launch; each wavefront takes events from an
 // atomic counter (lane 0, vector-memory atomic) until the batch is exhausted,
 // so every wave reaches the exit; res_ev[e] is event e's grid length.
@@ -892,6 +912,34 @@ int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
                        fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher launch");
+}
+
+int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                      int64_t ld, int64_t n_ev, int32_t res, const uint32_t* arm_mask, double* fisher_out,
+                      void* workspace, size_t ws_bytes, void* stream) {
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  NetK net;
+  rc = build_net_checked(dets, n_det, res, net);
+  if (rc) return rc;
+  if (n_ev == 0) return 0;
+  if (!fisher_out || !arm_mask) return fail(-EINVAL, "NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
+  const double* rec = (const double*)workspace;
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       arm_mask, fisher_out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec,
+                       n_ev, arm_mask, fisher_out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       arm_mask, fisher_out);
+  else
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       arm_mask, fisher_out);
+  return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
 }
 
 int gwf_snr(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,

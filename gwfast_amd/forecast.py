@@ -24,11 +24,70 @@ def _subset(events, sl):
     return {k: np.asarray(v)[sl] for k, v in events.items()}
 
 
-def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None, band=(8.0, 16.0)):
+def duty_masks(net, seeds, duty_factor, n):
+    """calculate_forecasts_from_catalog.py:423-447: per detector, numpy's
+    global generator reseeded with seeds[i], then one Bernoulli(duty_factor)
+    draw of n per arm (arm order); keys as DetNet.SNR(return_all=True)."""
+    masks = {}
+    for i, (name, sig) in enumerate(net.signals.items()):
+        np.random.seed(seeds[i])
+        if sig.detector_shape == 'L':
+            masks[name] = np.random.choice([0, 1], n, p=[1. - duty_factor, duty_factor])
+        else:
+            for a in range(3):
+                masks[name + '_%s' % a] = np.random.choice([0, 1], n, p=[1. - duty_factor, duty_factor])
+    return masks
+
+
+def draw_seeds(net):
+    """calculate_forecasts_from_catalog.py:856-860: one fresh seed per detector."""
+    np.random.seed(None)
+    return [np.random.randint(2 ** 32 - 1, size=1)[0] for _ in range(len(net.signals))]
+
+
+def _duty_snr_fisher(net, rows, res, snr_th, masks):
+    """Duty-cycled SNR (network SNR from the masked per-arm SNRs) and the
+    masked network Fisher of the detected events, one fused launch per
+    detector group (gwf_fisher_masked)."""
+    from . import screen as SC
+    from . import signal as S
+    n = rows.shape[1]
+    groups = net._groups()
+    net_sq = np.zeros(n)
+    group_masks = []
+    for g in groups:
+        sigs = [s for _, s in g]
+        sq = S.snr_sq_device(sigs, rows, res).cpu().numpy()
+        k = 0
+        gm = []
+        for name, s in g:
+            keys = [name] if s.detector_shape == 'L' else [name + '_%s' % a for a in range(3)]
+            m = np.array([masks[key] for key in keys])
+            net_sq += (((2 * np.sqrt(sq[k:k + s.n_arms])) * m) ** 2).sum(axis=0)
+            gm.append(m)
+            k += s.n_arms
+        group_masks.append(gm)
+    snr = np.sqrt(net_sq)
+    det = np.nonzero(snr > snr_th)[0]
+    idx = E.to_dev(det.astype(np.int64), E.torch().int64)
+    F = None
+    if det.size:
+        rd = SC.gather(rows, idx)
+        for g, gm in zip(groups, group_masks):
+            sigs = [s for _, s in g]
+            Fg = S.fisher_masked_device(sigs, sigs[0].wf_model, rd, res, S.mask_words([m[:, det] for m in gm]))
+            F = Fg if F is None else F + Fg
+    return E.to_dev(snr), idx, F
+
+
+def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None, band=(8.0, 16.0),
+                 duty_factor=None, seeds=None):
     """One chunk: returns a dict with 'snrs' (network SNR of every event;
     with a screen, NaN where the exact SNR was not evaluated), 'idxs_det',
     'fisher' (P', P', n_det), 'cov', 'eps', 'sky_area', 'errors' (P', n_det),
-    'cond_numbers', 'parNums'.  Device-resident until the end."""
+    'cond_numbers', 'parNums'.  Device-resident until the end.  duty_factor
+    < 1 applies the driver's per-arm duty-cycle masks (seeds: one per
+    detector) to the SNRs and Fisher matrices."""
     t = E.torch()
     from . import screen as SC
     check_evparams(events)
@@ -37,7 +96,13 @@ def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None,
     wf = sig0.wf_model
     rows = E.to_dev(E.param_rows(events, wf.is_tidal))
     n = rows.shape[1]
-    if screen is None:
+    if duty_factor is not None and duty_factor < 1:
+        if screen is not None:
+            raise NotImplementedError('the learned screen predicts duty-free SNRs; use it without duty_factor')
+        if seeds is None or len(seeds) != len(net.signals):
+            raise ValueError('duty_factor needs one seed per detector')
+        snrs, idx, F = _duty_snr_fisher(net, rows, res, snr_th, duty_masks(net, seeds, duty_factor, n))
+    elif screen is None:
         snr = net.snr_device(rows, res)
         idx = (snr > snr_th).nonzero().flatten()
         F = net.fisher_device(SC.gather(rows, idx), res) if idx.numel() else None
@@ -88,12 +153,18 @@ def to_file(res, out_path, suff=''):
 
 
 def run_catalog(events, net, out_path, batch_size=10000, snr_th=12.0, params_fix=(), res=1000, resume=False,
-                screen=None, band=(8.0, 16.0), idx_in=0, verbose=True):
+                screen=None, band=(8.0, 16.0), idx_in=0, verbose=True, duty_factor=None, seeds=None):
     """Chunked forecast of a catalogue with per-chunk files (suffix
     _<i>_to_<j>, global event indices from idx_in).  With resume=True a chunk
-    whose snrs file exists is skipped.  Returns the list of suffixes."""
+    whose snrs file exists is skipped.  duty_factor (< 1) and seeds (one per
+    detector; drawn when None, as the driver does) apply the duty cycle to
+    every chunk with the same seeds.  Returns the list of suffixes."""
     os.makedirs(out_path, exist_ok=True)
     n = len(np.atleast_1d(events['dL']))
+    if duty_factor is not None and duty_factor >= 1:
+        duty_factor = None
+    if duty_factor is not None and seeds is None:
+        seeds = draw_seeds(net)
     suffs = []
     for i0 in range(0, n, batch_size):
         i1 = min(n, i0 + batch_size)
@@ -104,7 +175,7 @@ def run_catalog(events, net, out_path, batch_size=10000, snr_th=12.0, params_fix
                 print('Batch already present, skipping...', flush=True)
             continue
         t0 = time.perf_counter()
-        r = compute_errs(_subset(events, slice(i0, i1)), net, snr_th, params_fix, res, screen, band)
+        r = compute_errs(_subset(events, slice(i0, i1)), net, snr_th, params_fix, res, screen, band, duty_factor, seeds)
         r['idxs_det'] = r['idxs_det'] + idx_in + i0
         to_file(r, out_path, suff)
         if verbose:

@@ -106,7 +106,20 @@ class DetNet(object):
                 allD['net'] = np.array([allD[k] for k in allD.keys()]).sum(axis=0) / SNRs_net
             allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
             return allF, allD
-        if return_all or any(sig.DutyFactor is not None for sig in self.signals.values()):
+        if not return_all and any(sig.DutyFactor is not None for sig in self.signals.values()):
+            total, rows = None, None
+            n = None
+            for g in self._groups():
+                sigs = [s for _, s in g]
+                rows = sigs[0]._rows(evParams)
+                n = rows.shape[1]
+                masks = [s._duty_masks(n) for s in sigs]
+                masks = [m if m is not None else np.ones((s.n_arms, n)) for m, s in zip(masks, sigs)]
+                F = S.fisher_masked_device(sigs, sigs[0].wf_model, rows, res, S.mask_words(masks))
+                total = F if total is None else total + F
+            S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
+            return total.cpu().numpy()
+        if return_all:
             allF = {}
             for name, sig in self.signals.items():
                 F_ = sig.FisherMatr(evParams, res=res, return_all=True, **opts)

@@ -332,6 +332,8 @@ def gather(rows, idx):
     """Columns idx of device parameter rows, as a new (P_rows, len(idx)) tensor."""
     T = torch()
     rows = rows.contiguous()
+    if idx.dtype != T.int64 or idx.device != rows.device:
+        raise TypeError('gather needs int64 event indices on the rows\' device')
     m = idx.numel()
     out = T.empty((rows.shape[0], m), dtype=T.float64, device=device())
     call('gwf_gather_events', _ptr(rows), rows.stride(0), rows.shape[0], _ptr(idx), m, _ptr(out), m, _stream())

@@ -152,14 +152,17 @@ class GWSignal(object):
                     allF[k] = allF[k] * masks[k]
                     allD[k] = allD[k] * (masks[k][None, :, None] if allD[k].ndim == 3 else masks[k][None, :])
             return allF, allD
-        if return_all or masks is not None:
+        if return_all:
             Fs = []
             for k in range(self.n_arms):
                 F = fisher_device([self], self.wf_model, rows, res, arm=k)
                 reparam_device(F, rows, use_m1m2, use_chi1chi2)
                 Fs.append(F.cpu().numpy() * (masks[k] if masks is not None else 1.0))
-            return Fs if return_all else np.sum(Fs, axis=0)
-        F = fisher_device([self], self.wf_model, rows, res)
+            return Fs
+        if masks is not None:
+            F = fisher_masked_device([self], self.wf_model, rows, res, mask_words([masks]))
+        else:
+            F = fisher_device([self], self.wf_model, rows, res)
         reparam_device(F, rows, use_m1m2, use_chi1chi2)
         return F.cpu().numpy()
 
@@ -353,6 +356,38 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None):
     else:
         E.call('gwf_fisher_ragged', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n,
                N.ptr(r_ev), arm, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    return out
+
+
+def mask_words(masks_per_signal):
+    """Per-event arm bit masks for gwf_fisher_masked from each signal's
+    (n_arms, N) 0/1 masks (None: every arm on), in network arm order; a
+    (N,) int32 device tensor holding the uint32 bits."""
+    words = None
+    off = 0
+    for m in masks_per_signal:
+        m = np.asarray(m)
+        if words is None:
+            words = np.zeros(m.shape[1], dtype=np.int64)
+        for a in range(m.shape[0]):
+            words |= (m[a] != 0).astype(np.int64) << (off + a)
+        off += m.shape[0]
+    return E.to_dev(words.astype(np.int32), E.torch().int32)
+
+
+def fisher_masked_device(signals, wf, rows, res, words):
+    """Network Fisher (P, P, N_ev) device tensor summing, per event, the arms
+    whose bit is set in words (one fused launch)."""
+    t = E.torch()
+    n = rows.shape[1]
+    if np.ndim(res) != 0:
+        raise NotImplementedError('duty-cycle masks need one grid length for all events')
+    if words.dtype != t.int32 or words.shape != (n,) or words.device != rows.device:
+        raise TypeError('arm masks must be an int32 device tensor of one word per event')
+    out = t.empty((wf.nParams, wf.nParams, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    E.call('gwf_fisher_masked', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, int(res),
+           N.ptr(words), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
 

@@ -126,6 +126,16 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
                int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes,
                void* stream);
 
+/* DetNet.FisherMatr with the duty-cycle masks of GWSignal(DutyFactor=...)
+ * (signal.py:932-936, 968-969, 1004-1005, 1038-1039, 1071-1072; network.py:
+ * 84-152) and of the catalogue driver (calculate_forecasts_from_catalog.py:
+ * 423-447, 527-556): one fused pass summing, per event, the Fisher of the
+ * physical arms whose bit is set in arm_mask[e] (bit a = network arm a, in
+ * the arm order of gwf_fisher).  arm_mask: (n_ev,) device uint32. */
+int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                      int64_t ld, int64_t n_ev, int32_t res, const uint32_t* arm_mask, double* fisher_out,
+                      void* workspace, size_t ws_bytes, void* stream);
+
 /* Ragged per-event grids (no reference counterpart: gwfast uses one res for
  * all events, signal.py:890-898): event e uses jnp.geomspace(fmin, fcut_e,
  * res_ev[e]) with res_ev a DEVICE int32 array (values < 2 are treated as 2).
@@ -281,7 +291,8 @@ size_t gwf_select_workspace_bytes(int64_t n);
 int gwf_select(const int8_t* cls, int64_t n, int32_t want, int64_t* idx_out, int64_t* count_out, void* workspace,
                size_t workspace_bytes, void* stream);
 
-/* out[r * ld_out + j] = params[r * ld + idx[j]] for r < n_rows, j < n_idx. */
+/* out[r * ld_out + j] = params[r * ld + idx[j]] for r < n_rows, j < n_idx;
+ * an index outside [0, ld) gives a NaN column. */
 int gwf_gather_events(const double* params, int64_t ld, int32_t n_rows, const int64_t* idx, int64_t n_idx,
                       double* out, int64_t ld_out, void* stream);
 

@@ -145,3 +145,64 @@ def test_fishertools_variable_changes(gpu):
         assert np.max(np.abs(got1 - want[:, :, 2]) / np.maximum(np.abs(want[:, :, 2]), 1e-300)) <= 1e-13, fn.__name__
     assert np.allclose(FT.J_m1m2_Mceta(Mc, eta), Jme, rtol=1e-14, atol=0)
     assert np.allclose(FT.J_Mceta_m1m2(m1, m2), Jmm, rtol=1e-14, atol=1e-300)
+
+
+@pytest.mark.parametrize('model', ['PD', 'NRT'])
+def test_masked_fisher_kernel(gpu, model):
+    """gwf_fisher_masked: per event, the sum of the Fisher matrices of the
+    arms whose bit is set, against per-arm launches; all bits set gives the
+    fused network Fisher, no bit gives zero."""
+    from gwfast_amd import signal as S
+    wf, _ = _models(model)
+    n, res = 48, 500
+    ev = (bbh if model == 'PD' else bns)(n, seed=51)
+    net = hip_net(wf, NET_C1, True)
+    sigs = list(net.signals.values())
+    sigs[0]._complete(ev)
+    rows = sigs[0]._rows(ev)
+    per_arm = [S.fisher_device(sigs, wf, rows, res, arm=k).cpu().numpy() for k in range(5)]
+    full = S.fisher_device(sigs, wf, rows, res).cpu().numpy()
+    rng = np.random.default_rng(52)
+    bits = rng.integers(0, 2, size=(5, n))
+    bits[:, 0] = 0
+    bits[:, 1] = 1
+    bits[:3, 2] = 1          # the whole triangle, no L detector
+    bits[:, 3] = [1, 0, 1, 0, 1]
+    Fm = S.fisher_masked_device(sigs, wf, rows, res, S.mask_words([bits[:3], bits[3:4], bits[4:5]])).cpu().numpy()
+    want = sum(per_arm[k] * bits[k] for k in range(5))
+    assert np.all(Fm[:, :, bits.sum(axis=0) == 0] == 0)
+    live = np.nonzero(bits.any(axis=0))[0]
+    e = norm_err(Fm[:, :, live], want[:, :, live])
+    print(model, 'masked vs per-arm sum %.3e' % e, flush=True)
+    assert e <= 1e-13
+    assert norm_err(Fm[:, :, 1:2], full[:, :, 1:2]) <= 1e-14
+
+
+def test_catalogue_forecast_with_duty_factor(gpu, tmp_path):
+    """The driver's duty cycle (calculate_forecasts_from_catalog.py:423-556):
+    per-detector seeds, per-arm masks on the SNRs, detection on the masked
+    network SNR, masked per-arm Fisher sum of the detected events."""
+    from gwfast_amd import forecast as FC
+    from gwfast_amd.waveforms import IMRPhenomD
+    ev = bbh(120, seed=61)
+    ev['dL'] = ev['dL'] * 4.0
+    wf = IMRPhenomD()
+    net = hip_net(wf, NET_C1, True)
+    seeds, p, res = [7, 8, 9], 0.6, 400
+    FC.run_catalog(ev, net, str(tmp_path), batch_size=120, res=res, duty_factor=p, seeds=seeds, verbose=False)
+    s_all = net.SNR(dict(ev), res=res, return_all=True)
+    masks = {}
+    for (name, _), seed in zip(NET_C1, seeds):
+        keys = ['ETS_0', 'ETS_1', 'ETS_2'] if name == 'ETS' else [name]
+        for key, m in zip(keys, _masks(seed, len(keys), 120, p)):
+            masks[key] = m
+    snr = np.sqrt(sum((s_all[k] * masks[k]) ** 2 for k in masks))
+    got = np.loadtxt(tmp_path / 'snrs_0_to_120.txt')
+    assert np.allclose(got, snr, rtol=1e-14, atol=0)
+    det = np.nonzero(snr > 12.0)[0]
+    assert 0 < len(det) < 120
+    assert np.array_equal(np.atleast_1d(np.loadtxt(tmp_path / 'idxs_det_0_to_120.txt')).astype(int), det)
+    F0 = net.FisherMatr({k: v[det] for k, v in ev.items()}, res=res, return_all=True)
+    want = sum(F0[k] * masks[k][det] for k in masks)
+    F = np.load(tmp_path / 'fishers_0_to_120.npy')
+    assert norm_err(F, want) <= 1e-13
