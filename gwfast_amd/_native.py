@@ -77,6 +77,8 @@ SIGNATURES = {
     'gwf_inversion_error': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
     'gwf_check_fisher': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_void_p, c_void_p, c_void_p, c_void_p]),
     'gwf_fisher_reparam': (ctypes.c_int, [c_void_p, c_int64, c_void_p, c_int32, c_int64, c_int32, c_int32, c_void_p]),
+    'gwf_cov_invert_ex': (ctypes.c_int, [c_void_p, c_int32, c_int64, c_int32, c_int32, c_int32, c_double, c_double,
+                                         c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'gwf_fisher_masked': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,
                                          c_int32, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     'gwf_fisher_block_transform': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32, c_void_p,

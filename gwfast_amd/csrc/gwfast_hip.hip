@@ -423,8 +423,82 @@ __device__ void jacobi_eig(double* A, int P, double* w, double* V) {
   }
 }
 
-__global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, double* __restrict__ C,
-                      double* __restrict__ eps, double* __restrict__ cond, int* __restrict__ method) {
+// mpmath's LU_decomp (the library behind fisherTools' 'inv' and 'lu'): row
+// pivot maximising |A[k][j]| / sum_{l>=j} |A[k][l]|; numerically singular
+// (false) when a row sum or a pivot is <= ||A||_1 * 2^-52.  In place: the unit
+// lower factor below the diagonal, the upper factor on and above; swaps in p.
+__device__ bool lu_pivoted(double* A, int P, int* p) {
+  double n1 = 0.0;
+  for (int j = 0; j < P; ++j) {
+    double s = 0.0;
+    for (int i = 0; i < P; ++i) s += fabs(A[i * P + j]);
+    n1 = fmax(n1, s);
+  }
+  const double tol = n1 * 2.220446049250313e-16;
+  for (int j = 0; j < P - 1; ++j) {
+    double biggest = 0.0;
+    int pj = -1;
+    for (int k = j; k < P; ++k) {
+      double s = 0.0;
+      for (int l = j; l < P; ++l) s += fabs(A[k * P + l]);
+      if (!(s > tol)) return false;
+      const double cur = (1.0 / s) * fabs(A[k * P + j]);
+      if (cur > biggest) {
+        biggest = cur;
+        pj = k;
+      }
+    }
+    if (pj < 0) return false;
+    p[j] = pj;
+    for (int l = 0; l < P; ++l) {
+      const double t = A[j * P + l];
+      A[j * P + l] = A[pj * P + l];
+      A[pj * P + l] = t;
+    }
+    if (!(fabs(A[j * P + j]) > tol)) return false;
+    for (int i = j + 1; i < P; ++i) {
+      A[i * P + j] /= A[j * P + j];
+      for (int k = j + 1; k < P; ++k) A[i * P + k] -= A[i * P + j] * A[j * P + k];
+    }
+  }
+  return fabs(A[(P - 1) * P + P - 1]) > tol;
+}
+
+// Inverse of a unit lower (unit = true) or an upper triangular matrix held
+// in T (row-major), to R.
+__device__ void tri_inverse(const double* T, int P, bool lower_unit, double* R) {
+  for (int i = 0; i < P * P; ++i) R[i] = 0.0;
+  if (lower_unit) {
+    for (int j = 0; j < P; ++j) {
+      R[j * P + j] = 1.0;
+      for (int i = j + 1; i < P; ++i) {
+        double s = 0.0;
+        for (int k = j; k < i; ++k) s -= T[i * P + k] * R[k * P + j];
+        R[i * P + j] = s;
+      }
+    }
+  } else {
+    for (int j = P - 1; j >= 0; --j) {
+      R[j * P + j] = 1.0 / T[j * P + j];
+      for (int i = j - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int k = i + 1; k <= j; ++k) s -= T[i * P + k] * R[k * P + j];
+        R[i * P + j] = s / T[i * P + i];
+      }
+    }
+  }
+}
+
+// fisherTools.CovMatr per event (fisherTools.py:53-196), one thread each.
+// meth / alt: GWF_INV_* ids.  Steps: Jacobi scaling ws = 1/sqrt(F_ii) (a zero
+// diagonal skips it, a negative one gives NaN); eigenvalues of the scaled
+// matrix A (condition number); a negative eigenvalue or a failed Cholesky
+// switches 'cho' to alt; then the chosen inverse of A, symmetrised and
+// unscaled; eps = max|C F - I| against F, or against the truncated scaled
+// matrix when 'svd' truncates (as the reference stores it, 128-136).
+__global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, int meth_in, int alt, int truncate,
+                      double svals_thresh, double cond_max, double* __restrict__ C, double* __restrict__ eps,
+                      double* __restrict__ cond, int* __restrict__ method) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_ev) return;
   double Fm[PMAX * PMAX], A[PMAX * PMAX], V[PMAX * PMAX], L[PMAX * PMAX], w[PMAX], ws[PMAX];
@@ -470,31 +544,34 @@ __global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, double*
     emin = fmin(emin, fabs(w[i]));
     neg = neg || (w[i] < 0.0);
   }
-  if (cond) cond[e] = emax / emin;
-  int m = 0;
-  // Cholesky A = L L^T
-  if (!neg) {
-    for (int i = 0; i < P * P; ++i) L[i] = 0.0;
-    for (int j = 0; j < P && m == 0; ++j) {
-      double d = A[j * P + j];
-      for (int k = 0; k < j; ++k) d -= L[j * P + k] * L[j * P + k];
-      if (!(d > 0.0)) {
-        m = 1;
-        break;
-      }
-      const double ljj = sqrt(d);
-      L[j * P + j] = ljj;
-      for (int i = j + 1; i < P; ++i) {
-        double s = A[i * P + j];
-        for (int k = 0; k < j; ++k) s -= L[i * P + k] * L[j * P + k];
-        L[i * P + j] = s / ljj;
+  const double cnd = emax / emin;
+  if (cond) cond[e] = cnd;
+  int m = meth_in;
+  if (m == GWF_INV_CHO) {
+    if (neg) {
+      m = alt;
+    } else {  // mpmath.cholesky: a pivot below 2^-52 is not positive-definite
+      for (int i = 0; i < P * P; ++i) L[i] = 0.0;
+      for (int j = 0; j < P && m == GWF_INV_CHO; ++j) {
+        double d = A[j * P + j];
+        for (int k = 0; k < j; ++k) d -= L[j * P + k] * L[j * P + k];
+        if (!(d >= 2.220446049250313e-16)) {
+          m = alt;
+          break;
+        }
+        const double ljj = sqrt(d);
+        L[j * P + j] = ljj;
+        for (int i = j + 1; i < P; ++i) {
+          double s = A[i * P + j];
+          for (int k = 0; k < j; ++k) s -= L[i * P + k] * L[j * P + k];
+          L[i * P + j] = s / ljj;
+        }
       }
     }
-  } else {
-    m = 1;
   }
   double Cc[PMAX * PMAX];
-  if (m == 0) {
+  bool trunc_used = false;
+  if (m == GWF_INV_CHO) {
     // c = L^{-1} (lower triangular), cc = c^T c
     double Ci[PMAX * PMAX];
     for (int i = 0; i < P * P; ++i) Ci[i] = 0.0;
@@ -512,19 +589,81 @@ __global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, double*
         for (int k = 0; k < P; ++k) s += Ci[k * P + i] * Ci[k * P + j];
         Cc[i * P + j] = s;
       }
-  } else {
-    // eigen-decomposition inverse V diag(1/w) V^T (the 'svd' fallback for a
-    // symmetric matrix); a zero eigenvalue fails like mpmath's 1/0
-    for (int i = 0; i < P; ++i) {
-      if (w[i] == 0.0) {
-        put_nan(-1);
-        return;
+  } else if (m == GWF_INV_SVD || m == GWF_INV_SVD_REG) {
+    // SVD of the symmetric A from its eigenpairs: singular values |w_k|,
+    // A^+ = sum_k sign(w_k) v_k v_k^T / |w_k|
+    double smax = 0.0;
+    for (int k = 0; k < P; ++k) smax = fmax(smax, fabs(w[k]));
+    const bool trunc = (m == GWF_INV_SVD) && truncate && fabs(cnd) > cond_max;
+    double sinv[PMAX], st[PMAX];
+    for (int k = 0; k < P; ++k) {
+      const double s = fabs(w[k]), sg = w[k] < 0.0 ? -1.0 : 1.0;
+      if (m == GWF_INV_SVD_REG) {
+        sinv[k] = s > svals_thresh ? sg / s : 0.0;  // singular values <= thresh left out
+      } else if (trunc) {
+        const bool keep = s / smax > svals_thresh;
+        sinv[k] = sg / (keep ? s : smax * svals_thresh);
+        st[k] = sg * (keep ? s : smax * svals_thresh);
+      } else {
+        if (s == 0.0) {  // 1/0: no finite inverse
+          put_nan(-1);
+          return;
+        }
+        sinv[k] = sg / s;
       }
     }
     for (int i = 0; i < P; ++i)
       for (int j = 0; j < P; ++j) {
         double s = 0.0;
-        for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] / w[k];
+        for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] * sinv[k];
+        Cc[i * P + j] = s;
+      }
+    if (trunc) {  // the truncated (scaled) Fisher the inversion error is taken against
+      trunc_used = true;
+      for (int i = 0; i < P; ++i)
+        for (int j = i; j < P; ++j) {
+          double s = 0.0;
+          for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] * st[k];
+          Fm[i * P + j] = s;
+          Fm[j * P + i] = s;
+        }
+    }
+  } else {  // 'inv' (mpmath inverse) or 'lu' (U^{-1} (P L)^{-1}, fisherTools.py:156-161)
+    int piv[PMAX];
+    for (int i = 0; i < P * P; ++i) L[i] = A[i];
+    if (!lu_pivoted(L, P, piv)) {
+      put_nan(-1);
+      return;
+    }
+    double Ui[PMAX * PMAX], Li[PMAX * PMAX];
+    tri_inverse(L, P, false, Ui);
+    tri_inverse(L, P, true, Li);
+    // permutation of the row swaps: row r of Pm has its 1 in column perm[r]
+    int perm[PMAX];
+    for (int r = 0; r < P; ++r) perm[r] = r;
+    for (int k = 0; k < P - 1; ++k) {
+      const int t = perm[k];
+      perm[k] = perm[piv[k]];
+      perm[piv[k]] = t;
+    }
+    // X = L^{-1} M: 'inv' takes M = Pm (the inverse of A = Pm^T L U), 'lu'
+    // takes M = Pm^T, as the reference's U^{-1} (Pm L)^{-1} does
+    double X[PMAX * PMAX];
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0.0;
+        if (m == GWF_INV_INV) {
+          for (int r = 0; r < P; ++r)
+            if (perm[r] == j) s = Li[i * P + r];
+        } else {
+          s = Li[i * P + perm[j]];
+        }
+        X[i * P + j] = s;
+      }
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < P; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < P; ++k) s += Ui[i * P + k] * X[k * P + j];
         Cc[i * P + j] = s;
       }
   }
@@ -544,6 +683,7 @@ __global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, double*
       }
     eps[e] = mx;
   }
+  (void)trunc_used;
   if (method) method[e] = m;
 }
 
@@ -1078,15 +1218,27 @@ int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P,
   return hip_check(hipGetLastError(), "gwf_gram_split launch");
 }
 
-int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
-                   int32_t* method_out, void* stream) {
+int gwf_cov_invert_ex(const double* F, int32_t P, int64_t n_ev, int32_t inv_method, int32_t alt_method,
+                      int32_t truncate, double svals_thresh, double cond_numb_max, double* cov_out, double* eps_out,
+                      double* cond_out, int32_t* method_out, void* stream) {
   if (P < 1 || P > PMAX) return fail(-EINVAL, "P must be 1..16");
   if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (inv_method < GWF_INV_CHO || inv_method > GWF_INV_LU || alt_method < GWF_INV_CHO || alt_method > GWF_INV_LU)
+    return fail(-EINVAL, "unknown inversion method");
+  if (inv_method == GWF_INV_CHO && alt_method == GWF_INV_CHO)
+    return fail(-EINVAL, "alt_method has to be different from invMethodIn");
   if (n_ev == 0) return 0;
   if (!F || !cov_out) return fail(-EINVAL, "NULL array");
   hipLaunchKernelGGL(k_cov, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
-                     cov_out, eps_out, cond_out, (int*)method_out);
+                     (int)inv_method, (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out,
+                     eps_out, cond_out, (int*)method_out);
   return hip_check(hipGetLastError(), "gwf_cov_invert launch");
+}
+
+int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
+                   int32_t* method_out, void* stream) {
+  return gwf_cov_invert_ex(F, P, n_ev, GWF_INV_CHO, GWF_INV_SVD, 0, 1e-15, 1e50, cov_out, eps_out, cond_out,
+                           method_out, stream);
 }
 
 int gwf_inversion_error(const double* F, const double* C, int32_t P, int64_t n_ev, double* eps_out, void* stream) {

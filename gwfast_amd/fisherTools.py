@@ -1,7 +1,8 @@
 """Covariance inversion and Fisher checks with gwfast's fisherTools API.
 
-Mirrors gwfast/fisherTools.py:32-196 (CovMatr), 199-210
-(compute_inversion_error) and 216-279 (CheckFisher).  The reference runs a
+Mirrors gwfast/fisherTools.py:32-196 (CovMatr, every inversion method),
+199-210 (compute_inversion_error), 216-279 (CheckFisher) and the helpers
+after them.  The reference runs a
 per-event mpmath loop at 53-bit precision and stores float128; here each
 event is inverted by one GPU thread in FP64 (gwf_cov_invert /
 gwf_check_fisher) and float64 is returned.
@@ -21,34 +22,45 @@ def _dev(F):
     return E.to_dev(np.asarray(F, dtype=np.float64))
 
 
-def cov_device(Fd, want_eps=True):
-    """(Cov, eps, cond, method) device tensors for a (P, P, N) device Fisher."""
+INV_METHODS = {'cho': 0, 'svd': 1, 'inv': 2, 'svd_reg': 3, 'lu': 4}
+
+
+def cov_device(Fd, want_eps=True, invMethodIn='cho', alt_method='svd', truncate=False, svals_thresh=1e-15,
+               condNumbMax=1e50):
+    """(Cov, eps, cond, method) device tensors for a (P, P, N) device Fisher;
+    method holds the INV_METHODS id used per event (-1: NaN)."""
     t = E.torch()
+    for name in (invMethodIn, alt_method):
+        if name not in INV_METHODS:
+            raise ValueError("Inversion method must be one of 'inv', 'cho', 'svd', 'svd_reg', 'lu'")
+    if invMethodIn == 'cho' and alt_method == 'cho':
+        raise ValueError('alt_method has to be different from invMethodIn')
     P, n = Fd.shape[0], Fd.shape[2]
     C = t.empty_like(Fd)
     eps = t.empty(n, dtype=t.float64, device=Fd.device) if want_eps else None
     cond = t.empty(n, dtype=t.float64, device=Fd.device)
     meth = t.empty(n, dtype=t.int32, device=Fd.device)
-    E.call('gwf_cov_invert', N.ptr(Fd), P, n, N.ptr(C), N.ptr(eps), N.ptr(cond), N.ptr(meth), N.stream_ptr())
+    E.call('gwf_cov_invert_ex', N.ptr(Fd), P, n, INV_METHODS[invMethodIn], INV_METHODS[alt_method], int(bool(truncate)),
+           float(svals_thresh), float(condNumbMax), N.ptr(C), N.ptr(eps), N.ptr(cond), N.ptr(meth), N.stream_ptr())
     return C, eps, cond, meth
 
 
 def CovMatr(FisherMatrix, invMethodIn='cho', condNumbMax=1e50, truncate=False, svals_thresh=1e-15, verbose=False,
             alt_method='svd', return_details=False):
-    """fisherTools.py:32-196 for invMethodIn='cho', alt_method='svd'
-    (the catalogue driver's settings, calculate_forecasts_from_catalog.py:613-619).
+    """fisherTools.py:32-196 with every inversion method ('cho', 'inv',
+    'svd' with optional truncation, 'svd_reg', 'lu') and alt_method.
     Returns (Cov (P,P,N), eps (N,)); return_details adds (cond, method)."""
-    if invMethodIn != 'cho' or alt_method != 'svd':
-        raise NotImplementedError("only invMethodIn='cho' with alt_method='svd' is on this path")
-    if truncate:
-        raise NotImplementedError('singular-value truncation is not supported')
     Fd = _dev(FisherMatrix)
-    C, eps, cond, meth = cov_device(Fd)
+    C, eps, cond, meth = cov_device(Fd, True, invMethodIn, alt_method, truncate, svals_thresh, condNumbMax)
     out = (C.cpu().numpy(), eps.cpu().numpy())
     if verbose:
         m = meth.cpu().numpy()
-        print('Inversion error: min=%s, max=%s; cholesky on %d, svd on %d, failed %d' % (
-            np.nanmin(out[1]), np.nanmax(out[1]), (m == 0).sum(), (m == 1).sum(), (m == -1).sum()))
+        e = out[1]
+        print(' Inversion error with method %s: min=%s, max=%s, mean=%s, std=%s ' % (
+            invMethodIn, np.nanmin(e), np.nanmax(e), np.nanmean(e), np.nanstd(e)))
+        if invMethodIn == 'cho':
+            print('Method %s not possible on %s non-positive definite matrices, %s was used in those cases. ' % (
+                invMethodIn, int((m == INV_METHODS[alt_method]).sum()), alt_method))
     if return_details:
         return out + (cond.cpu().numpy(), meth.cpu().numpy())
     return out

@@ -208,6 +208,27 @@ int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P,
 int gwf_cov_invert(const double* F, int32_t P, int64_t n_ev, double* cov_out, double* eps_out, double* cond_out,
                    int32_t* method_out, void* stream);
 
+/* Inversion methods of fisherTools.CovMatr (invMethodIn / alt_method). */
+#define GWF_INV_CHO 0
+#define GWF_INV_SVD 1
+#define GWF_INV_INV 2
+#define GWF_INV_SVD_REG 3
+#define GWF_INV_LU 4
+
+/* fisherTools.CovMatr with every option (fisherTools.py:32-196): inv_method
+ * and alt_method are GWF_INV_* ids ('cho' switches to alt_method on a
+ * negative eigenvalue of the scaled matrix or a Cholesky pivot below 2^-52,
+ * as mpmath's cholesky); 'svd' with truncate replaces singular values below
+ * svals_thresh * max by that floor when the condition number exceeds
+ * cond_numb_max, and eps is then taken against the truncated scaled matrix,
+ * as the reference stores it; 'svd_reg' leaves out singular values
+ * <= svals_thresh; 'inv' and 'lu' use mpmath's pivoted LU (numerically
+ * singular -> NaN), 'lu' composing U^{-1} (P L)^{-1} as the reference does.
+ * method_out holds the method used per event (-1: NaN). */
+int gwf_cov_invert_ex(const double* F, int32_t P, int64_t n_ev, int32_t inv_method, int32_t alt_method,
+                      int32_t truncate, double svals_thresh, double cond_numb_max, double* cov_out, double* eps_out,
+                      double* cond_out, int32_t* method_out, void* stream);
+
 /* fisherTools.compute_inversion_error (fisherTools.py:199-210) for given
  * Fisher and covariance (P, P, n_ev): eps_out (n_ev,) = max|C F - I|. */
 int gwf_inversion_error(const double* F, const double* C, int32_t P, int64_t n_ev, double* eps_out, void* stream);

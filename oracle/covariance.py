@@ -1,8 +1,9 @@
 """Oracle restatement of gwfast's covariance inversion and Fisher checks
 (test infrastructure only).
 
-Follows gwfast/fisherTools.py:32-196 (CovMatr, default 'cho' method with the
-'svd' fallback), 199-210 (compute_inversion_error) and 216-279 (CheckFisher).
+Follows gwfast/fisherTools.py:32-196 (CovMatr, every inversion method and
+the alternative method), 199-210 (compute_inversion_error), 216-279
+(CheckFisher) and 340-858 (the helpers after them).
 Like the reference it works per event in mpmath at the default 53-bit
 precision; the output is float64 here (the reference stores float128 where the
 platform has it, fisherTools.py:22-27).
@@ -11,11 +12,45 @@ import mpmath
 import numpy as np
 
 
-def cov_matr(F, alt_method='svd'):
-    """Returns (Cov (P,P,N), eps (N,), method (N,) with 0 = cholesky,
-    1 = svd fallback, -1 = NaN input/failure)."""
+METHOD_IDS = {'cho': 0, 'svd': 1, 'inv': 2, 'svd_reg': 3, 'lu': 4}
+
+
+def invert_scaled(Fp, method, cond=None, truncate=False, svals_thresh=1e-15, cond_max=1e50, c=None):
+    """The inverse of one scaled matrix Fp by method (fisherTools.py:113-161),
+    before symmetrisation: returns (cc, truncated matrix or None)."""
+    if method == 'cho':
+        return c.T * c, None
+    if method == 'inv':
+        return Fp ** -1, None
+    if method == 'svd':
+        U, Sm, V = mpmath.svd_r(Fp)
+        S = np.array([float(x) for x in Sm])
+        if truncate and abs(cond) > cond_max:
+            maxev = np.max(np.abs(S))
+            Sinv = [1 / s if abs(s) / maxev > svals_thresh else 1 / (maxev * svals_thresh) for s in S]
+            St = [s if abs(s) / maxev > svals_thresh else maxev * svals_thresh for s in S]
+            tF = U * mpmath.diag(St) * V
+            return V.T * mpmath.diag(Sinv) * U.T, (tF + tF.T) / 2
+        return V.T * mpmath.diag([1 / s for s in Sm]) * U.T, None
+    if method == 'svd_reg':
+        U, Sm, V = mpmath.svd_r(Fp)
+        S = np.array([float(x) for x in Sm])
+        Um, Vm = np.array(U.tolist(), dtype=float), np.array(V.tolist(), dtype=float)
+        kv = int(np.sum(S > svals_thresh))
+        return mpmath.matrix(Um[:, :kv] @ np.diag(1. / S[:kv]) @ Vm[:kv, :]), None
+    if method == 'lu':
+        Pm, L, U = mpmath.lu(Fp)
+        return U ** -1 * (Pm * L) ** -1, None
+    raise ValueError(method)
+
+
+def cov_matr(F, inv_method='cho', alt_method='svd', truncate=False, svals_thresh=1e-15, cond_max=1e50):
+    """Returns (Cov (P,P,N), eps (N,), method (N,) with the METHOD_IDS id used,
+    -1 = NaN input/failure).  eps is taken against the truncated scaled
+    matrix where 'svd' truncates (fisherTools.py:128-136, 189)."""
     P, _, N = F.shape
     cov = np.zeros(F.shape)
+    Fref = np.array(F, dtype=float, copy=True)
     method = np.zeros(N, dtype=int)
     for k in range(N):
         Fk = F[:, :, k]
@@ -36,29 +71,28 @@ def cov_matr(F, alt_method='svd'):
                 reweighted = True
             except ZeroDivisionError:
                 Fp = ff
-            inv_method = 'cho'
+            cond = np.max(np.abs(E)) / np.min(np.abs(E))
+            m = inv_method
             c = None
             if np.any(E < 0):
-                inv_method = alt_method
-            else:
+                if inv_method == 'cho':
+                    m = alt_method
+            elif m == 'cho':
                 try:
                     c = (mpmath.cholesky(Fp)) ** -1
                 except Exception:
-                    inv_method = alt_method
-            if inv_method == 'cho':
-                cc = c.T * c
-                method[k] = 0
-            else:
-                U, Sm, V = mpmath.svd_r(Fp)
-                cc = V.T * mpmath.diag([1 / s for s in Sm]) * U.T
-                method[k] = 1
+                    m = alt_method
+            cc, tF = invert_scaled(Fp, m, cond, truncate, svals_thresh, cond_max, c)
+            if tF is not None:
+                Fref[:, :, k] = np.array(tF.tolist(), dtype=float)
+            method[k] = METHOD_IDS[m]
             cc = (cc + cc.T) / 2
             C = ws * cc * ws if reweighted else cc
             cov[:, :, k] = np.array(C.tolist(), dtype=float)
         except Exception:
             cov[:, :, k] = np.nan
             method[k] = -1
-    return cov, compute_inversion_error(F, cov), method
+    return cov, compute_inversion_error(Fref, cov), method
 
 
 def compute_inversion_error(F, C):

@@ -206,3 +206,51 @@ def test_catalogue_forecast_with_duty_factor(gpu, tmp_path):
     want = sum(F0[k] * masks[k][det] for k in masks)
     F = np.load(tmp_path / 'fishers_0_to_120.npy')
     assert norm_err(F, want) <= 1e-13
+
+
+def _cov_cases():
+    """(name, F (P, P, N), kwargs) covering every inversion method."""
+    rng = np.random.default_rng(3)
+    P = 6
+    X = rng.standard_normal((4, P, P))
+    A = X @ np.transpose(X, (0, 2, 1)) + 3 * P * np.eye(P)
+    s = np.geomspace(1, 1e4, P)
+    good = np.transpose(A, (1, 2, 0)) * s[:, None, None] * s[None, :, None]
+    r = 1 - 1e-8
+    B = np.eye(5)
+    B[0, 1] = B[1, 0] = r
+    d = np.linspace(0.5, 3.0, 5)
+    blk = (d[:, None] * B * d[None, :])[:, :, None]
+    rng = np.random.default_rng(0)            # symmetric, indefinite, LU pivoting is a 3-cycle
+    Xs = rng.standard_normal((4, 4))
+    S = (Xs + Xs.T) / 2
+    np.fill_diagonal(S, np.abs(np.diag(S)) * 0.1 + 0.01)
+    cyc = S[:, :, None]
+    out = []
+    for m in ('cho', 'inv', 'svd', 'svd_reg', 'lu'):
+        out.append(('good-' + m, good, dict(invMethodIn=m)))
+    out.append(('svd_reg-block', blk, dict(invMethodIn='svd_reg', svals_thresh=1e-6)))
+    out.append(('svd-trunc-block', blk, dict(invMethodIn='svd', truncate=True, svals_thresh=1e-6, condNumbMax=1e3)))
+    for alt in ('svd', 'inv', 'lu', 'svd_reg'):
+        out.append(('cyc-cho-alt-' + alt, cyc, dict(invMethodIn='cho', alt_method=alt)))
+    out.append(('cyc-lu', cyc, dict(invMethodIn='lu')))
+    return out
+
+
+@pytest.mark.parametrize('case', _cov_cases(), ids=[c[0] for c in _cov_cases()])
+def test_covmatr_methods_vs_oracle(gpu, case):
+    """CovMatr with every invMethodIn / alt_method / truncation against the
+    oracle's mpmath restatement (fisherTools.py:32-196)."""
+    from gwfast_amd import fisherTools as FT
+    from oracle import covariance as OC
+    name, F, kw = case
+    C, eps, cond, meth = FT.CovMatr(F, return_details=True, **kw)
+    Co, epso, metho = OC.cov_matr(F, inv_method=kw.get('invMethodIn', 'cho'), alt_method=kw.get('alt_method', 'svd'),
+                                  truncate=kw.get('truncate', False), svals_thresh=kw.get('svals_thresh', 1e-15),
+                                  cond_max=kw.get('condNumbMax', 1e50))
+    assert np.array_equal(meth, metho), (meth, metho)
+    scale = np.max(np.abs(Co), axis=(0, 1))
+    err = np.max(np.abs(C - Co) / scale)
+    print(name, 'cov err %.3e, eps %s vs %s' % (err, eps, epso), flush=True)
+    assert err <= 1e-9
+    assert np.allclose(eps, epso, rtol=1e-3, atol=1e-10)

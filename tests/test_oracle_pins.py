@@ -676,3 +676,72 @@ def test_fishertools_mceta_to_m1m2_fish_is_fisher_in_m1m2():
     Flog[2, 2] = snr ** 2
     back = OC.scale_row_col(Flog, 2, 1. / ev['dL'])
     assert np.allclose(back[2, 2], F[2, 2], rtol=1e-12, atol=0)
+
+
+# --------------------------------------------------- CovMatr inversion methods
+def _two_block(r, P=5, d=None):
+    """Unit-diagonal A with a [[1, r], [r, 1]] block (eigenvalues 1 +- r,
+    eigenvectors (1, +-1)/sqrt 2) and identity elsewhere; F = D A D."""
+    A = np.eye(P)
+    A[0, 1] = A[1, 0] = r
+    d = np.linspace(0.5, 3.0, P) if d is None else d
+    return A, d[:, None] * A * d[None, :], d
+
+
+def test_cov_methods_agree_on_well_conditioned_matrices():
+    from oracle import covariance as OC
+    rng = np.random.default_rng(3)
+    P, n = 6, 4
+    X = rng.standard_normal((n, P, P))
+    A = X @ np.transpose(X, (0, 2, 1)) + 3 * P * np.eye(P)   # diagonally dominant: no LU row swaps
+    F = np.transpose(A, (1, 2, 0)) * np.geomspace(1, 1e4, P)[:, None, None] * np.geomspace(1, 1e4, P)[None, :, None]
+    want = np.transpose(np.linalg.inv(np.transpose(F, (2, 0, 1))), (1, 2, 0))
+    for m in ('cho', 'inv', 'svd', 'svd_reg', 'lu'):
+        C, eps, meth = OC.cov_matr(F, inv_method=m)
+        d = np.sqrt(np.einsum('iin->in', want))
+        assert np.max(np.abs(C - want) / (d[:, None] * d[None, :])) < 1e-12, m
+        assert np.all(meth == OC.METHOD_IDS[m]) and np.all(eps < 1e-10), m
+
+
+def test_cov_svd_reg_and_truncation_closed_forms():
+    """svd_reg drops singular values <= thresh of the scaled matrix; 'svd'
+    with truncate floors them at thresh * max and takes eps against the
+    truncated scaled matrix (fisherTools.py:120-149, 189)."""
+    from oracle import covariance as OC
+    r, thr = 1 - 1e-8, 1e-6
+    A, F, d = _two_block(r)
+    lam1, lam2 = 1 + r, 1 - r
+    q1, q2 = np.array([1, 1, 0, 0, 0]) / np.sqrt(2), np.array([1, -1, 0, 0, 0]) / np.sqrt(2)
+    rest = np.diag([0, 0, 1., 1., 1.])
+    Di = np.diag(1 / d)
+    cc = np.outer(q1, q1) / lam1 + rest
+    C, eps, meth = OC.cov_matr(F[:, :, None], inv_method='svd_reg', svals_thresh=thr)
+    assert np.allclose(C[:, :, 0], Di @ cc @ Di, rtol=1e-7, atol=1e-12) and meth[0] == 3
+    C, eps, meth = OC.cov_matr(F[:, :, None], inv_method='svd', truncate=True, svals_thresh=thr, cond_max=1e3)
+    floor = lam1 * thr
+    cc = np.outer(q1, q1) / lam1 + np.outer(q2, q2) / floor + rest
+    tF = np.outer(q1, q1) * lam1 + np.outer(q2, q2) * floor + rest
+    Cw = Di @ cc @ Di
+    assert np.allclose(C[:, :, 0], Cw, rtol=1e-6, atol=0)
+    assert abs(eps[0] / np.max(np.abs(Cw @ tF - np.eye(5))) - 1) < 1e-6
+    # below the condition-number bound nothing is truncated: the plain inverse
+    C, eps, meth = OC.cov_matr(F[:, :, None], inv_method='svd', truncate=True, svals_thresh=thr, cond_max=1e50)
+    assert np.allclose(C[:, :, 0], np.linalg.inv(F), rtol=1e-6, atol=0)
+
+
+def test_cov_lu_composition_as_written():
+    """'lu' composes U^-1 (P L)^-1 with mpmath's P A = L U
+    (fisherTools.py:156-161): the inverse times P^-2, the inverse itself when
+    the row swaps form an involution; 'inv' is the inverse."""
+    import mpmath
+    from oracle import covariance as OC
+    A = np.array([[0.1, 1, 0.2], [0.3, 0.1, 2.0], [3.0, 0.2, 0.1]])
+    M = mpmath.matrix(A.tolist())
+    Pm, _, _ = mpmath.lu(M)
+    Pn = np.array(Pm.tolist(), dtype=float)
+    assert not np.allclose(Pn @ Pn, np.eye(3))          # a 3-cycle
+    cc, _ = OC.invert_scaled(M, 'lu')
+    cc = np.array(cc.tolist(), dtype=float)
+    assert np.allclose(cc @ Pn @ Pn, np.linalg.inv(A), rtol=1e-12, atol=1e-14)
+    ci, _ = OC.invert_scaled(M, 'inv')
+    assert np.allclose(np.array(ci.tolist(), dtype=float), np.linalg.inv(A), rtol=1e-12, atol=1e-14)
