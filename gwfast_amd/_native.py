@@ -36,7 +36,7 @@ class GwfDetector(ctypes.Structure):
     _fields_ = [('shape', c_int32), ('earth_motion', c_int32), ('no_motion', c_int32), ('compute2arms', c_int32),
                 ('lat_deg', c_double), ('long_deg', c_double), ('xax_deg', c_double), ('fmin', c_double),
                 ('fmax', c_double), ('psd_f', c_void_p), ('psd_S', c_void_p), ('n_psd', c_int32),
-                ('reserved', c_int32), ('psd_f_first', c_double), ('psd_f_last', c_double)]
+                ('lin_spacing', c_int32), ('psd_f_first', c_double), ('psd_f_last', c_double)]
 
 
 _P = ctypes.POINTER

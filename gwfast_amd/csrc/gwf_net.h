@@ -20,6 +20,7 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
   net.res = res;
   net.fmin = dets[0].fmin;
   net.fmax = dets[0].fmax;
+  net.lin = dets[0].lin_spacing != 0;
   int na = 0, np = 0;
   for (int d = 0; d < n_det; ++d) {
     const gwf_detector& g = dets[d];
@@ -27,6 +28,8 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
     if (!g.psd_f || !g.psd_S || g.n_psd < 2) { err = "Enter a valid PSD or ASD path"; return -EINVAL; }
     if (g.fmin != net.fmin || g.fmax != net.fmax)
       { err = "detectors of one call must share fmin and fmax"; return -EINVAL; }
+    if ((g.lin_spacing != 0) != (net.lin != 0))
+      { err = "detectors of one call must share the grid spacing"; return -EINVAL; }
     if (!(g.fmin > 0.0)) { err = "fmin must be positive"; return -EINVAL; }
     // PSD table index (tables shared between detectors are evaluated once per site anyway)
     int p = -1;

@@ -71,6 +71,7 @@ struct ArmK {
 struct NetK {
   int n_sites, n_arms, res, any_earth;
   int n_sum_arms, n_psd;  // n_psd: distinct PSD tables
+  int lin;                // Fisher grids: linspace instead of geomspace
   double fmin, fmax;
   SiteK site[GWF_MAX_DETECTORS];
   ArmK arm[GWF_MAX_ARMS];
@@ -95,6 +96,14 @@ GWF_HD double exp10d(double x) {
 }
 GWF_HD double grid_f(double a, double b, int i, int div) { return exp10d(grid_lin(a, b, i, div)); }
 
+// End points of an event's grid: (log10 fmin, log10 fcut) for geomspace,
+// (fmin, fcut) for linspace (jnp.linspace: f_i = fmin (1 - s) + fcut s, last
+// point fcut), so that grid_lin gives log10 f or f itself.
+GWF_HD void grid_ends(bool lin, double fmin, double fcut, double* a, double* b) {
+  *a = lin ? fmin : log10(fmin);
+  *b = lin ? fcut : log10(fcut);
+}
+
 // Trapezoid weight of point i (np.trapz / jnp.trapezoid as a weighted sum).
 GWF_HD double trap_w(double a, double b, int i, int div, double fi) {
   if (div <= 0) return 0.0;
@@ -108,6 +117,8 @@ GWF_HD double trap_w(double a, double b, int i, int div, double fi) {
 // weight.  On the device the neighbours come from the adjacent lanes
 // (one exp10 per point instead of three); lane 0 takes the previous chunk's
 // last point, lane 63 the next chunk's first point.
+// LIN: linspace grid (a, b are fmin, fcut) instead of geomspace (log10 of them).
+template <bool LIN = false>
 struct GridWalk {
   double a, b;
   int div, lane, nlanes;
@@ -119,7 +130,7 @@ struct GridWalk {
     lane = lane_;
     nlanes = nlanes_;
     lin_cur = grid_lin(a, b, lane < res ? lane : 0, div);
-    f_cur = exp10d(lin_cur);
+    f_cur = LIN ? lin_cur : exp10d(lin_cur);
     prev_last = 0.0;
   }
   // Call once per chunk with base = 0, nlanes, 2 nlanes, ... (all lanes);
@@ -128,7 +139,7 @@ struct GridWalk {
     const int i = base + lane;
     const int inext = i + nlanes;
     lin_next = grid_lin(a, b, inext < res ? inext : 0, div);
-    f_next = exp10d(lin_next);
+    f_next = LIN ? lin_next : exp10d(lin_next);
 #if defined(__HIP_DEVICE_COMPILE__)
     double fl = __shfl_up(f_cur, 1, 64);
     double fr = __shfl_down(f_cur, 1, 64);
@@ -137,13 +148,17 @@ struct GridWalk {
     if (lane == nlanes - 1) fr = first_next;
     prev_last = __shfl(f_cur, nlanes - 1, 64);
 #else
-    const double fl = (i > 0) ? grid_f(a, b, i - 1, div) : 0.0;
-    const double fr = (i < div) ? grid_f(a, b, i + 1, div) : 0.0;
+    double fl = (i > 0) ? grid_lin(a, b, i - 1, div) : 0.0;
+    double fr = (i < div) ? grid_lin(a, b, i + 1, div) : 0.0;
+    if (!LIN) {
+      fl = (i > 0) ? exp10d(fl) : 0.0;
+      fr = (i < div) ? exp10d(fr) : 0.0;
+    }
 #endif
     const double fi = f_cur;
     *f = fi;
     *w = (div <= 0) ? 0.0 : 0.5 * (((i < div) ? fr : fi) - ((i > 0) ? fl : fi));
-    *lnf = lin_cur * 2.302585092994045684;
+    *lnf = LIN ? log(fi) : lin_cur * 2.302585092994045684;
     f_cur = f_next;
     lin_cur = lin_next;
   }
@@ -558,15 +573,16 @@ GWF_HD void gram_update(double* G, const double* re, const double* im, double w)
 // arm_sel < 0: the network sum (over net.sum_arm); else only that physical arm.
 // MASKED: the network sum over the physical arms whose bit is set in mask
 // (bit a = physical arm a); a site with all its arms set uses its sum arms.
-template <int MODEL, int NT, int P, bool MASKED = false>
+// LIN: linearly spaced grid (FisherMatr spacing='lin').
+template <int MODEL, int NT, int P, bool MASKED = false, bool LIN = false>
 GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* rec, int res, int lane, int nlanes,
                          int arm_sel, double* G, uint32_t mask = 0u) {
   const double* ev = rec + REC_EV;
-  const double a = log10(net.fmin);
-  const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
+  double a, b;
+  grid_ends(LIN, net.fmin, fcut_eff(ev[EV_FCUT], net.fmax), &a, &b);
   const bool sum_mode = arm_sel < 0;
   const int narm = sum_mode ? net.n_sum_arms : 1;
-  GridWalk gw;
+  GridWalk<LIN> gw;
   gw.init(a, b, res, lane, nlanes);
   for (int base = 0; base < res; base += nlanes) {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
@@ -665,7 +681,7 @@ GWF_HD void snr_event(const gwf_waveform& w, const NetK& net, const double* rec,
   const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
   const double tc = ev[EV_TC], idL = ev[EV_IDL], cp = ev[EV_CP], cc = ev[EV_CC];
   const bool earth = net.any_earth != 0;
-  GridWalk gw;
+  GridWalk<false> gw;
   gw.init(a, b, res, lane, nlanes);
   for (int base = 0; base < res; base += nlanes) {
     const int i = base + lane;

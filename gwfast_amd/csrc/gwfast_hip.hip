@@ -66,8 +66,8 @@ __global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const d
   if (fgrid) {  // minimum of the caller's grid column (waveforms.py:1139)
     f0 = fgrid[e];
     for (int64_t i = 1; i < nf; ++i) f0 = fmin(f0, fgrid[i * n_ev + e]);
-  } else {
-    f0 = grid_f(log10(net.fmin), log10(net.fmin), 0, 1);
+  } else {  // first grid point: 10**log10(fmin) for geomspace, fmin for linspace
+    f0 = net.lin ? net.fmin : grid_f(log10(net.fmin), log10(net.fmin), 0, 1);
   }
   setup_event<MODEL, NT>(w, net, par, ld, e, f0, rec + e * (int64_t)REC_DOUBLES);
 }
@@ -94,7 +94,7 @@ __device__ __forceinline__ void store_packed(const double* G, double* __restrict
   }
 }
 
-template <int MODEL, int NT, int P>
+template <int MODEL, int NT, int P, bool LIN>
 __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
                                                 int arm_sel, double* __restrict__ F) {
   const int64_t e = wave_event();
@@ -104,13 +104,13 @@ __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_wavef
   double G[NG];
 #pragma unroll
   for (int k = 0; k < NG; ++k) G[k] = 0.0;
-  fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, arm_sel, G);
+  fisher_event<MODEL, NT, P, false, LIN>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, arm_sel, G);
   store_packed<P>(G, F, n_ev, e, lane);
 }
 
 // Network Fisher with per-event arm masks (duty cycle): bit a of arm_mask[e]
 // switches physical arm a of the network on for event e.
-template <int MODEL, int NT, int P>
+template <int MODEL, int NT, int P, bool LIN>
 __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher_masked(gwf_waveform w, NetK net,
                                                                              const double* __restrict__ rec,
                                                                              int64_t n_ev,
@@ -124,14 +124,14 @@ __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher_masked(gw
   double G[NG];
 #pragma unroll
   for (int k = 0; k < NG; ++k) G[k] = 0.0;
-  fisher_event<MODEL, NT, P, true>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, -1, G, mask);
+  fisher_event<MODEL, NT, P, true, LIN>(w, net, rec + e * (int64_t)REC_DOUBLES, net.res, lane, 64, -1, G, mask);
   store_packed<P>(G, F, n_ev, e, lane);
 }
 
 // Ragged grids: one persistent launch; each wavefront takes events from an
 // atomic counter (lane 0, vector-memory atomic) until the batch is exhausted,
 // so every wave reaches the exit; res_ev[e] is event e's grid length.
-template <int MODEL, int NT, int P>
+template <int MODEL, int NT, int P, bool LIN>
 __global__ __launch_bounds__(256) void k_fisher_ragged(gwf_waveform w, NetK net, const double* __restrict__ rec,
                                                        int64_t n_ev, const int* __restrict__ res_ev, int arm_sel,
                                                        unsigned int* counter, double* __restrict__ F) {
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(256) void k_fisher_ragged(gwf_waveform w, NetK net,
     for (int k = 0; k < NG; ++k) G[k] = 0.0;
     int res = res_ev[e];
     res = res < 2 ? 2 : res;
-    fisher_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, res, lane, 64, arm_sel, G);
+    fisher_event<MODEL, NT, P, false, LIN>(w, net, rec + e * (int64_t)REC_DOUBLES, res, lane, 64, arm_sel, G);
     store_packed<P>(G, F, n_ev, e, lane);
   }
 }
@@ -195,12 +195,12 @@ __global__ void k_grid(NetK net, const double* __restrict__ rec, int64_t n_ev, i
   if (idx >= (int64_t)res * n_ev) return;
   const int64_t i = idx / n_ev, e = idx % n_ev;
   const double* ev = rec + e * (int64_t)REC_DOUBLES + REC_EV;
-  const double a = log10(net.fmin);
-  const double b = log10(fcut_eff(ev[EV_FCUT], net.fmax));
+  double a, b;
+  grid_ends(net.lin != 0, net.fmin, fcut_eff(ev[EV_FCUT], net.fmax), &a, &b);
   const double lin = grid_lin(a, b, (int)i, res - 1);
-  const double f = exp10d(lin);
+  const double f = net.lin ? lin : exp10d(lin);
   fo[idx] = f;
-  if (So) So[idx] = psd_at(net.psd[0], f, lin * 2.302585092994045684);
+  if (So) So[idx] = psd_at(net.psd[0], f, net.lin ? log(f) : lin * 2.302585092994045684);
 }
 
 template <int MODEL, int NT>
@@ -862,6 +862,57 @@ int nt_of(const gwf_waveform* wf) {
   return ((wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) || wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2) ? 6 : 4;
 }
 
+// Model dispatch of the Fisher kernels for one grid spacing.
+template <bool LIN>
+void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st, const NetK& net, const double* rec,
+                   int64_t n_ev, int arm, double* out) {
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       arm, out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
+  else
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
+}
+
+template <bool LIN>
+void launch_fisher_masked(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st, const NetK& net,
+                          const double* rec, int64_t n_ev, const uint32_t* mask, double* out) {
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       mask, out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net,
+                       rec, n_ev, mask, out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       mask, out);
+  else
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       mask, out);
+}
+
+template <bool LIN>
+void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st, const NetK& net,
+                          const double* rec, int64_t n_ev, const int32_t* res_ev, int arm, unsigned int* counter,
+                          double* out) {
+  if (wf->model == GWF_MODEL_IMRPHENOMD)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       res_ev, arm, counter, out);
+  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net,
+                       rec, n_ev, res_ev, arm, counter, out);
+  else if (nt_of(wf) == 6)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       res_ev, arm, counter, out);
+  else
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 4, 11, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       res_ev, arm, counter, out);
+}
+
+
 int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int64_t ld, int64_t n_ev,
                  const double* fgrid, int64_t nf, double* rec, hipStream_t st) {
   const dim3 grid((unsigned)((n_ev + 255) / 256)), block(256);
@@ -1039,18 +1090,10 @@ int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
   const double* rec = (const double*)workspace;
-  if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
-                       fisher_out);
-  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-    hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
-                       (int)arm, fisher_out);
-  else if (nt_of(wf) == 6)
-    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
-                       fisher_out);
+  if (net.lin)
+    launch_fisher<true>(wf, grid, block, st, net, rec, n_ev, arm, fisher_out);
   else
-    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm,
-                       fisher_out);
+    launch_fisher<false>(wf, grid, block, st, net, rec, n_ev, arm, fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher launch");
 }
 
@@ -1067,18 +1110,10 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
   const double* rec = (const double*)workspace;
-  if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev,
-                       arm_mask, fisher_out);
-  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec,
-                       n_ev, arm_mask, fisher_out);
-  else if (nt_of(wf) == 6)
-    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
-                       arm_mask, fisher_out);
+  if (net.lin)
+    launch_fisher_masked<true>(wf, grid, block, st, net, rec, n_ev, arm_mask, fisher_out);
   else
-    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev,
-                       arm_mask, fisher_out);
+    launch_fisher_masked<false>(wf, grid, block, st, net, rec, n_ev, arm_mask, fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
 }
 
@@ -1128,18 +1163,10 @@ int gwf_fisher_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   // persistent grid: enough waves to fill every SIMD several times, never more than the events
   int64_t blocks = std::min<int64_t>((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK, 256 * 8);
   const dim3 grid((unsigned)blocks), block(BLOCK);
-  if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
-                       (int)arm, counter, fisher_out);
-  else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec,
-                       n_ev, res_ev, (int)arm, counter, fisher_out);
-  else if (nt_of(wf) == 6)
-    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
-                       (int)arm, counter, fisher_out);
+  if (net.lin)
+    launch_fisher_ragged<true>(wf, grid, block, st, net, rec, n_ev, res_ev, arm, counter, fisher_out);
   else
-    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, res_ev,
-                       (int)arm, counter, fisher_out);
+    launch_fisher_ragged<false>(wf, grid, block, st, net, rec, n_ev, res_ev, arm, counter, fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher_ragged launch");
 }
 

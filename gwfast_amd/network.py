@@ -83,12 +83,13 @@ class DetNet(object):
         """network.py:84-152: sum of the detectors' Fisher matrices."""
         self._prepare(evParams)
         res = kwargs.pop('res', 1000)
-        res = int(res) if np.ndim(res) == 0 else np.asarray(res)
+        res = None if res is None else (int(res) if np.ndim(res) == 0 else np.asarray(res))
         use_m1m2 = kwargs.get('use_m1m2', False)
         use_chi1chi2 = kwargs.get('use_chi1chi2', True)
-        S._check_fisher_options(res, kwargs.get('df'), kwargs.get('spacing', 'geom'), use_m1m2, use_chi1chi2,
-                                kwargs.get('computeDerivFinDiff', False))
-        opts = {'use_m1m2': use_m1m2, 'use_chi1chi2': use_chi1chi2}
+        df, spacing = kwargs.get('df'), kwargs.get('spacing', 'geom')
+        S._check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, kwargs.get('computeDerivFinDiff', False))
+        lin = spacing == 'lin'
+        opts = {'use_m1m2': use_m1m2, 'use_chi1chi2': use_chi1chi2, 'df': df, 'spacing': spacing}
         if return_derivatives or return_SNR_derivatives:
             allF, allD = {}, {}
             for name, sig in self.signals.items():
@@ -102,7 +103,8 @@ class DetNet(object):
                     allF[name] = F_[0]
                     allD[name] = D_[0]
             if return_SNR_derivatives:
-                SNRs_net = kwargs['SNRs'] if 'SNRs' in kwargs else self.SNR(evParams, res=res)
+                # the reference's network SNR here is at SNR's default resolution (network.py:139-143)
+                SNRs_net = kwargs['SNRs'] if 'SNRs' in kwargs else self.SNR(evParams, return_all=False)
                 allD['net'] = np.array([allD[k] for k in allD.keys()]).sum(axis=0) / SNRs_net
             allF['net'] = np.array([allF[k] for k in allF.keys()]).sum(axis=0)
             return allF, allD
@@ -115,7 +117,8 @@ class DetNet(object):
                 n = rows.shape[1]
                 masks = [s._duty_masks(n) for s in sigs]
                 masks = [m if m is not None else np.ones((s.n_arms, n)) for m, s in zip(masks, sigs)]
-                F = S.fisher_masked_device(sigs, sigs[0].wf_model, rows, res, S.mask_words(masks))
+                F = S.fisher_masked_device(sigs, sigs[0].wf_model, rows, S.resolve_res(sigs[0], evParams, res, df),
+                                           S.mask_words(masks), lin=lin)
                 total = F if total is None else total + F
             S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
             return total.cpu().numpy()
@@ -134,7 +137,7 @@ class DetNet(object):
         for g in self._groups():
             sigs = [s for _, s in g]
             rows = sigs[0]._rows(evParams)
-            F = S.fisher_device(sigs, sigs[0].wf_model, rows, res)
+            F = S.fisher_device(sigs, sigs[0].wf_model, rows, S.resolve_res(sigs[0], evParams, res, df), lin=lin)
             total = F if total is None else total + F
         S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
         return total.cpu().numpy()

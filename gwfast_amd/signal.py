@@ -75,7 +75,7 @@ class GWSignal(object):
                          for _ in range(self.n_arms)], dtype=np.float64)
 
     # ------------------------------------------------------------------ ABI
-    def c_struct(self):
+    def c_struct(self, lin=False):
         if self._psd_dev is None:
             self._psd_dev = _shared_psd(self.strainFreq, self.noiseCurve)
         d = N.GwfDetector()
@@ -91,6 +91,7 @@ class GWSignal(object):
         d.n_psd = len(self.strainFreq)
         d.psd_f_first = float(self.strainFreq[0])
         d.psd_f_last = float(self.strainFreq[-1])
+        d.lin_spacing = int(bool(lin))
         return d
 
     @property
@@ -137,6 +138,8 @@ class GWSignal(object):
         rounding; the finite-difference option is not provided."""
         _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff)
         self._complete(evParams)
+        res = resolve_res(self, evParams, res, df)
+        lin = spacing == 'lin'
         rows = self._rows(evParams)
         n = rows.shape[1]
         masks = self._duty_masks(n)
@@ -144,7 +147,7 @@ class GWSignal(object):
         if return_derivatives or return_SNR_derivatives:
             if np.ndim(res) != 0:
                 raise NotImplementedError('derivative outputs need one grid length for all events')
-            allF, allD = self._fisher_with_derivs(evParams, rows, int(res), return_derivatives)
+            allF, allD = self._fisher_with_derivs(evParams, rows, int(res), return_derivatives, lin)
             for k in range(len(allF)):
                 allF[k] = reparam_host(allF[k], evParams, use_m1m2, use_chi1chi2)
                 allD[k] = reparam_rows_host(allD[k], evParams, use_m1m2, use_chi1chi2)
@@ -155,20 +158,20 @@ class GWSignal(object):
         if return_all:
             Fs = []
             for k in range(self.n_arms):
-                F = fisher_device([self], self.wf_model, rows, res, arm=k)
+                F = fisher_device([self], self.wf_model, rows, res, arm=k, lin=lin)
                 reparam_device(F, rows, use_m1m2, use_chi1chi2)
                 Fs.append(F.cpu().numpy() * (masks[k] if masks is not None else 1.0))
             return Fs
         if masks is not None:
-            F = fisher_masked_device([self], self.wf_model, rows, res, mask_words([masks]))
+            F = fisher_masked_device([self], self.wf_model, rows, res, mask_words([masks]), lin=lin)
         else:
-            F = fisher_device([self], self.wf_model, rows, res)
+            F = fisher_device([self], self.wf_model, rows, res, lin=lin)
         reparam_device(F, rows, use_m1m2, use_chi1chi2)
         return F.cpu().numpy()
 
-    def _fisher_with_derivs(self, evParams, rows, res, return_derivatives):
+    def _fisher_with_derivs(self, evParams, rows, res, return_derivatives, lin=False):
         n = rows.shape[1]
-        f, S = grid_device(self, rows, res)
+        f, S = grid_device(self, rows, res, lin)
         P = self.wf_model.nParams
         allF, allD = [], []
         for k in range(self.n_arms):
@@ -295,18 +298,33 @@ def reparam_rows_host(D, evParams, use_m1m2, use_chi1chi2):
     return out
 
 
+def resolve_res(sig, evParams, res, df):
+    """signal.py:884-892: res, or from the step df the largest
+    floor(1 + (fcut - fmin)/df) over the events (one grid length for all)."""
+    if res is not None:
+        return res
+    if df is None:
+        raise ValueError('Provide either resolution in frequency or step size.')
+    fcut = np.asarray(sig.wf_model.fcut(**evParams))
+    if sig.fmax is not None:
+        fcut = np.where(fcut > sig.fmax, sig.fmax, fcut)
+    return int(np.amax(np.floor(1 + (fcut - sig.fmin) / df)))
+
+
 def _check_fisher_options(res, df, spacing, use_m1m2, use_chi1chi2, computeDerivFinDiff):
-    if res is None or (np.ndim(res) == 0 and int(res) < 2):
-        raise NotImplementedError('df-spaced grids are not supported: pass res')
-    if spacing != 'geom':
-        raise NotImplementedError("only spacing='geom' is supported")
+    if res is None and df is None:
+        raise ValueError('Provide either resolution in frequency or step size.')
+    if res is not None and np.ndim(res) == 0 and int(res) < 2:
+        raise ValueError('res must be at least 2')
+    if spacing not in ('geom', 'lin'):
+        raise ValueError("spacing must be 'geom' or 'lin'")
     if computeDerivFinDiff:
         raise NotImplementedError('finite-difference derivatives are not provided (derivatives are exact)')
 
 
 # ------------------------------------------------------------ device helpers
-def _dets(signals):
-    arr = (N.GwfDetector * len(signals))(*[s.c_struct() for s in signals])
+def _dets(signals, lin=False):
+    arr = (N.GwfDetector * len(signals))(*[s.c_struct(lin) for s in signals])
     return arr
 
 
@@ -341,8 +359,9 @@ def snr_sq_device(signals, rows, res):
     return out
 
 
-def fisher_device(signals, wf, rows, res, arm=-1, out=None):
-    """Fused network Fisher (P, P, N_ev) device tensor; arm >= 0 selects one arm."""
+def fisher_device(signals, wf, rows, res, arm=-1, out=None, lin=False):
+    """Fused network Fisher (P, P, N_ev) device tensor; arm >= 0 selects one
+    arm; lin: linearly spaced grids (FisherMatr spacing='lin')."""
     t = E.torch()
     n = rows.shape[1]
     P = wf.nParams
@@ -351,10 +370,10 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None):
     ws, nb = E.workspace(n)
     r, r_ev = _res_arg(res, n, rows.device)
     if r_ev is None:
-        E.call('gwf_fisher', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, r, arm,
+        E.call('gwf_fisher', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n, r, arm,
                N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     else:
-        E.call('gwf_fisher_ragged', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n,
+        E.call('gwf_fisher_ragged', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n,
                N.ptr(r_ev), arm, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
@@ -375,7 +394,7 @@ def mask_words(masks_per_signal):
     return E.to_dev(words.astype(np.int32), E.torch().int32)
 
 
-def fisher_masked_device(signals, wf, rows, res, words):
+def fisher_masked_device(signals, wf, rows, res, words, lin=False):
     """Network Fisher (P, P, N_ev) device tensor summing, per event, the arms
     whose bit is set in words (one fused launch)."""
     t = E.torch()
@@ -386,18 +405,18 @@ def fisher_masked_device(signals, wf, rows, res, words):
         raise TypeError('arm masks must be an int32 device tensor of one word per event')
     out = t.empty((wf.nParams, wf.nParams, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    E.call('gwf_fisher_masked', wf.c_struct(), _dets(signals), len(signals), N.ptr(rows), rows.stride(0), n, int(res),
+    E.call('gwf_fisher_masked', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n, int(res),
            N.ptr(words), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
 
-def grid_device(sig, rows, res):
+def grid_device(sig, rows, res, lin=False):
     t = E.torch()
     n = rows.shape[1]
     f = t.empty((res, n), dtype=t.float64, device=rows.device)
     S = t.empty((res, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    E.call('gwf_grid', sig.wf_model.c_struct(), _dets([sig]), N.ptr(rows), rows.stride(0), n, res, N.ptr(f), N.ptr(S),
+    E.call('gwf_grid', sig.wf_model.c_struct(), _dets([sig], lin), N.ptr(rows), rows.stride(0), n, res, N.ptr(f), N.ptr(S),
            N.ptr(ws), nb, N.stream_ptr())
     return f, S
 

@@ -75,7 +75,9 @@ typedef struct gwf_detector {
   const double* psd_f;  /* device, strictly increasing frequencies [Hz] */
   const double* psd_S;  /* device, PSD (ASD^2 when the file is an ASD) */
   int32_t n_psd;
-  int32_t reserved;
+  int32_t lin_spacing;  /* Fisher grids (gwf_fisher*, gwf_grid): 0 geomspace, 1 linspace
+                           (FisherMatr spacing, signal.py:893-896); SNR grids are always
+                           geometric (signal.py:720-721) */
   double psd_f_first;   /* host copy of psd_f[0] */
   double psd_f_last;    /* host copy of psd_f[n_psd - 1] */
 } gwf_detector;

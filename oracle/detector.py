@@ -109,12 +109,16 @@ class Detector:
         return (Ap + 1j * Ac) * D.exp(Psi * 1j)
 
     # -- integrals -------------------------------------------------------------
-    def grid(self, wf, ev, res):
+    def grid(self, wf, ev, res, spacing='geom'):
+        """signal.py:884-901: geometric (or linear) grids fmin..fcut, (res, N_ev)."""
         fcut = wf.fcut(ev)
         if self.fmax is not None:
             fcut = np.where(fcut > self.fmax, self.fmax, fcut)
         fminarr = np.full(np.shape(fcut), self.fmin)
-        fgrids = geomspace_jax(fminarr, fcut, int(res))
+        if spacing == 'lin':
+            fgrids = linspace_jax(fminarr, fcut, int(res))
+        else:
+            fgrids = geomspace_jax(fminarr, fcut, int(res))
         strainGrids = np.interp(fgrids, self.strainFreq, self.noiseCurve, left=1., right=1.)
         return fgrids, strainGrids
 
@@ -306,13 +310,13 @@ class Detector:
                 F[b, a] = F[a, b]
         return F
 
-    def fisher_arms(self, wf, ev, res=1000, mode='faithful', use_m1m2=False, use_chi1chi2=True):
+    def fisher_arms(self, wf, ev, res=1000, mode='faithful', use_m1m2=False, use_chi1chi2=True, spacing='geom'):
         """Per-arm Fisher matrices (signal.py:782-1099), list of (P, P, N_ev).
 
         mode='faithful' follows the reference's derivative construction (AD
         rows through exp(i Psi), closed-form rows with their own phase
         association); mode='dual' takes every row from duals of GWstrain."""
-        fgrids, strainGrids = self.grid(wf, ev, res)
+        fgrids, strainGrids = self.grid(wf, ev, res, spacing)
         P = wf.nParams
         base = self.signal_derivatives if mode == 'faithful' else self.full_dual_derivatives
 
@@ -355,6 +359,16 @@ def _seed_masses_spins(evd, ev, K, use_m1m2, use_chi1chi2, slots=(0, 1, 2, 3)):
         cS = D.Dual.seed(0.5 * (ev['chi1z'] + ev['chi2z']), s2, K)
         cA = D.Dual.seed(0.5 * (ev['chi1z'] - ev['chi2z']), s3, K)
         evd['chi1z'], evd['chi2z'] = cS + cA, cS - cA
+
+
+def linspace_jax(start, stop, num):
+    """jnp.linspace(start, stop, num) along axis 0 (JAX 0.4.30): interior
+    points start*(1-s) + stop*s with s = i/(num-1), the last point stop."""
+    start = np.asarray(start, dtype=float)
+    stop = np.asarray(stop, dtype=float)
+    div = num - 1
+    s = (np.arange(div, dtype=float) / div)[:, None]
+    return np.concatenate([start[None, :] * (1 - s) + stop[None, :] * s, stop[None, :]], axis=0)
 
 
 def geomspace_jax(start, stop, num):

@@ -254,3 +254,34 @@ def test_covmatr_methods_vs_oracle(gpu, case):
     print(name, 'cov err %.3e, eps %s vs %s' % (err, eps, epso), flush=True)
     assert err <= 1e-9
     assert np.allclose(eps, epso, rtol=1e-3, atol=1e-10)
+
+
+@pytest.mark.parametrize('model', ['TF2', 'PD'])
+def test_fisher_linear_spacing_and_df(gpu, model):
+    """FisherMatr(spacing='lin') (signal.py:893-894) against the oracle on
+    linear grids, and the df option (884-892): one grid length, the largest
+    floor(1 + (fcut - fmin)/df) over the events."""
+    from oracle import detector as OD
+    wf, owf = _models(model)
+    ev = bbh(6, seed=71)
+    sites = NET_C1 if model == 'PD' else NET_C1[1:2]
+    net = hip_net(wf, sites, True)
+    odets = oracle_net(sites, True)
+    F = net.FisherMatr(dict(ev), res=1500, spacing='lin')
+    Fd = OD.network_fisher(odets, owf, ev, 1500, mode='dual', spacing='lin')
+    e = norm_err(F, Fd)
+    print(model, 'lin-spaced Fisher err %.3e' % e, flush=True)
+    assert e <= 1e-10
+    assert norm_err(F, net.FisherMatr(dict(ev), res=1500)) > 1e-6   # a different grid gives a different matrix
+    df = 2.0
+    fcut = owf.fcut(ev)
+    want_res = int(np.amax(np.floor(1 + (fcut - 2.0) / df)))
+    Fdf = net.FisherMatr(dict(ev), res=None, df=df, spacing='lin')
+    assert norm_err(Fdf, net.FisherMatr(dict(ev), res=want_res, spacing='lin')) == 0.0
+    sig = next(iter(net.signals.values()))
+    Fs, Ds = sig.FisherMatr(dict(ev), res=300, spacing='lin', return_derivatives=True)
+    f, _ = odets[0].grid(owf, ev, 300, spacing='lin')
+    Do = odets[0].full_dual_derivatives(owf, ev, f)
+    Do[7] /= 86400.
+    scale = np.max(np.abs(Do), axis=(1, 2), keepdims=True)
+    assert np.max(np.abs(Ds[0] - Do) / scale) <= 1e-5

@@ -745,3 +745,13 @@ def test_cov_lu_composition_as_written():
     assert np.allclose(cc @ Pn @ Pn, np.linalg.inv(A), rtol=1e-12, atol=1e-14)
     ci, _ = OC.invert_scaled(M, 'inv')
     assert np.allclose(np.array(ci.tolist(), dtype=float), np.linalg.inv(A), rtol=1e-12, atol=1e-14)
+
+
+def test_linspace_jax_grid():
+    """Linear Fisher grids (signal.py:893-894): end points exact, uniform
+    steps, equal to numpy's linspace to rounding."""
+    a, b = np.array([2.0, 3.5]), np.array([120.0, 2000.0])
+    g = OD.linspace_jax(a, b, 101)
+    assert np.array_equal(g[0], a) and np.array_equal(g[-1], b)
+    assert np.allclose(np.diff(g, axis=0), (b - a) / 100, rtol=1e-12, atol=0)
+    assert np.allclose(g, np.linspace(a, b, 101), rtol=1e-15, atol=0)
