@@ -740,7 +740,8 @@ GWF_HD void snr_event(const gwf_waveform& w, const NetK& net, const double* rec,
 // stride js (point stride 1 pair).
 template <int MODEL, int NT, int P>
 GWF_HD void strain_event(const gwf_waveform& w, const NetK& net, const double* rec, int armi, const double* f, int64_t fs,
-                         int64_t nf, int lane, int nlanes, double* h, int64_t hs, double* J, int64_t js) {
+                         int64_t nf, int lane, int nlanes, double* h, int64_t hs, double* J, int64_t js,
+                         double* comp = nullptr, int64_t cis = 0, int64_t ccs = 0) {
   const double* ev = rec + REC_EV;
   const ArmK& arm = net.arm[armi];
   const int s = arm.site;
@@ -758,6 +759,12 @@ GWF_HD void strain_event(const gwf_waveform& w, const NetK& net, const double* r
     double re[P], im[P], Ap, Ac;
     arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, fi, re, im, &Ap, &Ac);
     const double psi = (2. * PI * fi * (tc * 3600. * 24.) - phic - wp.Phi.v) + 0. + (2. * PI * fi) * g.dt;
+    if (comp) {  // GWstrain(return_single_comp=...): Ap, Ac and the phase Psi
+      comp[i * cis] = Ap;
+      comp[ccs + i * cis] = Ac;
+      comp[2 * ccs + i * cis] = psi;
+    }
+    if (!h && !J) continue;
     double sp, cps;
     sincos_d(psi, &sp, &cps);
     if (h) {

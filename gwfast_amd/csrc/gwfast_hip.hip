@@ -181,12 +181,14 @@ __global__ __launch_bounds__(256, 3) void k_snr(gwf_waveform w, NetK net, const 
 template <int MODEL, int NT, int P>
 __global__ __launch_bounds__(256) void k_strain(gwf_waveform w, NetK net, const double* __restrict__ rec, int64_t n_ev,
                                                 int armi, const double* __restrict__ f, int64_t nf,
-                                                double* __restrict__ h, double* __restrict__ J) {
+                                                double* __restrict__ h, double* __restrict__ J,
+                                                double* __restrict__ comp) {
   const int64_t e = wave_event();
   if (e >= n_ev) return;
   const int lane = threadIdx.x & 63;
   strain_event<MODEL, NT, P>(w, net, rec + e * (int64_t)REC_DOUBLES, armi, f + e, n_ev, nf, lane, 64,
-                             h ? h + 2 * e : nullptr, 2 * n_ev, J ? J + 2 * e * nf : nullptr, 2 * n_ev * nf);
+                             h ? h + 2 * e : nullptr, 2 * n_ev, J ? J + 2 * e * nf : nullptr, 2 * n_ev * nf,
+                             comp ? comp + e : nullptr, n_ev, nf * n_ev);
 }
 
 __global__ void k_grid(NetK net, const double* __restrict__ rec, int64_t n_ev, int res, double* __restrict__ fo,
@@ -1170,9 +1172,9 @@ int gwf_fisher_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   return hip_check(hipGetLastError(), "gwf_fisher_ragged launch");
 }
 
-int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,
-               int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
-               size_t ws_bytes, void* stream) {
+static int strain_impl(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,
+                       int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, double* comp_out,
+                       void* workspace, size_t ws_bytes, void* stream) {
   int rc = check_common(wf, params, ld, n_ev, workspace, ws_bytes);
   if (rc) return rc;
   NetK net;
@@ -1188,17 +1190,30 @@ int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, con
   const double* rec = (const double*)workspace;
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f,
-                       n_f, h_out, J_out);
+                       n_f, h_out, J_out, comp_out);
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
-                       (int)arm, f, n_f, h_out, J_out);
+                       (int)arm, f, n_f, h_out, J_out, comp_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
-                       h_out, J_out);
+                       h_out, J_out, comp_out);
   else
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 4, 11>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
-                       h_out, J_out);
+                       h_out, J_out, comp_out);
   return hip_check(hipGetLastError(), "gwf_strain launch");
+}
+
+int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params, int64_t ld,
+               int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
+               size_t ws_bytes, void* stream) {
+  return strain_impl(wf, det, arm, params, ld, n_ev, f, n_f, h_out, J_out, nullptr, workspace, ws_bytes, stream);
+}
+
+int gwf_strain_components(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params,
+                          int64_t ld, int64_t n_ev, const double* f, int64_t n_f, double* comp_out, void* workspace,
+                          size_t ws_bytes, void* stream) {
+  if (!comp_out && n_ev > 0 && n_f > 0) return fail(-EINVAL, "comp_out is NULL");
+  return strain_impl(wf, det, arm, params, ld, n_ev, f, n_f, nullptr, nullptr, comp_out, workspace, ws_bytes, stream);
 }
 
 int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, int64_t n_f, double tcoal_scale,

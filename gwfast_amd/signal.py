@@ -10,7 +10,7 @@ import numpy as np
 
 from . import _engine as E
 from . import _native as N
-from .gwfastUtils import check_evparams, Lam12_from_Lamt_delLam
+from .gwfastUtils import check_evparams, Lam12_from_Lamt_delLam, Mceta_from_m1m2
 
 
 class GWSignal(object):
@@ -196,9 +196,16 @@ class GWSignal(object):
                  chi1y=0., chi2y=0., LambdaTilde=0., deltaLambda=0., ecc=0., rot=0., is_m1m2=False,
                  is_chi1chi2=False, is_prec_ang=False, is_Lam1Lam2=False, return_single_comp=None):
         """signal.py:485-655: complex strain on the grid f (N_f, N_ev) for the
-        arm at rotation rot (0, 60, 120 degrees)."""
-        if is_m1m2 or is_prec_ang or return_single_comp is not None:
-            raise NotImplementedError('is_m1m2, is_prec_ang and return_single_comp are not on this path')
+        arm at rotation rot (0, 60, 120 degrees); is_m1m2 reads (Mc, eta) as
+        (m1, m2); return_single_comp in 'Ap', 'Ac', 'Psip', 'Psic', 'At',
+        'Psit' returns that real component instead (629-652)."""
+        if is_prec_ang:
+            raise NotImplementedError('is_prec_ang needs a precessing model, which is not on this path')
+        comps = ('Ap', 'Ac', 'Psip', 'Psic', 'At', 'Psit')
+        if return_single_comp is not None and return_single_comp not in comps:
+            raise ValueError('Single component to return has to be among Ap, Ac, Psip, Psic')
+        if is_m1m2:
+            Mc, eta = Mceta_from_m1m2(np.asarray(Mc, dtype=np.float64), np.asarray(eta, dtype=np.float64))
         chi1z, chi2z = (chiS, chiA) if is_chi1chi2 else (np.asarray(chiS) + chiA, np.asarray(chiS) - chiA)
         ev = {'Mc': Mc, 'eta': eta, 'dL': dL, 'theta': theta, 'phi': phi, 'iota': iota, 'psi': psi,
               'tcoal': tcoal, 'Phicoal': Phicoal, 'chi1z': chi1z, 'chi2z': chi2z}
@@ -216,6 +223,13 @@ class GWSignal(object):
         if f2.ndim == 1:
             f2 = f2[:, None] * np.ones((1, rows.shape[1]))
         fd = E.to_dev(f2)
+        if return_single_comp is not None:
+            t = E.torch()
+            c = strain_components_device(self, rows, fd, arm)
+            Ap, Ac, Psi = c[0], c[1], c[2]
+            out = {'Ap': lambda: Ap, 'Ac': lambda: Ac, 'Psip': lambda: Psi, 'Psic': lambda: Psi + np.pi * 0.5,
+                   'At': lambda: t.hypot(Ap, Ac), 'Psit': lambda: Psi + t.atan2(Ac, Ap)}[return_single_comp]()
+            return out.cpu().numpy()
         h, _ = strain_device(self, rows, fd, arm, want_h=True, want_J=False, null_stream=False)
         hc = h.cpu().numpy()
         return hc[..., 0] + 1j * hc[..., 1]
@@ -436,6 +450,20 @@ def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True)
     E.call('gwf_strain', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
            N.ptr(h), N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
     return h, J
+
+
+def strain_components_device(sig, rows, f, arm):
+    """(Ap, Ac, Psi) (3, N_f, N_ev) device tensor for one physical arm."""
+    t = E.torch()
+    n = rows.shape[1]
+    nf = f.shape[0]
+    out = t.empty((3, nf, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    det = sig.c_struct()
+    det.compute2arms = 0
+    E.call('gwf_strain_components', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n,
+           N.ptr(f), nf, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    return out
 
 
 def strain_device_into(sig, rows, f, arm, J):

@@ -174,6 +174,14 @@ int gwf_strain(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, con
                int64_t n_ev, const double* f, int64_t n_f, double* h_out, double* J_out, void* workspace,
                size_t ws_bytes, void* stream);
 
+/* GWstrain(..., return_single_comp=...) (signal.py:629-652): the plus and
+ * cross amplitudes Ap, Ac and the phase Psi = 2 pi f tcoal - Phicoal -
+ * Phi_GW + 2 pi f Delta t of one arm on the caller grid f (n_f, n_ev);
+ * comp_out (3, n_f, n_ev) = (Ap, Ac, Psi). */
+int gwf_strain_components(const gwf_waveform* wf, const gwf_detector* det, int32_t arm, const double* params,
+                          int64_t ld, int64_t n_ev, const double* f, int64_t n_f, double* comp_out, void* workspace,
+                          size_t ws_bytes, void* stream);
+
 /* Vector-Jacobian product of the strain for reverse-mode callers (no
  * reference counterpart; JAX does this inside jacrev, signal.py:1178-1187):
  * g_out[p][e] = sum_f (gh[f][e].re J[p][e][f].re + gh[f][e].im J[p][e][f].im)

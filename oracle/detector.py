@@ -99,13 +99,17 @@ class Detector:
         """signal.py:468-483."""
         return 2. * np.pi * f * (ev['tcoal'] * 3600. * 24.) - ev['Phicoal'] - wf.Phi(f, ev)
 
-    def gw_strain(self, wf, ev, f, rot=0.):
-        """signal.py:485-655 for aligned-spin, non-LAL models: complex strain."""
+    def strain_components(self, wf, ev, f, rot=0.):
+        """signal.py:629-633: Ap, Ac and the total phase Psi of GWstrain."""
         _, dt, _ = self._times(wf, ev, f)
         phiL = (2. * np.pi * f) * dt
         Ap, Ac = self.gw_amplitudes(wf, ev, f, rot=rot)
         Psi = self.gw_phase(wf, ev, f)
-        Psi = Psi + 0. + phiL
+        return Ap, Ac, Psi + 0. + phiL
+
+    def gw_strain(self, wf, ev, f, rot=0.):
+        """signal.py:485-655 for aligned-spin, non-LAL models: complex strain."""
+        Ap, Ac, Psi = self.strain_components(wf, ev, f, rot)
         return (Ap + 1j * Ac) * D.exp(Psi * 1j)
 
     # -- integrals -------------------------------------------------------------

@@ -285,3 +285,34 @@ def test_fisher_linear_spacing_and_df(gpu, model):
     Do[7] /= 86400.
     scale = np.max(np.abs(Do), axis=(1, 2), keepdims=True)
     assert np.max(np.abs(Ds[0] - Do) / scale) <= 1e-5
+
+
+def test_gwstrain_single_components_and_m1m2(gpu):
+    """GWstrain(return_single_comp=...) (signal.py:629-652) against the
+    oracle's amplitudes and phase, and is_m1m2 (522-524)."""
+    from gwfast_amd.gwfastUtils import m1m2_from_Mceta
+    wf, owf = _models('PD')
+    ev = bbh(5, seed=81)
+    sites = NET_C1[:1]
+    sig = hip_net(wf, sites, True).signals['ETS']
+    od = oracle_net(sites, True)[0]
+    f, _ = od.grid(owf, ev, 400)
+    args = (ev['Mc'], ev['eta'], ev['dL'], ev['theta'], ev['phi'], ev['iota'], ev['psi'], ev['tcoal'], ev['Phicoal'],
+            ev['chi1z'], ev['chi2z'])
+    Ap, Ac, Psi = od.strain_components(owf, ev, f, rot=60.)
+    got = {c: sig.GWstrain(f, *args, rot=60., is_chi1chi2=True, return_single_comp=c)
+           for c in ('Ap', 'Ac', 'Psip', 'Psic', 'At', 'Psit')}
+    amax = np.max(np.hypot(Ap, Ac))
+    assert np.max(np.abs(got['Ap'] - Ap)) <= 1e-12 * amax
+    assert np.max(np.abs(got['Ac'] - Ac)) <= 1e-12 * amax
+    assert np.max(np.abs(got['At'] - np.hypot(Ap, Ac))) <= 1e-12 * amax
+    tol = 1e-13 * np.max(np.abs(Psi)) + 1e-6      # |Psi| reaches 1e9 rad: compare to its rounding
+    assert np.max(np.abs(got['Psip'] - Psi)) <= tol
+    assert np.max(np.abs(got['Psic'] - (Psi + np.pi / 2))) <= tol
+    assert np.max(np.abs(got['Psit'] - (Psi + np.arctan2(Ac, Ap)))) <= tol
+    with pytest.raises(ValueError):
+        sig.GWstrain(f, *args, rot=60., is_chi1chi2=True, return_single_comp='Phi')
+    m1, m2 = m1m2_from_Mceta(ev['Mc'], ev['eta'])
+    h1 = sig.GWstrain(f, *args, is_chi1chi2=True)
+    h2 = sig.GWstrain(f, m1, m2, *args[2:], is_chi1chi2=True, is_m1m2=True)
+    assert np.max(np.abs(h1 - h2)) <= 1e-6 * np.max(np.abs(h1))
