@@ -69,6 +69,9 @@ SIGNATURES = {
                                   c_int64, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     'gwf_strain_components': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,
                                              c_void_p, c_int64, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'gwf_grid_fcut': (ctypes.c_int, [_P(GwfDetector), c_void_p, c_int64, c_int32, c_void_p, c_void_p, c_void_p]),
+    'gwf_overlap': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p,
+                                   c_void_p]),
     'gwf_strain_vjp': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_int64, c_double, c_void_p, c_void_p]),
     'gwf_gram': (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p,
                                 c_void_p]),

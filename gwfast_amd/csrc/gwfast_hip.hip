@@ -205,6 +205,51 @@ __global__ void k_grid(NetK net, const double* __restrict__ rec, int64_t n_ev, i
   if (So) So[idx] = psd_at(net.psd[0], f, net.lin ? log(f) : lin * 2.302585092994045684);
 }
 
+// Grid fmin..fcut[e] (geomspace, or linspace when net.lin) and the PSD on it.
+__global__ void k_grid_fcut(NetK net, const double* __restrict__ fcut, int64_t n_ev, int res, double* __restrict__ fo,
+                            double* __restrict__ So) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)res * n_ev) return;
+  const int64_t i = idx / n_ev, e = idx % n_ev;
+  double a, b;
+  grid_ends(net.lin != 0, net.fmin, fcut[e], &a, &b);
+  const double lin = grid_lin(a, b, (int)i, res - 1);
+  const double f = net.lin ? lin : exp10d(lin);
+  fo[idx] = f;
+  if (So) So[idx] = psd_at(net.psd[0], f, net.lin ? log(f) : lin * 2.302585092994045684);
+}
+
+// Overlap integrals of two strains on one grid, one wave per event:
+// out[0] += 4 trapz(Re(h1 conj h2)/S), out[1] += 4 trapz(|h1|^2/S),
+// out[2] += 4 trapz(|h2|^2/S) (signal.py:1870-1920).  h: (nf, n_ev) complex.
+__global__ __launch_bounds__(256) void k_overlap(const double* __restrict__ h1, const double* __restrict__ h2,
+                                                 const double* __restrict__ f, const double* __restrict__ S,
+                                                 int64_t n_ev, int64_t nf, int accumulate, double* __restrict__ out) {
+  const int64_t e = wave_event();
+  if (e >= n_ev) return;
+  const int lane = threadIdx.x & 63;
+  double o12 = 0.0, o11 = 0.0, o22 = 0.0;
+  for (int64_t i = lane; i < nf; i += 64) {
+    const double fi = f[i * n_ev + e];
+    const double fl = i > 0 ? f[(i - 1) * n_ev + e] : fi;
+    const double fr = i < nf - 1 ? f[(i + 1) * n_ev + e] : fi;
+    const double w = 4.0 * 0.5 * (fr - fl) / S[i * n_ev + e];
+    const double ar = h1[2 * (i * n_ev + e)], ai = h1[2 * (i * n_ev + e) + 1];
+    const double br = h2[2 * (i * n_ev + e)], bi = h2[2 * (i * n_ev + e) + 1];
+    o12 += w * (ar * br + ai * bi);
+    o11 += w * (ar * ar + ai * ai);
+    o22 += w * (br * br + bi * bi);
+  }
+  o12 = wave_sum(o12);
+  o11 = wave_sum(o11);
+  o22 = wave_sum(o22);
+  if (lane == 0) {
+    out[e] = accumulate ? out[e] + o12 : o12;
+    out[n_ev + e] = accumulate ? out[n_ev + e] + o11 : o11;
+    out[2 * n_ev + e] = accumulate ? out[2 * n_ev + e] + o22 : o22;
+  }
+}
+
 template <int MODEL, int NT>
 __global__ void k_wfeval(gwf_waveform w, const double* __restrict__ rec, int64_t n_ev, const double* __restrict__ f,
                          int64_t nf, double* __restrict__ Ao, double* __restrict__ Po, double* __restrict__ To) {
@@ -996,6 +1041,32 @@ int gwf_grid(const gwf_waveform* wf, const gwf_detector* det, const double* para
   hipLaunchKernelGGL(k_grid, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, net, (const double*)workspace, n_ev,
                      (int)res, f_out, S_out);
   return hip_check(hipGetLastError(), "gwf_grid launch");
+}
+
+int gwf_grid_fcut(const gwf_detector* det, const double* fcut, int64_t n_ev, int32_t res, double* f_out,
+                  double* S_out, void* stream) {
+  if (res < 2) return fail(-EINVAL, "res must be >= 2");
+  if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
+  if (n_ev == 0) return 0;
+  if (!fcut || !f_out) return fail(-EINVAL, "NULL array");
+  NetK net;
+  int rc = build_net_checked(det, 1, res, net);
+  if (rc) return rc;
+  const int64_t tot = (int64_t)res * n_ev;
+  hipLaunchKernelGGL(k_grid_fcut, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, net, fcut,
+                     n_ev, (int)res, f_out, S_out);
+  return hip_check(hipGetLastError(), "gwf_grid_fcut launch");
+}
+
+int gwf_overlap(const double* h1, const double* h2, const double* f, const double* S, int64_t n_ev, int64_t n_f,
+                int32_t accumulate, double* out, void* stream) {
+  if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "invalid sizes");
+  if (n_ev == 0) return 0;
+  if (!h1 || !h2 || !f || !S || !out) return fail(-EINVAL, "NULL array");
+  if (n_ev > (int64_t)0x7fffffff / WAVES_PER_BLOCK) return fail(-EINVAL, "too many events for one call");
+  hipLaunchKernelGGL(k_overlap, dim3((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), dim3(BLOCK), 0,
+                     (hipStream_t)stream, h1, h2, f, S, n_ev, n_f, (int)accumulate, out);
+  return hip_check(hipGetLastError(), "gwf_overlap launch");
 }
 
 int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, int64_t n_ev, const double* f,

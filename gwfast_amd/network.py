@@ -142,6 +142,19 @@ class DetNet(object):
         S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
         return total.cpu().numpy()
 
+    def WFOverlap(self, WF1, WF2, evParams1, evParams2, res=1000, **kwargs):
+        """network.py:198-232: overlap summed over the detectors, normalised by
+        the network SNRs of the two waveforms."""
+        overlap_all = 0.
+        SNR1_all = 0.
+        SNR2_all = 0.
+        for sig in self.signals.values():
+            overlap_int, SNR1, SNR2 = sig.WFOverlap(WF1, WF2, evParams1, evParams2, res=res, return_separate=True)
+            overlap_all = overlap_all + overlap_int
+            SNR1_all = SNR1_all + SNR1 ** 2
+            SNR2_all = SNR2_all + SNR2 ** 2
+        return overlap_all / np.sqrt(SNR1_all * SNR2_all)
+
     # ------------------------------------------------------- device-resident
     def fisher_device(self, rows, res=1000, out=None):
         """Network Fisher of device-resident parameter rows (P_rows, N_ev):

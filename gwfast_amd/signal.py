@@ -215,14 +215,9 @@ class GWSignal(object):
             else:
                 ev['LambdaTilde'], ev['deltaLambda'] = LambdaTilde, deltaLambda
         ev = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in ev.items()}
-        arm = int(round(rot / 60.))
-        if abs(rot - 60. * arm) > 1e-12 or arm not in range(self.n_arms):
-            raise ValueError('rot must be 0 (L) or 0, 60, 120 degrees (T)')
+        arm = _arm_of(self, rot)
         rows = self._rows(ev)
-        f2 = np.asarray(f, dtype=np.float64)
-        if f2.ndim == 1:
-            f2 = f2[:, None] * np.ones((1, rows.shape[1]))
-        fd = E.to_dev(f2)
+        fd = _grid_dev(f, rows.shape[1])
         if return_single_comp is not None:
             t = E.torch()
             c = strain_components_device(self, rows, fd, arm)
@@ -233,6 +228,49 @@ class GWSignal(object):
         h, _ = strain_device(self, rows, fd, arm, want_h=True, want_J=False, null_stream=False)
         hc = h.cpu().numpy()
         return hc[..., 0] + 1j * hc[..., 1]
+
+    def GWAmplitudes(self, evParams, f, rot=0.):
+        """signal.py:425-466: plus and cross amplitudes (Ap, Ac) of the arm at
+        rotation rot on the grid f (N_f, N_ev)."""
+        ev = self._complete(dict(evParams))
+        arm = _arm_of(self, rot)
+        rows = self._rows(ev)
+        c = strain_components_device(self, rows, _grid_dev(f, rows.shape[1]), arm)
+        return c[0].cpu().numpy(), c[1].cpu().numpy()
+
+    def WFOverlap(self, WF1, WF2, evParams1, evParams2, res=1000, return_separate=False, **kwargs):
+        """signal.py:1759-1935: normalised overlap of the strains of WF1 on
+        evParams1 and WF2 on evParams2, summed over the arms, on one grid from
+        fmin to the larger cut; return_separate gives (overlap integral,
+        SNR1, SNR2)."""
+        t = E.torch()
+        ev1, ev2 = _complete_for(WF1, evParams1), _complete_for(WF2, evParams2)
+        check_evparams(ev1)
+        check_evparams(ev2)
+        fcut1 = np.asarray(WF1.fcut(**ev1))
+        fcut2 = np.asarray(WF2.fcut(**ev2))
+        fcutUse = np.where(fcut1 > fcut2, fcut1, fcut2)
+        if self.fmax is not None:
+            # as the reference writes it (signal.py:1858-1859): fcut1 where below fmax
+            fcutUse = np.where(fcutUse > self.fmax, self.fmax, fcut1)
+        r1 = E.to_dev(E.param_rows(ev1, WF1.is_tidal))
+        r2 = E.to_dev(E.param_rows(ev2, WF2.is_tidal))
+        n = r1.shape[1]
+        f = t.empty((int(res), n), dtype=t.float64, device=r1.device)
+        S = t.empty_like(f)
+        E.call('gwf_grid_fcut', ctypes_byref(self.c_struct()), N.ptr(E.to_dev(fcutUse)), n, int(res), N.ptr(f), N.ptr(S),
+               N.stream_ptr())
+        out = t.empty((3, n), dtype=t.float64, device=r1.device)
+        for k in range(self.n_arms):
+            h1, _ = strain_device(self, r1, f, k, want_J=False, wf=WF1)
+            h2, _ = strain_device(self, r2, f, k, want_J=False, wf=WF2)
+            E.call('gwf_overlap', N.ptr(h1), N.ptr(h2), N.ptr(f), N.ptr(S), n, int(res), int(k > 0), N.ptr(out),
+                   N.stream_ptr())
+        o = out.cpu().numpy()
+        overlap_int, SNRh1, SNRh2 = o[0], np.sqrt(o[1]), np.sqrt(o[2])
+        if return_separate:
+            return overlap_int, SNRh1, SNRh2
+        return overlap_int / (SNRh1 * SNRh2)
 
     def GWPhase(self, evParams, f):
         """signal.py:468-483: 2 pi f tcoal - Phicoal - Phi_GW."""
@@ -435,19 +473,49 @@ def grid_device(sig, rows, res, lin=False):
     return f, S
 
 
-def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True):
-    """h (N_f, N_ev, 2) and J (P, N_ev, N_f, 2) device tensors for one arm."""
+def _arm_of(sig, rot):
+    arm = int(round(rot / 60.))
+    if abs(rot - 60. * arm) > 1e-12 or arm not in range(sig.n_arms):
+        raise ValueError('rot must be 0 (L) or 0, 60, 120 degrees (T)')
+    return arm
+
+
+def _grid_dev(f, n):
+    f2 = np.asarray(f, dtype=np.float64)
+    if f2.ndim == 1:
+        f2 = f2[:, None] * np.ones((1, n))
+    return E.to_dev(f2)
+
+
+def _complete_for(wf, evParams):
+    """The input completion of WFOverlap for one model (signal.py:1776-1812)."""
+    ev = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in evParams.items()}
+    if 'chi1z' not in ev:
+        try:
+            ev['chi1z'] = ev['chiS'] + ev['chiA']
+            ev['chi2z'] = ev['chiS'] - ev['chiA']
+        except KeyError:
+            raise ValueError('Two among chi1z, chi2z and chiS, chiA have to be provided.')
+    if wf.is_tidal and 'LambdaTilde' not in ev and 'Lambda1' not in ev:
+        raise ValueError('Two among Lambda1, Lambda2 and LambdaTilde and deltaLambda have to be provided.')
+    return ev
+
+
+def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True, wf=None):
+    """h (N_f, N_ev, 2) and J (P, N_ev, N_f, 2) device tensors for one arm
+    (of the model wf, the signal's own by default)."""
     t = E.torch()
+    wf = sig.wf_model if wf is None else wf
     n = rows.shape[1]
     nf = f.shape[0]
-    P = sig.wf_model.nParams
+    P = wf.nParams
     h = t.empty((nf, n, 2), dtype=t.float64, device=rows.device) if want_h else None
     J = t.empty((P, n, nf, 2), dtype=t.float64, device=rows.device) if want_J else None
     ws, nb = E.workspace(n)
     det = sig.c_struct()
     if not null_stream:
         det.compute2arms = 0
-    E.call('gwf_strain', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
+    E.call('gwf_strain', wf.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
            N.ptr(h), N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
     return h, J
 

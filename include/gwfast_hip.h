@@ -182,6 +182,19 @@ int gwf_strain_components(const gwf_waveform* wf, const gwf_detector* det, int32
                           int64_t ld, int64_t n_ev, const double* f, int64_t n_f, double* comp_out, void* workspace,
                           size_t ws_bytes, void* stream);
 
+/* The common grid of GWSignal.WFOverlap (signal.py:1852-1863): geomspace
+ * (linspace when det->lin_spacing) from det->fmin to fcut[e], and the PSD
+ * interpolated on it.  f_out, S_out (res, n_ev). */
+int gwf_grid_fcut(const gwf_detector* det, const double* fcut, int64_t n_ev, int32_t res, double* f_out,
+                  double* S_out, void* stream);
+
+/* The overlap integrals of GWSignal.WFOverlap for one arm (signal.py:
+ * 1870-1920): out (3, n_ev) = 4 trapz(Re(h1 conj h2)/S), 4 trapz(|h1|^2/S),
+ * 4 trapz(|h2|^2/S) on the grid f, S (n_f, n_ev), h1, h2 (n_f, n_ev) complex;
+ * accumulate != 0 adds to out (sum over arms and detectors). */
+int gwf_overlap(const double* h1, const double* h2, const double* f, const double* S, int64_t n_ev, int64_t n_f,
+                int32_t accumulate, double* out, void* stream);
+
 /* Vector-Jacobian product of the strain for reverse-mode callers (no
  * reference counterpart; JAX does this inside jacrev, signal.py:1178-1187):
  * g_out[p][e] = sum_f (gh[f][e].re J[p][e][f].re + gh[f][e].im J[p][e][f].im)

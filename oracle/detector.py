@@ -335,6 +335,27 @@ class Detector:
         d3 = -(d1 + d2)
         return [self._fisher_from_derivs(d, strainGrids, fgrids, P) for d in (d1, d2, d3)]
 
+    def wf_overlap(self, wf1, wf2, ev1, ev2, res=1000):
+        """signal.py:1852-1935: (overlap integral, SNR1, SNR2) on the common
+        grid fmin..max(fcut1, fcut2), summed over the arms (third arm of a
+        triangle = -(first + second))."""
+        fcut1, fcut2 = wf1.fcut(ev1), wf2.fcut(ev2)
+        fcutUse = np.where(fcut1 > fcut2, fcut1, fcut2)
+        if self.fmax is not None:
+            fcutUse = np.where(fcutUse > self.fmax, self.fmax, fcut1)
+        f = geomspace_jax(np.full(np.shape(fcutUse), self.fmin), fcutUse, int(res))
+        S = np.interp(f, self.strainFreq, self.noiseCurve, left=1., right=1.)
+        rots = [0.] if self.detector_shape == 'L' else [0., 60.]
+        h1 = [self.gw_strain(wf1, ev1, f, rot=r) for r in rots]
+        h2 = [self.gw_strain(wf2, ev2, f, rot=r) for r in rots]
+        if self.detector_shape == 'T':
+            h1.append(-(h1[0] + h1[1]))
+            h2.append(-(h2[0] + h2[1]))
+        ov = sum(4. * np.trapezoid((a * np.conjugate(b)).real / S, f, axis=0) for a, b in zip(h1, h2))
+        s1 = np.sqrt(sum(4. * np.trapezoid((np.conjugate(a) * a).real / S, f, axis=0) for a in h1))
+        s2 = np.sqrt(sum(4. * np.trapezoid((np.conjugate(b) * b).real / S, f, axis=0) for b in h2))
+        return ov, s1, s2
+
     def fisher(self, wf, ev, res=1000, mode='faithful', **kw):
         return np.sum(self.fisher_arms(wf, ev, res, mode, **kw), axis=0)
 
@@ -387,6 +408,13 @@ def geomspace_jax(start, stop, num):
     lin = a[None, :] * (1 - s) + b[None, :] * s
     lin = np.concatenate([lin, b[None, :]], axis=0)
     return 10.0 ** lin
+
+
+def network_overlap(dets, wf1, wf2, ev1, ev2, res=1000):
+    """network.py:198-232."""
+    parts = [d.wf_overlap(wf1, wf2, ev1, ev2, res) for d in dets]
+    ov = sum(p[0] for p in parts)
+    return ov / np.sqrt(sum(p[1] ** 2 for p in parts) * sum(p[2] ** 2 for p in parts))
 
 
 def network_snr(dets, wf, ev, res=1000):

@@ -316,3 +316,36 @@ def test_gwstrain_single_components_and_m1m2(gpu):
     h1 = sig.GWstrain(f, *args, is_chi1chi2=True)
     h2 = sig.GWstrain(f, m1, m2, *args[2:], is_chi1chi2=True, is_m1m2=True)
     assert np.max(np.abs(h1 - h2)) <= 1e-6 * np.max(np.abs(h1))
+
+
+def test_wf_overlap_and_amplitudes(gpu):
+    """GWSignal / DetNet.WFOverlap (signal.py:1759-1935, network.py:198-232)
+    against the oracle: identical signals give 1, a coalescence-phase shift d
+    gives cos d, mixed models match the oracle's overlap; GWAmplitudes
+    (signal.py:425-466) against the oracle's amplitudes."""
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    from oracle import detector as OD
+    from oracle import waveforms as OW
+    wf, owf = _models('PD')
+    ev = bbh(6, seed=91)
+    net = hip_net(wf, NET_C1, True)
+    odets = oracle_net(NET_C1, True)
+    assert np.allclose(net.WFOverlap(wf, wf, dict(ev), dict(ev), res=500), 1.0, rtol=1e-12, atol=0)
+    ev2 = dict(ev)
+    ev2['Phicoal'] = ev['Phicoal'] + 0.7
+    ev2['dL'] = ev['dL'] * 2.0
+    got = net.WFOverlap(wf, wf, dict(ev), dict(ev2), res=500)
+    assert np.allclose(got, np.cos(0.7), rtol=0, atol=1e-6)
+    tf2, otf2 = TaylorF2_RestrictedPN(), OW.TaylorF2()
+    ev3 = dict(ev)
+    ev3['Mc'] = ev['Mc'] * (1 + 1e-4)
+    got = net.WFOverlap(wf, tf2, dict(ev), dict(ev3), res=500)
+    want = OD.network_overlap(odets, owf, otf2, ev, ev3, 500)
+    print('mixed-model overlap', got, want, flush=True)
+    assert np.allclose(got, want, rtol=0, atol=1e-6)
+    sig = net.signals['CE1Id']
+    f, _ = odets[1].grid(owf, ev, 300)
+    Ap, Ac = sig.GWAmplitudes(dict(ev), f)
+    Apo, Aco = odets[1].gw_amplitudes(owf, ev, f)
+    amax = np.max(np.hypot(Apo, Aco))
+    assert np.max(np.abs(Ap - Apo)) <= 1e-12 * amax and np.max(np.abs(Ac - Aco)) <= 1e-12 * amax

@@ -755,3 +755,22 @@ def test_linspace_jax_grid():
     assert np.array_equal(g[0], a) and np.array_equal(g[-1], b)
     assert np.allclose(np.diff(g, axis=0), (b - a) / 100, rtol=1e-12, atol=0)
     assert np.allclose(g, np.linspace(a, b, 101), rtol=1e-15, atol=0)
+
+
+def test_overlap_closed_forms():
+    """WFOverlap (signal.py:1759-1935): 1 for identical signals, cos(d) for a
+    coalescence-phase shift d (h is proportional to exp(-i Phicoal)),
+    independent of the distances, and SNR1 equal to the SNR integral."""
+    wf = OW.IMRPhenomD()
+    from gwfast_amd import catalog
+    ev = catalog.draw_bbh(3, seed=15)
+    d = glob.detectors['ETS']
+    det = OD.Detector(glob.PSD_ET, 'T', d['lat'], d['long'], d['xax'], useEarthMotion=True)
+    ov, s1, s2 = det.wf_overlap(wf, wf, ev, ev, 400)
+    assert np.allclose(ov / (s1 * s2), 1.0, rtol=1e-12, atol=0)
+    assert np.allclose(s1, det.snr(wf, ev, 400), rtol=1e-12, atol=0)
+    ev2 = dict(ev)
+    ev2['Phicoal'] = ev['Phicoal'] + 0.7
+    ev2['dL'] = ev['dL'] * 3.0
+    ov, s1, s2 = det.wf_overlap(wf, wf, ev, ev2, 400)
+    assert np.allclose(ov / (s1 * s2), np.cos(0.7), rtol=1e-9, atol=0)
