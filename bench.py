@@ -105,6 +105,66 @@ def plain_path(net, rows, n, steps=3, mode='fp64'):
     return n / dt, F
 
 
+def pcie_leg(net, ev, n, steps=5):
+    """The same step with the parameters starting in (pinned) host memory and
+    the Fisher matrices copied back to it: Fisher/s including both PCIe
+    transfers."""
+    import torch
+    from gwfast_amd import _engine as E
+    rows_h = torch.from_numpy(E.param_rows(ev, False)).pin_memory()
+    F_h = torch.empty((11, 11, n), dtype=torch.float64).pin_memory()
+    rows_d = torch.empty_like(rows_h, device='cuda')
+
+    def once():
+        rows_d.copy_(rows_h, non_blocking=True)
+        F = net.fisher_device(rows_d, RES)
+        F_h.copy_(F, non_blocking=True)
+    once()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        once()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    return {'value': n / dt, 'unit': 'Fisher matr/s', 'ms_per_step': 1e3 * dt,
+            'bytes_per_step': int(rows_h.numel() * 8 + F_h.numel() * 8),
+            'path': 'pinned host rows -> HBM, DetNet.fisher_device, (11, 11, N) -> pinned host'}
+
+
+def ragged_leg(n=100_000, steps=3, seed=11):
+    """Config C3: TaylorF2 with tidal terms (13 parameters), the same network
+    with Earth rotation, 10^5 BNS events each on its own grid length drawn
+    uniformly in [500, 10^4] (gwf_fisher_ragged, one persistent launch)."""
+    import torch
+    from gwfast_amd import _engine as E, catalog, gwfastGlobals as glob
+    from gwfast_amd.network import DetNet
+    from gwfast_amd.signal import GWSignal
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    wf = TaylorF2_RestrictedPN(is_tidal=True)
+    psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
+    sigs = {}
+    for name, p in SITES:
+        d = glob.detectors[name]
+        sigs[name] = GWSignal(wf, psd_path=psd[p], detector_shape=d['shape'], det_lat=d['lat'], det_long=d['long'],
+                              det_xax=d['xax'], useEarthMotion=True, fmin=2., verbose=False)
+    net = DetNet(sigs, verbose=False)
+    ev = catalog.draw_bns(n, seed=seed)
+    res = np.random.default_rng(seed).integers(500, 10_001, n).astype(np.int32)
+    rows = E.to_dev(E.param_rows(ev, True))
+    F = net.fisher_device(rows, res)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        F = net.fisher_device(rows, res)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / steps
+    Fh = F.cpu().numpy()
+    return {'value': n / dt, 'unit': 'Fisher matr/s', 'ms_per_step': 1e3 * dt, 'n_events': n, 'n_params': 13,
+            'n_freqs': '500..10000 per event (uniform), %d points in total' % int(res.sum()),
+            'points_per_s': float(res.sum()) / dt, 'finite': bool(np.all(np.isfinite(Fh))),
+            'path': 'TaylorF2 tidal, ETS+CE1Id+CE2NM, Earth rotation, gwf_fisher_ragged'}
+
+
 def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, threshold=12.0, band=(8.0, 16.0)):
     """The learned SNR screen on a 10^6-event catalogue of the same network:
     train the MLP on exact SNRs of an independent catalogue (seed 1001), then
@@ -170,6 +230,7 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=512)
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
+    ap.add_argument('--no-extra', action='store_true', help='skip the PCIe-inclusive and ragged (C3) legs')
     ap.add_argument('--screen-only', action='store_true', help='run only the learned-screen leg')
     ap.add_argument('--screen-events', type=int, default=1_000_000)
     ap.add_argument('--screen-train', type=int, default=1_000_000)
@@ -264,6 +325,9 @@ def main():
             'max_norm_err_vs_plain_fp64': float(np.max(np.abs(Fs - Fp) / (d[:, None, :] * d[None, :, :]))),
             'path': 'as plain_fp64 with gwf_gram_split (FP32 MFMA 16x16x4, hi/lo operands, FP64 reduction)'}
         del Fs
+    if not args.no_extra:
+        out['pcie_inclusive'] = pcie_leg(net, ev, n)
+        out['ragged_c3'] = ragged_leg()
     if not args.no_screen:
         del F, ws
         out['screen_1e6'] = screen_leg(net, n_cat=args.screen_events, n_train=args.screen_train,
