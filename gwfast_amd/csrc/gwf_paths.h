@@ -668,6 +668,106 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
   }
 }
 
+// ------------------------------------- Fisher body, producer / consumer split
+// The per-frequency work of fisher_event in two halves that run in separate
+// waves of one workgroup, exchanging 64-point chunks through LDS: the
+// producer walks the grid and evaluates the PSDs, the waveform with its
+// tangents, tau(f) and the Earth-rotation phase of tau; the consumer does the
+// per-site geometry, the arm derivative rows and the Gram updates.  Neither
+// wave holds the other's registers (the consumer's Gram accumulators are not
+// live in the producer), so two waves fit on a SIMD.
+template <int NT>
+struct PcLayout {
+  static constexpr int F = 0, WT = 1, LNF = 2, S0 = 3, S1 = 4, AV = 5, AD = 6, PD = 6 + NT, TV = 6 + 2 * NT,
+                       TD = 7 + 2 * NT, CD = 9 + 2 * NT, SD = 10 + 2 * NT, NV = 11 + 2 * NT;
+};
+
+// Producer: point base + lane of the grid into buf[NV][64] (nothing past res).
+template <int MODEL, int NT, bool LIN>
+GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec, GridWalk<LIN>& gw, int base, int res,
+                       int lane, double* buf) {
+  typedef PcLayout<NT> L;
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
+  // keep the record loads inside the chunk (scalar loads) rather than hoisted
+  // into vector registers across the whole grid walk
+  __asm__ volatile("" ::: "memory");
+#endif
+  double f, wt, lnf;
+  gw.step(base, res, &f, &wt, &lnf);  // every lane: neighbours come by shuffles
+  if (base + lane >= res) return;
+  PsdPre pre0, pre1;
+  psd_prefetch(net.psd[0], f, lnf, pre0);
+  if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
+  WfPoint<MODEL, NT> wp;
+  wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
+  double cdel, sdel;
+  earth_delta(net.any_earth != 0, wp.tau.v, &cdel, &sdel);
+  const double s0 = psd_finish(net.psd[0], f, lnf, pre0);
+  const double s1 = net.n_psd > 1 ? psd_finish(net.psd[1], f, lnf, pre1) : 1.0;
+  buf[L::F * 64 + lane] = f;
+  buf[L::WT * 64 + lane] = 4.0 * wt;
+  buf[L::LNF * 64 + lane] = lnf;
+  buf[L::S0 * 64 + lane] = s0;
+  buf[L::S1 * 64 + lane] = s1;
+  buf[L::AV * 64 + lane] = wp.A.v;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    buf[(L::AD + k) * 64 + lane] = wp.A.d[k];
+    buf[(L::PD + k) * 64 + lane] = wp.Phi.d[k];
+  }
+  buf[L::TV * 64 + lane] = wp.tau.v;
+  buf[L::TD * 64 + lane] = wp.tau.d[0];
+  buf[(L::TD + 1) * 64 + lane] = wp.tau.d[1];
+  buf[L::CD * 64 + lane] = cdel;
+  buf[L::SD * 64 + lane] = sdel;
+}
+
+// Consumer: the Gram updates of point base + lane (network sum arms).
+template <int MODEL, int NT, int P>
+GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, int lane, const double* buf, double* G) {
+  typedef PcLayout<NT> L;
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
+  __asm__ volatile("" ::: "memory");
+#endif
+  if (base + lane >= res) return;
+  const double* ev = rec + REC_EV;
+  const double f = buf[L::F * 64 + lane], wt = buf[L::WT * 64 + lane];
+  WfPoint<MODEL, NT> wp;
+  wp.A.v = buf[L::AV * 64 + lane];
+  wp.Phi.v = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    wp.A.d[k] = buf[(L::AD + k) * 64 + lane];
+    wp.Phi.d[k] = buf[(L::PD + k) * 64 + lane];
+  }
+  wp.tau.v = buf[L::TV * 64 + lane];
+  wp.tau.d[0] = buf[L::TD * 64 + lane];
+  wp.tau.d[1] = buf[(L::TD + 1) * 64 + lane];
+  const double cdel = buf[L::CD * 64 + lane], sdel = buf[L::SD * 64 + lane];
+  const double tc = ev[EV_TC];
+  int cur_site = -1, cur_psd = -1;
+  double Sf = 1.0;
+  Geo g;
+  for (int k = 0; k < net.n_sum_arms; ++k) {
+    const ArmK& arm = net.sum_arm[k];
+    const int s = arm.site;
+    const SiteK& S = net.site[s];
+    const double* sc = rec + REC_SITE + s * SC_COUNT;
+    if (s != cur_site) {
+      if (S.psd != cur_psd) {
+        Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
+                        : (S.psd == 1 ? buf[L::S1 * 64 + lane] : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
+        cur_psd = S.psd;
+      }
+      site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
+      cur_site = s;
+    }
+    double re[P], im[P], Ap, Ac;
+    arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+    gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+  }
+}
+
 // --------------------------------------------------------------- SNR body
 // Per-arm trapz((Ap^2 + Ac^2)/S, f) of every arm of the network in one grid
 // walk: the amplitude, tau(f) and the PSD segments are evaluated once per
