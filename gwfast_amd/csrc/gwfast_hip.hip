@@ -109,22 +109,28 @@ __global__ __launch_bounds__(256, GWF_FISHER_MIN_BLOCKS) void k_fisher(gwf_wavef
 }
 
 // The network Fisher with the per-frequency work split between a producer
-// and a consumer wave per event (pc_produce / pc_consume), two events per
-// 256-thread block, chunks of 64 points double-buffered in LDS.  Every wave
-// passes the same number of barriers (the loop bounds are block-uniform).
+// and a consumer wave per event (pc_produce / pc_consume), GWF_PC_EPB events
+// per block (waves 0..EPB-1 produce, EPB..2 EPB-1 consume), chunks of 64
+// points double-buffered in LDS, two waves per SIMD.  Every wave passes the
+// same number of barriers (the loop bounds are block-uniform).
+#ifndef GWF_PC_EPB
+#define GWF_PC_EPB 4  // measured: 4 events per 512-thread block 2.65 ms, 2 per 256-thread block 2.80 ms
+#endif
+constexpr int PC_EPB = GWF_PC_EPB;
 template <int MODEL, int NT, int P, bool LIN>
-__global__ __launch_bounds__(256, 2) void k_fisher_pc(gwf_waveform w, NetK net, const double* __restrict__ rec,
-                                                      int64_t n_ev, double* __restrict__ F) {
+__global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pc(gwf_waveform w, NetK net,
+                                                                      const double* __restrict__ rec, int64_t n_ev,
+                                                                      double* __restrict__ F) {
   typedef PcLayout<NT> L;
-  __shared__ double buf[2][2][L::NV * 64];
+  __shared__ double buf[PC_EPB][2][L::NV * 64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slot = wave & 1;
-  const int64_t e = (int64_t)blockIdx.x * 2 + slot;
+  const int slot = wave % PC_EPB;
+  const int64_t e = (int64_t)blockIdx.x * PC_EPB + slot;
   const bool valid = e < n_ev;
   const double* r = rec + (valid ? e : 0) * (int64_t)REC_DOUBLES;
   const int res = net.res;
   const int nch = (res + 63) / 64;
-  if (wave < 2) {  // producers of events 0, 1 of the block; waves 2, 3 consume
+  if (wave < PC_EPB) {
     GridWalk<LIN> gw;
     double a = 0.0, b = 0.0;
     if (valid) grid_ends(LIN, net.fmin, fcut_eff(r[REC_EV + EV_FCUT], net.fmax), &a, &b);
@@ -948,7 +954,7 @@ int nt_of(const gwf_waveform* wf) {
 }
 
 // Model dispatch of the Fisher kernels for one grid spacing.
-// The network sum runs on the producer / consumer kernel (measured 2.74 ms
+// The network sum runs on the producer / consumer kernel (measured 2.65 ms
 // against 2.91 ms for k_fisher per 10^4 events x 2000 points, identical
 // results); GWF_FISHER_PC=0 selects the single-wave kernel.
 #ifndef GWF_FISHER_PC
@@ -959,16 +965,16 @@ template <bool LIN>
 void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st, const NetK& net, const double* rec,
                    int64_t n_ev, int arm, double* out) {
   if (GWF_FISHER_PC && arm < 0) {  // producer / consumer waves, two events per block
-    const dim3 g2((unsigned)((n_ev + 1) / 2));
+    const dim3 g2((unsigned)((n_ev + PC_EPB - 1) / PC_EPB)), b2(128 * PC_EPB);
     if (wf->model == GWF_MODEL_IMRPHENOMD)
-      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_IMRPHENOMD, 4, 11, LIN>), g2, block, 0, st, *wf, net, rec, n_ev, out);
+      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_IMRPHENOMD, 4, 11, LIN>), g2, b2, 0, st, *wf, net, rec, n_ev, out);
     else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), g2, block, 0, st, *wf, net, rec,
+      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), g2, b2, 0, st, *wf, net, rec,
                          n_ev, out);
     else if (nt_of(wf) == 6)
-      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 6, 13, LIN>), g2, block, 0, st, *wf, net, rec, n_ev, out);
+      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 6, 13, LIN>), g2, b2, 0, st, *wf, net, rec, n_ev, out);
     else
-      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 4, 11, LIN>), g2, block, 0, st, *wf, net, rec, n_ev, out);
+      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 4, 11, LIN>), g2, b2, 0, st, *wf, net, rec, n_ev, out);
     return;
   }
   if (wf->model == GWF_MODEL_IMRPHENOMD)
