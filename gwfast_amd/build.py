@@ -24,6 +24,10 @@ def command(out=OUT):
             '-I', os.path.join(ROOT, 'include'), '-I', os.path.join(HERE, 'csrc'),
             # no packed-FP32 instructions anywhere in the device code
             '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+            # no machine-level hoisting of loop invariants: in the Fisher kernels it
+            # hoisted 64-bit constant materialisations out of the frequency loop and
+            # spilled them to scratch (46 VGPRs, 107 MB of spill writes per launch)
+            '-mllvm', '-disable-machine-licm',
             *SRCS, '-o', out]
 
 

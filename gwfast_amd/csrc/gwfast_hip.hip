@@ -123,7 +123,9 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pc(gwf_wave
                                                                       double* __restrict__ F) {
   typedef PcLayout<NT> L;
   __shared__ double buf[PC_EPB][2][L::NV * 64];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave index made provably uniform, so the record pointer stays scalar
+  // (scalar loads of the event record instead of per-lane vector loads)
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int slot = wave % PC_EPB;
   const int64_t e = (int64_t)blockIdx.x * PC_EPB + slot;
   const bool valid = e < n_ev;
