@@ -28,6 +28,8 @@ def to_dev(a, dtype=None):
     and boolean arrays keep their own dtype."""
     t = torch()
     a = np.ascontiguousarray(a)
+    if not a.flags.writeable:      # broadcast views: torch wants memory it may own
+        a = a.copy()
     if dtype is None and a.dtype.kind not in 'iub':
         dtype = t.float64
     return t.as_tensor(a, dtype=dtype, device=device())
@@ -54,9 +56,11 @@ def workspace(n_ev):
     return _WS.get(n_ev)
 
 
-def param_rows(evParams, is_tidal):
+def param_rows(evParams, is_tidal, is_eccentric=False):
     """evParams dict -> (P_rows, N) float64 with LambdaTilde/deltaLambda for
-    tidal models (signal.py:858-866 computes them from Lambda1/Lambda2)."""
+    tidal models (signal.py:858-866 computes them from Lambda1/Lambda2) and
+    the eccentricity for eccentric ones (signal.py:876-882), in Fisher row
+    order."""
     from .gwfastUtils import Lamt_delLam_from_Lam12
     n = len(np.atleast_1d(evParams['Mc']))
     rows = [np.broadcast_to(np.asarray(evParams[k], dtype=np.float64), (n,)) for k in PARAM_ROWS[:11]]
@@ -72,6 +76,11 @@ def param_rows(evParams, is_tidal):
                 raise ValueError('Two among Lambda1, Lambda2 and LambdaTilde and deltaLambda have to be provided.')
         rows += [np.broadcast_to(np.asarray(Lt, dtype=np.float64), (n,)),
                  np.broadcast_to(np.asarray(dL, dtype=np.float64), (n,))]
+    if is_eccentric:
+        try:
+            rows.append(np.broadcast_to(np.asarray(evParams['ecc'], dtype=np.float64), (n,)))
+        except KeyError:
+            raise ValueError('Eccentricity has to be provided.')
     return np.ascontiguousarray(np.array(rows, dtype=np.float64))
 
 

@@ -29,7 +29,8 @@ class GwfWaveform(ctypes.Structure):
     _fields_ = [('model', c_int32), ('is_tidal', c_int32), ('use_3p5PN_SpinHO', c_int32),
                 ('phiref_vlso', c_int32), ('kerr_isco', c_int32), ('use_quadmon', c_int32),
                 ('apply_fcut', c_int32), ('n_qnm', c_int32), ('fcut_par', c_double), ('fref', c_double),
-                ('qnm_a', c_void_p), ('qnm_fring', c_void_p), ('qnm_fdamp', c_void_p)]
+                ('qnm_a', c_void_p), ('qnm_fring', c_void_p), ('qnm_fdamp', c_void_p),
+                ('is_eccentric', c_int32), ('reserved', c_int32), ('fref_ecc', c_double)]
 
 
 class GwfDetector(ctypes.Structure):

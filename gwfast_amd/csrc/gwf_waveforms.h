@@ -50,6 +50,10 @@ enum TfSlot {
   TF_MG = 0, TF_ETA,
   TF_C0, TF_C2, TF_C3, TF_C4, TF_C5, TF_C5L, TF_C6, TF_C6L, TF_C7, TF_C10, TF_C12,
   TF_PHIR, TF_AMP, TF_FCUT,
+  // eccentric TaylorF2 (waveforms.py:814-845): coefficients of v^(-34/3) x
+  // (1, v^2, v^3, v^4, v^5, v^6, v^6 ln v), each times 3/(128 eta) (-2355/1462)
+  // e0^2 v0^(19/3)
+  TF_E0, TF_E2, TF_E3, TF_E4, TF_E5, TF_E6, TF_E6L,
   TF_COUNT
 };
 constexpr int REC_SLOTS = PD_COUNT;  // >= TF_COUNT
@@ -815,9 +819,14 @@ GWF_HD double tf2_fcut(const gwf_waveform& w, double Mc, double eta, double chi1
 // deltaLambda; tidal).  The phase is stored as coefficients of
 // v^-5, v^-3, v^-2, v^-1, 1, ln v, v, v ln v, v^2, v^5, v^7 (all times
 // 3/(128 eta)), so Phi(f) = sum c_k b_k(v) + phiR - pi/4.
+// Eccentric TaylorF2 is the TaylorF2 instantiation with N = 5 tangents
+// (Mc, eta, chi1z, chi2z, ecc); the tidal one has N = 6.
+constexpr bool tf2_eccentric(int N) { return N == 5; }
+
 template <int N>
 GWF_HD void tf2_setup(const gwf_waveform& w, const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2,
-                      const D<N>& Lamt, const D<N>& delLam, double* rec, int stride) {
+                      const D<N>& Lamt, const D<N>& delLam, double* rec, int stride, const D<N>& ecc = D<N>(),
+                      double f0 = 1.0) {
   typedef D<N> T;
   const T MG = Mc * GMSUN_C3 / dpow(eta, 3. / 5.);
   const T Seta = seta_of(eta);
@@ -871,6 +880,49 @@ GWF_HD void tf2_setup(const gwf_waveform& w, const D<N>& Mc, const D<N>& eta, co
   // sqrt(5/24) pi^(-2/3) clightGpc (G Mc)^(5/6)  (waveforms.py:875, without 1/dL)
   put(TF_AMP, sqrt(5. / 24.) * pow(PI, -2. / 3.) * CLIGHT_GPC * dpow(GMSUN_C3 * Mc, 5. / 6.));
   put(TF_FCUT, mkc<N>(tf2_fcut(w, Mc.v, eta.v, chi1.v, chi2.v)));
+  if (tf2_eccentric(N)) {
+    // low-eccentricity 3PN phase of arXiv:1605.00304 as written in waveforms.py:814-845;
+    // v0 at fRef_ecc, else at the grid minimum (amin of v over the grid)
+    const double fe = (w.fref_ecc > 0.0) ? w.fref_ecc : f0;
+    const T v0 = dcbrt(PI * MG * fe);
+    const T v02 = v0 * v0, v03 = v02 * v0, v04 = v02 * v02, v05 = v04 * v0, v06 = v03 * v03;
+    const T eta2 = eta * eta, eta3 = eta2 * eta;
+    const double eg = 0.57721566490153286061, l2 = log(2.), l3 = log(3.);
+    const T twoV = 29.9076223 / 8.1976608 + 18.766963 / 2.927736 * eta;
+    const T twoV0 = 2.833 / 1.008 - 19.7 / 3.6 * eta;
+    const double threeV = -28.19123 / 2.82600 * PI, threeV0 = 37.7 / 7.2 * PI;
+    const T fourV4 = 16.237683263 / 3.330429696 + 241.33060753 / 9.71375328 * eta + 156.2608261 / 6.9383952 * eta2;
+    const T fourV2V02 = 84.7282939759 / 8.2632420864 - 7.18901219 / 3.68894736 * eta - 36.97091711 / 1.05398496 * eta2;
+    const T fourV04 = -1.193251 / 3.048192 - 66.317 / 9.072 * eta + 18.155 / 1.296 * eta2;
+    const T fiveV5 = -28.31492681 / 1.18395270 * PI - 115.52066831 / 2.70617760 * PI * eta;
+    const T fiveV3V02 = -79.86575459 / 2.84860800 * PI + 55.5367231 / 1.0173600 * PI * eta;
+    const T fiveV2V03 = 112.751736071 / 5.902315776 * PI + 70.75145051 / 2.10796992 * PI * eta;
+    const T fiveV05 = 76.4881 / 9.0720 * PI - 94.9457 / 2.2680 * PI * eta;
+    const T sixV6 = -436.03153867072577087 / 1.32658535116800000 + 53.6803271 / 1.9782000 * eg +
+                    157.22503703 / 3.25555200 * PI * PI +
+                    (2991.72861614477 / 6.89135247360 - 15.075413 / 1.446912 * PI * PI) * eta +
+                    345.5209264991 / 4.1019955200 * eta2 + 506.12671711 / 8.78999040 * eta3 +
+                    384.3505163 / 5.9346000 * l2 - 112.1397129 / 1.7584000 * l3;
+    const T sixV4V02 = 46.001356684079 / 3.357073133568 + 253.471410141755 / 5.874877983744 * eta -
+                       169.3852244423 / 2.3313007872 * eta2 - 307.833827417 / 2.497822272 * eta3;
+    const double sixV3V03 = -106.2809371 / 2.0347200 * PI * PI;
+    const T sixV2V04 = -3.56873002170973 / 2.49880440692736 - 260.399751935005 / 8.924301453312 * eta +
+                       15.0484695827 / 3.5413894656 * eta2 + 340.714213265 / 3.794345856 * eta3;
+    const T sixV06 = 265.31900578691 / 1.68991764480 - 33.17 / 1.26 * eg + 12.2833 / 1.0368 * PI * PI +
+                     (91.55185261 / 5.48674560 - 3.977 / 1.152 * PI * PI) * eta - 5.732473 / 1.306368 * eta2 -
+                     30.90307 / 1.39968 * eta3 + 87.419 / 1.890 * l2 - 260.01 / 5.60 * l3;
+    const double k6 = 53.6803271 / 3.9564000, k0 = 33.17 / 2.52;
+    // overall -2.355/1.462 e^2 (v0/v)^(19/3), over v^5 with the rest of the phase
+    const T K = TF2OA * (-2.355 / 1.462) * ecc * ecc * dpow(v0, 19. / 3.);
+    put(TF_E0, K * (1. + twoV0 * v02 + threeV0 * v03 + fourV04 * v04 + fiveV05 * v05 +
+                    (sixV06 - k0 * dlog(16. * v02)) * v06));
+    put(TF_E2, K * (twoV + fourV2V02 * v02 + fiveV2V03 * v03 + sixV2V04 * v04));
+    put(TF_E3, K * (threeV + fiveV3V02 * v02 + sixV3V03 * v03));
+    put(TF_E4, K * (fourV4 + sixV4V02 * v02));
+    put(TF_E5, K * fiveV5);
+    put(TF_E6, K * (sixV6 + k6 * log(16.)));
+    put(TF_E6L, K * (2. * k6));  // k6 ln(16 v^2) = k6 ln 16 + 2 k6 ln v
+  }
 }
 
 // Per-frequency TaylorF2 phase.  v is (pi MG f)^(1/3) as a dual.
@@ -884,10 +936,20 @@ GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v, const D<N>&
   const T v2 = v * v;
   const T v5 = v2 * v2 * v;
   const T v7 = v5 * v2;
-  return ldd<N>(rec + TF_C0 * stride) * iv5 + ldd<N>(rec + TF_C2 * stride) * iv3 + ldd<N>(rec + TF_C3 * stride) * iv2 +
+  T ph = ldd<N>(rec + TF_C0 * stride) * iv5 + ldd<N>(rec + TF_C2 * stride) * iv3 + ldd<N>(rec + TF_C3 * stride) * iv2 +
          ldd<N>(rec + TF_C4 * stride) * iv + ldd<N>(rec + TF_C5 * stride) + ldd<N>(rec + TF_C5L * stride) * lv +
          ldd<N>(rec + TF_C6 * stride) * v + ldd<N>(rec + TF_C6L * stride) * (v * lv) + ldd<N>(rec + TF_C7 * stride) * v2 +
          ldd<N>(rec + TF_C10 * stride) * v5 + ldd<N>(rec + TF_C12 * stride) * v7 + rec[TF_PHIR * stride];
+  if (tf2_eccentric(N)) {
+    // v^(-34/3) (E0 + E2 v^2 + E3 v^3 + E4 v^4 + E5 v^5 + (E6 + E6L ln v) v^6)
+    const double p = pow(v.v, -34. / 3.);
+    const T w = chain(v, p, (-34. / 3.) * p / v.v);
+    const T v3 = v2 * v, v4 = v2 * v2, v6 = v3 * v3;
+    ph = ph + w * (ldd<N>(rec + TF_E0 * stride) + ldd<N>(rec + TF_E2 * stride) * v2 + ldd<N>(rec + TF_E3 * stride) * v3 +
+                   ldd<N>(rec + TF_E4 * stride) * v4 + ldd<N>(rec + TF_E5 * stride) * v5 +
+                   (ldd<N>(rec + TF_E6 * stride) + ldd<N>(rec + TF_E6L * stride) * lv) * v6);
+  }
+  return ph;
 }
 
 // ------------------------------------------------------- tau_star (both)

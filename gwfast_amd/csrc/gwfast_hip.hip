@@ -946,12 +946,15 @@ int check_wf(const gwf_waveform* wf) {
   if (is_pd(wf->model) && (!wf->qnm_a || !wf->qnm_fring || !wf->qnm_fdamp || wf->n_qnm < 2))
     return fail(-EINVAL, "IMRPhenomD needs the QNM tables");
   if (wf->model == GWF_MODEL_IMRPHENOMD && wf->is_tidal) return fail(-EINVAL, "IMRPhenomD is not tidal");
+  if (wf->is_eccentric && (wf->model != GWF_MODEL_TAYLORF2 || wf->is_tidal))
+    return fail(-EINVAL, "eccentricity is supported for non-tidal TaylorF2 only");
   if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2 && !wf->is_tidal)
     return fail(-EINVAL, "IMRPhenomD_NRTidalv2 is tidal (13 parameters)");
   return 0;
 }
 
 int nt_of(const gwf_waveform* wf) {
+  if (wf->model == GWF_MODEL_TAYLORF2 && wf->is_eccentric) return 5;  // eccentric TaylorF2, 12 parameters
   return ((wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) || wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2) ? 6 : 4;
 }
 
@@ -973,6 +976,8 @@ void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st
     else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
       hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), g2, b2, 0, st, *wf, net, rec,
                          n_ev, out);
+    else if (nt_of(wf) == 5)
+      hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 5, 12, LIN>), g2, b2, 0, st, *wf, net, rec, n_ev, out);
     else if (nt_of(wf) == 6)
       hipLaunchKernelGGL((k_fisher_pc<GWF_MODEL_TAYLORF2, 6, 13, LIN>), g2, b2, 0, st, *wf, net, rec, n_ev, out);
     else
@@ -984,6 +989,8 @@ void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
                        arm, out);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
   else
@@ -999,6 +1006,9 @@ void launch_fisher_masked(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net,
                        rec, n_ev, mask, out);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       mask, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
                        mask, out);
@@ -1017,6 +1027,9 @@ void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN>), grid, block, 0, st, *wf, net,
                        rec, n_ev, res_ev, arm, counter, out);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       res_ev, arm, counter, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
                        res_ev, arm, counter, out);
@@ -1034,6 +1047,8 @@ int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid,
                        nf, rec);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 5>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else
@@ -1157,6 +1172,9 @@ int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, 
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f,
                        ampl_out, phi_out, tau_out);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 5>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
+                       tau_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
                        tau_out);
@@ -1331,6 +1349,9 @@ static int strain_impl(const gwf_waveform* wf, const gwf_detector* det, int32_t 
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev,
                        (int)arm, f, n_f, h_out, J_out, comp_out);
+  else if (nt_of(wf) == 5)
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 5, 12>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
+                       h_out, J_out, comp_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
                        h_out, J_out, comp_out);
@@ -1367,7 +1388,7 @@ int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, i
 
 int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
              int32_t accumulate, double* fisher_out, void* stream) {
-  if (P != 11 && P != 13) return fail(-EINVAL, "gwf_gram supports P = 11 or 13");
+  if (P < 11 || P > 13) return fail(-EINVAL, "gwf_gram supports P = 11, 12 or 13");
   if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
   if (n_ev == 0) return 0;
   if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
@@ -1376,6 +1397,8 @@ int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64
   hipStream_t st = (hipStream_t)stream;
   if (P == 11)
     hipLaunchKernelGGL((k_gram<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else if (P == 12)
+    hipLaunchKernelGGL((k_gram<12>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   else
     hipLaunchKernelGGL((k_gram<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   return hip_check(hipGetLastError(), "gwf_gram launch");
@@ -1383,7 +1406,7 @@ int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64
 
 int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
                    int32_t accumulate, double* fisher_out, void* stream) {
-  if (P != 11 && P != 13) return fail(-EINVAL, "gwf_gram_split supports P = 11 or 13");
+  if (P < 11 || P > 13) return fail(-EINVAL, "gwf_gram_split supports P = 11, 12 or 13");
   if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
   if (n_ev == 0) return 0;
   if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
@@ -1392,6 +1415,8 @@ int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P,
   hipStream_t st = (hipStream_t)stream;
   if (P == 11)
     hipLaunchKernelGGL((k_gram_split<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else if (P == 12)
+    hipLaunchKernelGGL((k_gram_split<12>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   else
     hipLaunchKernelGGL((k_gram_split<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   return hip_check(hipGetLastError(), "gwf_gram_split launch");
@@ -1491,7 +1516,7 @@ int gwf_fisher_block_transform(const double* F, double* out, int32_t P, int64_t 
 
 int gwf_fisher_reparam(const double* params, int64_t ld, double* F, int32_t P, int64_t n_ev, int32_t use_m1m2,
                        int32_t use_chiSA, void* stream) {
-  if (P != 11 && P != 13) return fail(-EINVAL, "P must be 11 or 13");
+  if (P < 11 || P > 13) return fail(-EINVAL, "P must be 11, 12 or 13");
   if (n_ev < 0 || ld < n_ev) return fail(-EINVAL, "invalid sizes");
   if (n_ev == 0 || (!use_m1m2 && !use_chiSA)) return 0;
   if (!params || !F) return fail(-EINVAL, "NULL array");

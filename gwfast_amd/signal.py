@@ -116,7 +116,7 @@ class GWSignal(object):
         return evParams
 
     def _rows(self, evParams):
-        return E.to_dev(E.param_rows(evParams, self.wf_model.is_tidal))
+        return E.to_dev(E.param_rows(evParams, self.wf_model.is_tidal, self.wf_model.is_eccentric))
 
     # --------------------------------------------------------------- public
     def SNRInteg(self, evParams, res=1000, return_all=False):
@@ -214,6 +214,8 @@ class GWSignal(object):
                 ev['Lambda1'], ev['Lambda2'] = LambdaTilde, deltaLambda
             else:
                 ev['LambdaTilde'], ev['deltaLambda'] = LambdaTilde, deltaLambda
+        if self.wf_model.is_eccentric:
+            ev['ecc'] = ecc
         ev = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in ev.items()}
         arm = _arm_of(self, rot)
         rows = self._rows(ev)
@@ -253,8 +255,8 @@ class GWSignal(object):
         if self.fmax is not None:
             # as the reference writes it (signal.py:1858-1859): fcut1 where below fmax
             fcutUse = np.where(fcutUse > self.fmax, self.fmax, fcut1)
-        r1 = E.to_dev(E.param_rows(ev1, WF1.is_tidal))
-        r2 = E.to_dev(E.param_rows(ev2, WF2.is_tidal))
+        r1 = E.to_dev(E.param_rows(ev1, WF1.is_tidal, WF1.is_eccentric))
+        r2 = E.to_dev(E.param_rows(ev2, WF2.is_tidal, WF2.is_eccentric))
         n = r1.shape[1]
         f = t.empty((int(res), n), dtype=t.float64, device=r1.device)
         S = t.empty_like(f)

@@ -19,13 +19,15 @@ from . import gwfastGlobals as glob
 
 class WaveFormModel(ABC):
     """gwfast/waveforms.py:49-199 (aligned-spin subset: no precession, no
-    eccentricity, no higher modes)."""
+    higher modes; eccentricity for TaylorF2 without tidal terms)."""
 
     def __init__(self, objType, fcutPar, is_newtonian=False, is_tidal=False, is_HigherModes=False,
                  is_chi1chi2=True, is_Precessing=False, is_LAL=False, is_prec_ang=False, is_eccentric=False,
                  is_holomorphic=False, apply_fcut=True):
-        if is_newtonian or is_HigherModes or is_Precessing or is_LAL or is_eccentric:
-            raise NotImplementedError('only aligned-spin, quasi-circular, dominant-mode models are on this path')
+        if is_newtonian or is_HigherModes or is_Precessing or is_LAL:
+            raise NotImplementedError('only aligned-spin, dominant-mode models are on this path')
+        if is_eccentric and is_tidal:
+            raise NotImplementedError('eccentricity is supported without tidal terms (12 parameters)')
         self.objType = objType
         self.fcutPar = fcutPar
         self.is_newtonian = is_newtonian
@@ -46,6 +48,9 @@ class WaveFormModel(ABC):
             self.ParNums['LambdaTilde'] = 11
             self.ParNums['deltaLambda'] = 12
             self.nParams = 13
+        if is_eccentric:
+            self.ParNums['ecc'] = 11
+            self.nParams = 12
         if is_chi1chi2:
             self.ParNums['chi1z'] = self.ParNums.pop('chiS')
             self.ParNums['chi2z'] = self.ParNums.pop('chiA')
@@ -76,8 +81,8 @@ class WaveFormModel(ABC):
             f2 = np.repeat(f[:, None], n, axis=1) if (scalar_ev or f.shape[0] != n) else f[None, :]
         else:
             f2 = f
-        rows = E.param_rows({k: np.atleast_1d(v) for k, v in ev.items()}, self.is_tidal) if not self.is_tidal \
-            else self._tidal_rows(ev)
+        rows = E.param_rows({k: np.atleast_1d(v) for k, v in ev.items()}, self.is_tidal, self.is_eccentric) \
+            if not self.is_tidal else self._tidal_rows(ev)
         t = E.torch()
         fd = E.to_dev(f2)
         pd = E.to_dev(rows)
@@ -120,7 +125,9 @@ class WaveFormModel(ABC):
             ev.setdefault(k, np.zeros(n))
         if self.is_tidal and 'Lambda1' not in ev and 'LambdaTilde' not in ev:
             ev['Lambda1'] = ev['Lambda2'] = np.zeros(n)
-        rows = E.param_rows(ev, self.is_tidal)
+        if self.is_eccentric:
+            ev.setdefault('ecc', np.zeros(n))  # the cut does not depend on it
+        rows = E.param_rows(ev, self.is_tidal, self.is_eccentric)
         t = E.torch()
         pd = E.to_dev(rows)
         out = t.empty(n, dtype=t.float64, device=E.device())
@@ -131,7 +138,8 @@ class WaveFormModel(ABC):
 
 class TaylorF2_RestrictedPN(WaveFormModel):
     """TaylorF2 3.5PN restricted waveform (waveforms.py:697-953), optionally
-    with 5 and 6PN tidal terms; eccentricity is not supported."""
+    with 5 and 6PN tidal terms or with the low-eccentricity 3PN phase
+    (is_eccentric, fRef_ecc; waveforms.py:814-845), not both."""
 
     def __init__(self, fHigh=None, is_tidal=False, use_3p5PN_SpinHO=False, phiref_vlso=False, is_eccentric=False,
                  fRef_ecc=None, which_ISCO='Schw', use_QuadMonTid=False, **kwargs):
@@ -157,6 +165,8 @@ class TaylorF2_RestrictedPN(WaveFormModel):
         w.use_quadmon = int(self.use_QuadMonTid)
         w.apply_fcut = int(self.apply_fcut)
         w.fcut_par = float(self.fcutPar)
+        w.is_eccentric = int(bool(self.is_eccentric))
+        w.fref_ecc = float(self.fRef_ecc) if self.fRef_ecc is not None else 0.0
         return w
 
 

@@ -61,6 +61,9 @@ typedef struct gwf_waveform {
   const double* qnm_a;      /* device, IMRPhenomD QNMData_a */
   const double* qnm_fring;  /* device, QNMData_fring */
   const double* qnm_fdamp;  /* device, QNMData_fdamp */
+  int32_t is_eccentric;     /* TaylorF2 is_eccentric (non-tidal): 12 parameters, row 11 = ecc */
+  int32_t reserved;
+  double fref_ecc;          /* TaylorF2 fRef_ecc [Hz]; <= 0: the grid minimum */
 } gwf_waveform;
 
 /* One detector site: GWSignal(...) (signal.py:68-82). */

@@ -161,9 +161,12 @@ class Detector:
         522-524) and chiS, chiA (use_chi1chi2=False, signal.py:527-533).
         Returns complex array (n_AD, N_ev, N_f)."""
         tidal = wf.is_tidal
-        K = 6 if tidal else 4
+        ecc = getattr(wf, 'is_eccentric', False)
+        K = 6 if tidal else (5 if ecc else 4)
         evd = dict(ev)
         _seed_masses_spins(evd, ev, K, use_m1m2, use_chi1chi2)
+        if ecc:  # AD argument list [Mc, eta, chiS, chiA, ecc] (signal.py:1229-1233)
+            evd['ecc'] = D.Dual.seed(ev['ecc'], 4, K)
         if tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             Ltd = D.Dual.seed(Lt, 4, K)
@@ -296,6 +299,8 @@ class Detector:
         for k, key in enumerate(keys):
             evd[key] = D.Dual.seed(ev[key], k, P)
         _seed_masses_spins(evd, ev, P, use_m1m2, use_chi1chi2, slots=(0, 1, 9, 10))
+        if getattr(wf, 'is_eccentric', False):
+            evd['ecc'] = D.Dual.seed(ev['ecc'], 11, P)
         if wf.is_tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             L1, L2 = lam12_from_lamt_dellam(D.Dual.seed(Lt, 11, P), D.Dual.seed(dLm, 12, P), evd['eta'])

@@ -153,21 +153,66 @@ class TaylorF2:
     name = 'TaylorF2_RestrictedPN'
 
     def __init__(self, fHigh=None, is_tidal=False, use_3p5PN_SpinHO=False, phiref_vlso=False,
-                 which_ISCO='Schw', use_QuadMonTid=False):
+                 which_ISCO='Schw', use_QuadMonTid=False, is_eccentric=False, fRef_ecc=None):
         if fHigh is None:
             fHigh = 1. / (6. * np.pi * np.sqrt(6.) * glob.GMsun_over_c3)
         self.fcutPar = fHigh
         self.is_tidal = is_tidal
+        self.is_eccentric = is_eccentric
+        self.fRef_ecc = fRef_ecc
         self.use_3p5PN_SpinHO = use_3p5PN_SpinHO
         self.phiref_vlso = phiref_vlso
         self.which_ISCO = which_ISCO
         self.use_QuadMonTid = use_QuadMonTid
         self.is_holomorphic = True
         self.objType = 'BNS' if is_tidal else 'BBH'
-        self.nParams = 13 if is_tidal else 11
+        self.nParams = 13 if is_tidal else (12 if is_eccentric else 11)
+
+    def phi_ecc(self, v, Mtot_sec, eta, ecc):
+        """waveforms.py:814-845: the low-eccentricity phase to 3PN
+        (arXiv:1605.00304), added to the bracket of the TaylorF2 phase; v0 at
+        fRef_ecc or the grid minimum of v."""
+        v0 = D.amin0(v) if self.fRef_ecc is None else (np.pi * Mtot_sec * self.fRef_ecc) ** (1. / 3.)
+        eta2 = eta * eta
+        eg = np.euler_gamma
+        k = {}
+        k['twoV'] = 29.9076223 / 8.1976608 + 18.766963 / 2.927736 * eta
+        k['twoV0'] = 2.833 / 1.008 - 19.7 / 3.6 * eta
+        k['threeV'] = -28.19123 / 2.82600 * np.pi
+        k['threeV0'] = 37.7 / 7.2 * np.pi
+        k['fourV4'] = 16.237683263 / 3.330429696 + 241.33060753 / 9.71375328 * eta + 156.2608261 / 6.9383952 * eta2
+        k['fourV2V02'] = 84.7282939759 / 8.2632420864 - 7.18901219 / 3.68894736 * eta - 36.97091711 / 1.05398496 * eta2
+        k['fourV04'] = -1.193251 / 3.048192 - 66.317 / 9.072 * eta + 18.155 / 1.296 * eta2
+        k['fiveV5'] = -28.31492681 / 1.18395270 * np.pi - 115.52066831 / 2.70617760 * np.pi * eta
+        k['fiveV3V02'] = -79.86575459 / 2.84860800 * np.pi + 55.5367231 / 1.0173600 * np.pi * eta
+        k['fiveV2V03'] = 112.751736071 / 5.902315776 * np.pi + 70.75145051 / 2.10796992 * np.pi * eta
+        k['fiveV05'] = 76.4881 / 9.0720 * np.pi - 94.9457 / 2.2680 * np.pi * eta
+        k['sixV6'] = (-436.03153867072577087 / 1.32658535116800000 + 53.6803271 / 1.9782000 * eg
+                      + 157.22503703 / 3.25555200 * np.pi * np.pi
+                      + (2991.72861614477 / 6.89135247360 - 15.075413 / 1.446912 * np.pi * np.pi) * eta
+                      + 345.5209264991 / 4.1019955200 * eta2 + 506.12671711 / 8.78999040 * eta2 * eta
+                      + 384.3505163 / 5.9346000 * np.log(2.) - 112.1397129 / 1.7584000 * np.log(3.))
+        k['sixV4V02'] = (46.001356684079 / 3.357073133568 + 253.471410141755 / 5.874877983744 * eta
+                         - 169.3852244423 / 2.3313007872 * eta2 - 307.833827417 / 2.497822272 * eta2 * eta)
+        k['sixV3V03'] = -106.2809371 / 2.0347200 * np.pi * np.pi
+        k['sixV2V04'] = (-3.56873002170973 / 2.49880440692736 - 260.399751935005 / 8.924301453312 * eta
+                         + 15.0484695827 / 3.5413894656 * eta2 + 340.714213265 / 3.794345856 * eta2 * eta)
+        k['sixV06'] = (265.31900578691 / 1.68991764480 - 33.17 / 1.26 * eg + 12.2833 / 1.0368 * np.pi * np.pi
+                       + (91.55185261 / 5.48674560 - 3.977 / 1.152 * np.pi * np.pi) * eta - 5.732473 / 1.306368 * eta2
+                       - 30.90307 / 1.39968 * eta2 * eta + 87.419 / 1.890 * np.log(2.) - 260.01 / 5.60 * np.log(3.))
+        amp = -2.355 / 1.462 * ecc * ecc * ((v0 / v) ** (19. / 3.))
+        return amp * (1. + (k['twoV'] * v * v + k['twoV0'] * v0 * v0)
+                      + (k['threeV'] * v ** 3 + k['threeV0'] * v0 ** 3)
+                      + (k['fourV4'] * v ** 4 + k['fourV2V02'] * v * v * v0 * v0 + k['fourV04'] * v0 ** 4)
+                      + (k['fiveV5'] * v ** 5 + k['fiveV3V02'] * v ** 3 * v0 * v0 + k['fiveV2V03'] * v * v * v0 ** 3
+                         + k['fiveV05'] * v0 ** 5)
+                      + ((k['sixV6'] + 53.6803271 / 3.9564000 * D.log(16. * v * v)) * v ** 6
+                         + k['sixV4V02'] * v ** 4 * v0 * v0 + k['sixV3V03'] * v ** 3 * v0 ** 3
+                         + k['sixV2V04'] * v * v * v0 ** 4
+                         + (k['sixV06'] - 33.17 / 2.52 * D.log(16. * v0 * v0)) * v0 ** 6))
 
     def Phi(self, f, ev):
-        """waveforms.py:743-862 (eccentricity off)."""
+        """waveforms.py:743-862."""
         Mtot_sec = ev['Mc'] * glob.GMsun_over_c3 / (ev['eta'] ** (3. / 5.))
         v = (np.pi * Mtot_sec * f) ** (1. / 3.)
         eta = ev['eta']
@@ -194,10 +239,11 @@ class TaylorF2:
             phi_Tidal = (-0.5 * 39. * Lam_t) * (v ** 10.) + (-3115. / 64. * Lam_t + 6595. / 364. * Seta * delLam) * (v ** 12.)
         else:
             phi_Tidal = 0.
+        phi_Ecc = self.phi_ecc(v, Mtot_sec, eta, ev['ecc']) if self.is_eccentric else 0.
         phase = TF2OverallAmpl * (1. + 0. * v + c['two'] * v * v + c['three'] * v ** 3 + c['four'] * v ** 4
                                   + (five + five_log * D.log(v)) * v ** 5
                                   + (c['six'] + c['six_log'] * D.log(v)) * v ** 6
-                                  + c['seven'] * v ** 7 + phi_Tidal + 0.) / (v ** 5.)
+                                  + c['seven'] * v ** 7 + phi_Tidal + phi_Ecc) / (v ** 5.)
         return phase + phiR - np.pi * 0.25
 
     def Ampl(self, f, ev):

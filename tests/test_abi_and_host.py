@@ -55,7 +55,7 @@ def test_library_binds_without_gpu(so_path):
 
 def test_struct_layouts_match_header():
     # field order and sizes of the ABI structs
-    assert ctypes.sizeof(N.GwfWaveform) == 8 * 4 + 2 * 8 + 3 * 8
+    assert ctypes.sizeof(N.GwfWaveform) == 8 * 4 + 2 * 8 + 3 * 8 + 2 * 4 + 8
     assert ctypes.sizeof(N.GwfDetector) == 4 * 4 + 5 * 8 + 2 * 8 + 2 * 4 + 2 * 8
 
 
@@ -70,8 +70,10 @@ def test_parnums_and_nparams():
     d = IMRPhenomD()
     assert d.nParams == 11 and d.fcutPar == 0.2 and d.objType == 'BBH'
     assert len(d.QNMgrid_a) == 1003
+    e = TaylorF2_RestrictedPN(is_eccentric=True)
+    assert e.nParams == 12 and e.ParNums['ecc'] == 11
     with pytest.raises(NotImplementedError):
-        TaylorF2_RestrictedPN(is_eccentric=True)
+        TaylorF2_RestrictedPN(is_eccentric=True, is_tidal=True)
 
 
 def test_gwsignal_validation():

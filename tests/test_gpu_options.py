@@ -349,3 +349,44 @@ def test_wf_overlap_and_amplitudes(gpu):
     Apo, Aco = odets[1].gw_amplitudes(owf, ev, f)
     amax = np.max(np.hypot(Apo, Aco))
     assert np.max(np.abs(Ap - Apo)) <= 1e-12 * amax and np.max(np.abs(Ac - Aco)) <= 1e-12 * amax
+
+
+def test_eccentric_taylorf2(gpu):
+    """TaylorF2_RestrictedPN(is_eccentric=True) (waveforms.py:814-845): the
+    phase, the SNR and the 12-parameter Fisher (ParNums['ecc'] = 11) against
+    the oracle, both with v0 = min v over the grid and with fRef_ecc."""
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    from oracle import detector as OD
+    from oracle import waveforms as OW
+    ev = bbh(6, seed=97)
+    ev['ecc'] = np.linspace(0.01, 0.15, 6)
+    for fref in (None, 5.0):
+        wf = TaylorF2_RestrictedPN(is_eccentric=True, fRef_ecc=fref)
+        owf = OW.TaylorF2(is_eccentric=True, fRef_ecc=fref)
+        assert wf.ParNums['ecc'] == 11 and wf.nParams == 12
+        f = np.geomspace(2.0, 0.9 * owf.fcut(ev), 500)
+        ph = wf.Phi(f, **ev)
+        pho = owf.Phi(f, ev)
+        assert np.max(np.abs(ph - pho)) <= 1e-13 * np.max(np.abs(pho)) + 1e-9
+        sites = NET_C1
+        net = hip_net(wf, sites, True)
+        odets = oracle_net(sites, True)
+        snr = net.SNR(dict(ev), res=1000)
+        snro = OD.network_snr(odets, owf, ev, 1000)
+        assert np.max(np.abs(snr / snro - 1)) <= 1e-11
+        F = net.FisherMatr(dict(ev), res=1000)
+        assert F.shape == (12, 12, 6)
+        Fd = OD.network_fisher(odets, owf, ev, 1000, mode='dual')
+        e = norm_err(F, Fd)
+        print('eccentric TF2 fRef_ecc=%s Fisher err %.3e' % (fref, e), flush=True)
+        assert e <= 1e-10
+        assert np.all(np.diag(F[:, :, 0]) > 0) and F[11, 11, 0] > 0
+    ev0 = dict(ev)
+    ev0['ecc'] = np.zeros(6)
+    wf = TaylorF2_RestrictedPN(is_eccentric=True)
+    wfc = TaylorF2_RestrictedPN()
+    net, netc = hip_net(wf, NET_C1[1:2], True), hip_net(wfc, NET_C1[1:2], True)
+    F0, Fc = net.FisherMatr(dict(ev0), res=800), netc.FisherMatr(dict(ev0), res=800)
+    assert norm_err(F0[:11, :11], Fc) <= 1e-13      # e0 = 0: the circular Fisher on the other rows
+    with pytest.raises(ValueError):
+        net.SNR({k: v for k, v in ev.items() if k != 'ecc'}, res=200)

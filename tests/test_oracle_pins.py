@@ -774,3 +774,58 @@ def test_overlap_closed_forms():
     ev2['dL'] = ev['dL'] * 3.0
     ov, s1, s2 = det.wf_overlap(wf, wf, ev, ev2, 400)
     assert np.allclose(ov / (s1 * s2), np.cos(0.7), rtol=1e-9, atol=0)
+
+
+# ---------------------------------------------------------- eccentric TaylorF2
+def _ecc_ev(n=4, seed=16):
+    from gwfast_amd import catalog
+    ev = catalog.draw_bbh(n, seed=seed)
+    ev['ecc'] = np.linspace(0.02, 0.2, n)
+    return ev
+
+
+def test_eccentric_tf2_limits():
+    """waveforms.py:814-845 restated: e0 = 0 gives the circular phase; at
+    low frequency the correction tends to the leading low-eccentricity term
+    -(2355/1462) e0^2 (v0/v)^(19/3) 3/(128 eta v^5) (Krolak et al. 1995;
+    Yunes & Arun 2009); it is even in e0, so d Phi/d e0 = 2 (Phi - Phi_circ)/e0."""
+    wf, wc = OW.TaylorF2(is_eccentric=True), OW.TaylorF2()
+    ev = _ecc_ev()
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    f = np.geomspace(1e-7, 1e-3, 80)[:, None] / M[None, :]
+    e0 = dict(ev)
+    e0['ecc'] = np.zeros_like(ev['ecc'])
+    assert np.array_equal(wf.Phi(f, e0), wc.Phi(f, ev))
+    dphi = wf.Phi(f, ev) - wc.Phi(f, ev)
+    v = (np.pi * M * f) ** (1. / 3.)
+    v0 = v[0]
+    lead = 3. / (128. * ev['eta']) * (-2355. / 1462.) * ev['ecc'] ** 2 * (v0 / v) ** (19. / 3.) / v ** 5
+    rel = np.abs(dphi / lead - 1.)
+    assert np.all(rel[:5] < 10. * v[:5] ** 2)   # v < 8e-3: only the O(v^2) 1PN terms remain
+    evd = dict(ev)
+    evd['ecc'] = D.Dual.seed(ev['ecc'], 0, 1)
+    g = wf.Phi(f, evd)
+    # dphi is a difference of two phases: allow its cancellation error
+    tol = 1e-10 * np.abs(2. * dphi / ev['ecc']) + 1e-14 * np.abs(wc.Phi(f, ev)) * 2. / ev['ecc']
+    assert np.all(np.abs(g.d[0] - 2. * dphi / ev['ecc']) <= tol)
+
+
+def test_eccentric_tf2_dual_vs_complex_step():
+    """The eccentric phase is holomorphic in (Mc, eta, e0) as the reference
+    declares TaylorF2 (waveforms.py:741): dual tangents equal complex-step
+    derivatives; v0 = amin(v) follows Mc and eta along its arg-min."""
+    wf = OW.TaylorF2(is_eccentric=True)
+    ev = _ecc_ev(3, 17)
+    M = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    f = np.geomspace(0.001, 0.02, 50)[:, None] / M[None, :]
+    keys = ['Mc', 'eta', 'ecc']
+    evd = dict(ev)
+    for k, key in enumerate(keys):
+        evd[key] = D.Dual.seed(ev[key], k, 3)
+    ph = wf.Phi(f, evd)
+    for k, key in enumerate(keys):
+        h = 1e-20 * np.abs(ev[key])
+        evc = {q: np.asarray(v, dtype=complex) for q, v in ev.items()}
+        evc[key] = ev[key] + 1j * h
+        dph = np.imag(wf.Phi(f.astype(complex), evc)) / h
+        assert np.max(np.abs(ph.d[k] - dph)) <= 1e-10 * np.max(np.abs(dph)), key
