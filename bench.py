@@ -307,7 +307,7 @@ def main():
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP64_PEAK_TFLOPS,
                      'traffic': K_FISHER_TRAFFIC_10K if n == N_EV else None,
-                     'kernel': 'k_fisher_pc<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
+                     'kernel': 'k_fisher_pq<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
                      'note': 'FP64 dense peak; algorithmic flops = Gram 4 P^2 N_f N_ifo per Fisher'},
     }
     if not args.no_plain:

@@ -3,6 +3,7 @@
 // geometry, 144-147 for the arm angle, 745-765 / 1057 for the triangle's
 // null-stream third arm).
 #pragma once
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstring>
@@ -81,6 +82,7 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
         A.c2x = c2[k];
       }
       A.wgt = 1.0;
+      A.swgt = 1.0;
     }
     // network-sum arms of this site: eigenpairs of M = sum_k (s_k, c_k)^T (s_k, c_k)
     double m11 = 0., m12 = 0., m22 = 0.;
@@ -105,8 +107,8 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
       const double l2 = m22 + (s / c) * m12 * (c != 0 ? 1.0 : 0.0);
       ArmK A1, A2;
       A1.site = A2.site = d;
-      A1.s2x = c; A1.c2x = -s; A1.wgt = l1;
-      A2.s2x = s; A2.c2x = c; A2.wgt = l2;
+      A1.s2x = c; A1.c2x = -s; A1.wgt = l1; A1.swgt = std::sqrt(std::max(l1, 0.0));
+      A2.s2x = s; A2.c2x = c; A2.wgt = l2; A2.swgt = std::sqrt(std::max(l2, 0.0));
       if (l1 != 0.0) net.sum_arm[net.n_sum_arms++] = A1;
       if (l2 != 0.0) net.sum_arm[net.n_sum_arms++] = A2;
     }

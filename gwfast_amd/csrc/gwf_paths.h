@@ -60,6 +60,7 @@ struct ArmK {
   int site;
   double s2x, c2x;  // sin/cos(2 (xax + rot)); a derived (null-stream) arm holds -(sum of two)
   double wgt;       // Gram weight (1 for a physical arm)
+  double swgt;      // sqrt(max(wgt, 0))
 };
 
 // Two arm tables: the physical arms (per-arm outputs, SNR, strain) and the

@@ -89,7 +89,7 @@ int gwf_abi_version(void);
 const char* gwf_last_error(void);
 
 /* Bytes of device workspace the calls below need for n_ev events
- * (per-event coefficient records). */
+ * (per-event coefficient records, then 256 bytes of queue counters). */
 size_t gwf_workspace_bytes(int64_t n_ev);
 
 /* Number of interferometer arms of a network: L -> 1, T -> 3. */
@@ -163,8 +163,11 @@ int gwf_prepare(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det,
                 int64_t n_ev, int32_t res, void* workspace, size_t ws_bytes, void* stream);
 int gwf_snr_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
                      double* snrsq_out, const void* workspace, size_t ws_bytes, void* stream);
+/* gwf_fisher_prepared also keeps two queue counters after the records (the
+ * network sum's producer / consumer launch), so its workspace is writable and
+ * at least gwf_workspace_bytes(n_ev). */
 int gwf_fisher_prepared(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, int64_t n_ev, int32_t res,
-                        int32_t arm, double* fisher_out, const void* workspace, size_t ws_bytes, void* stream);
+                        int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes, void* stream);
 
 /* GWSignal.GWstrain and the derivative rows of _SignalDerivatives for one arm
  * of one detector on the caller grid f (n_f, n_ev) (signal.py:485-655,

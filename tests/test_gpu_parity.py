@@ -243,6 +243,26 @@ def test_ragged_grids_match_uniform(gpu):
         assert np.max(np.abs(snr[sel] - su) / su) <= 1e-15
 
 
+def test_ragged_chunk_edges(gpu):
+    """Ragged grid lengths at the 64-point chunk edges of the producer /
+    consumer pairs (2, 3, 63, 64, 65, 127, 128, 129 and 1000 points), more
+    events than the resident pairs, IMRPhenomD on the 5-interferometer
+    network: every event equals a uniform launch at its own length."""
+    from gwfast_amd.waveforms import IMRPhenomD
+    lens = np.array([2, 3, 63, 64, 65, 127, 128, 129, 1000], dtype=np.int32)
+    n = 1500
+    res = lens[np.arange(n) % len(lens)]
+    ev = bbh(n, seed=33)
+    net = hip_net(IMRPhenomD(), NET_C1, True)
+    F = net.FisherMatr(dict(ev), res=res)
+    assert np.all(np.isfinite(F))
+    for r in lens:
+        sel = np.where(res == r)[0][:40]
+        sub = {k: v[sel] for k, v in ev.items()}
+        Fu = net.FisherMatr(dict(sub), res=int(r))
+        assert norm_err(F[:, :, sel], Fu) <= 1e-15, r
+
+
 # ------------------------------------------------------------ post-Fisher
 def test_post_fisher_batch_vs_oracle(gpu):
     """fixParams / addPrior (bit-exact gathers), sigma and the 90% sky area
