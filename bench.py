@@ -30,11 +30,11 @@ RES = 2000
 SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
 GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per k_fisher_pc launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
-# (39,153 KiB, x2.0: a 256 MiB device copy reads 131,087 KiB by that counter)
-# and WRITE_SIZE (22,578 KiB, x1.0) runs of this script; summary in
+# HBM bytes per k_fisher_pq launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
+# (38,143 KiB, x2.0: a 256 MiB device copy reads 131,087 KiB by that counter)
+# and WRITE_SIZE (10,463 KiB, x1.0) runs of this script; summary in
 # profiles/k_fisher_imrphenomd_net_10k_2000f.md
-K_FISHER_TRAFFIC_10K = (39153.2 * 2.0 + 22577.5) * 1024
+K_FISHER_TRAFFIC_10K = (38143.3 * 2.0 + 10462.7) * 1024
 
 
 def build_net():
