@@ -150,19 +150,44 @@ __device__ __forceinline__ void pq_fail(unsigned int* counter, int lane) {
   if (lane == 0) atomicOr(counter + 1, 1u);
 }
 
-// Consumer with the Gram on the matrix cores: the arm rows of the chunk's 64
-// points, scaled by sqrt(w) (w = trapezoid weight x arm weight / S), go
-// through a per-wave LDS tile X[point][param] (row stride PQ_XS doubles) and
-// G += X^T X runs as 16 v_mfma_f64_16x16x4f64 per (arm, re|im): lane l feeds
-// X[4 s + l / 16][l % 16] as both operands (A[i][k] is lane i + 16 k, B[k][j]
-// lane j + 16 k), and the 16 x 16 accumulator holds G[i][j] in lane
-// j + 16 (i % 4), element i / 4.  Points past res contribute zero rows.
+// Gram on the matrix cores: the arm rows of the chunk's 64 points, scaled by
+// sqrt(w) (w = trapezoid weight x arm weight / S), go through a per-wave LDS
+// tile X[point][param] (row stride PQ_XS doubles) and G += X^T X runs as 16
+// v_mfma_f64_16x16x4f64 per (arm, re|im): lane l feeds X[4 s + l / 16][l % 16]
+// as both operands (A[i][k] is lane i + 16 k, B[k][j] lane j + 16 k), and the
+// 16 x 16 accumulator holds G[i][j] in lane j + 16 (i % 4), element i / 4.
+// Points past res contribute zero rows; NACC accumulators rotate so that
+// consecutive MFMAs do not wait for each other.
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PQ_XS = 17;
+#ifndef GWF_PQ_NACC
+#define GWF_PQ_NACC 2  // measured: 4 accumulators no faster
+#endif
+constexpr int PQ_NACC = GWF_PQ_NACC;
 
+template <int P>
+__device__ __forceinline__ void mfma_rows(const double* re, const double* im, double* X, int lane, v4d* acc) {
+  const int rd = (lane >> 4) * PQ_XS + (lane & 15);
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    const double* v = part ? im : re;
+#pragma unroll
+    for (int i = 0; i < P; ++i) X[lane * PQ_XS + i] = v[i];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const double x = X[4 * t * PQ_XS + rd];
+      acc[t % PQ_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc[t % PQ_NACC], 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Consumer with every Gram update on the matrix cores (mfma_rows); dead lanes
+// (points past res) take part with zero rows.
 template <int MODEL, int NT, int P>
-__device__ __forceinline__ void pc_consume_mfma(const NetK& net, const double* rec, int base, int res, int lane,
-                                                const double* buf, double* X, v4d& acc0, v4d& acc1) {
+__device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec, int base, int res, int lane,
+                                              const double* buf, double* X, v4d* acc) {
   typedef PcLayout<NT> L;
   __asm__ volatile("" ::: "memory");
   const bool live = base + lane < res;
@@ -219,40 +244,22 @@ __device__ __forceinline__ void pc_consume_mfma(const NetK& net, const double* r
 #pragma unroll
       for (int i = 0; i < P; ++i) re[i] = im[i] = 0.0;
     }
-    const int rd = (lane >> 4) * PQ_XS + (lane & 15);
-#pragma unroll
-    for (int i = 0; i < P; ++i) X[lane * PQ_XS + i] = re[i];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int t = 0; t < 16; t += 2) {
-      const double v0 = X[4 * t * PQ_XS + rd], v1 = X[4 * (t + 1) * PQ_XS + rd];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v1, acc1, 0, 0, 0);
-    }
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int i = 0; i < P; ++i) X[lane * PQ_XS + i] = im[i];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int t = 0; t < 16; t += 2) {
-      const double v0 = X[4 * t * PQ_XS + rd], v1 = X[4 * (t + 1) * PQ_XS + rd];
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(v0, v0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(v1, v1, acc1, 0, 0, 0);
-    }
-    __builtin_amdgcn_wave_barrier();
+    mfma_rows<P>(re, im, X, lane, acc);
   }
 }
 
-// Stores the accumulated G (acc0 + acc1, layout of pc_consume_mfma): the
-// upper-triangle element (i <= j) goes to both F[i][j] and F[j][i].
+// Stores the event's Fisher from the NACC matrix-core accumulators (element
+// (i, j) in lane j + 16 (i % 4), item i / 4): the upper-triangle element
+// (i <= j) goes to both F[i][j] and F[j][i].
 template <int P>
-__device__ __forceinline__ void store_mfma(const v4d& acc0, const v4d& acc1, double* __restrict__ F, int64_t n_ev,
-                                           int64_t e, int lane) {
+__device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F, int64_t n_ev, int64_t e, int lane) {
   const int j = lane & 15;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int i = (lane >> 4) + 4 * r;
-    const double v = acc0[r] + acc1[r];
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PQ_NACC; ++q) v += acc[q][r];
     if (i < P && j < P && i <= j) {
       F[(int64_t)(i * P + j) * n_ev + e] = v;
       if (i != j) F[(int64_t)(j * P + i) * n_ev + e] = v;
@@ -277,13 +284,13 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
                                                                       const int* __restrict__ res_ev,
                                                                       unsigned int* counter, double* __restrict__ F) {
   typedef PcLayout<NT> L;
-  constexpr bool MF = P >= GWF_PQ_MFMA_MIN_P;  // Gram on the matrix cores (pc_consume_mfma)
+  constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P;  // Gram on the matrix cores (pc_consume_mx)
   // ring depth: two buffers where three would not fit beside the MFMA tiles
-  constexpr int NB = (MF && L::NV > 19) ? 2 : PQ_NBUF;
+  constexpr int NB = (MX && L::NV > 19) ? 2 : PQ_NBUF;
   __shared__ double buf[PC_EPB][NB][L::NV * 64];
   __shared__ int head[PC_EPB][NB][4];  // event, chunk, grid length of the item in each buffer
   __shared__ int cnt[PC_EPB][2];            // items produced, items consumed
-  __shared__ double xt[PC_EPB][MF ? 64 * PQ_XS : 1];  // consumers' transposition tiles
+  __shared__ double xt[PC_EPB][MX ? 64 * PQ_XS : 1];  // consumers' transposition tiles
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int slot = wave % PC_EPB;
   if (threadIdx.x < 2 * PC_EPB) cnt[threadIdx.x >> 1][threadIdx.x & 1] = 0;
@@ -332,10 +339,10 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       if (!RAGGED) e += stride;
     }
-  } else if constexpr (MF) {
+  } else if constexpr (MX) {
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
-    v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+    v4d acc[PQ_NACC];
     for (int k = 0;; ++k) {
       const int s = k % NB;
       if (!pq_wait(&cnt[slot][0], k + 1)) {
@@ -347,13 +354,12 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       const int t = __builtin_amdgcn_readfirstlane(head[slot][s][1]);
       const int res = __builtin_amdgcn_readfirstlane(head[slot][s][2]);
       if (t == 0) {
-        acc0 = v4d{0.0, 0.0, 0.0, 0.0};
-        acc1 = acc0;
+#pragma unroll
+        for (int q = 0; q < PQ_NACC; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
       }
-      pc_consume_mfma<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X, acc0,
-                                    acc1);
+      pc_consume_mx<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X, acc);
       pq_release(&cnt[slot][1], k + 1);
-      if ((t + 1) * 64 >= res) store_mfma<P>(acc0, acc1, F, n_ev, e, lane);
+      if ((t + 1) * 64 >= res) store_mx<P>(acc, F, n_ev, e, lane);
     }
   } else {
     constexpr int NG = P * (P + 1) / 2;
