@@ -730,9 +730,12 @@ GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec
   buf[L::SD * 64 + lane] = sdel;
 }
 
-// Consumer: the Gram updates of point base + lane (network sum arms).
-template <int MODEL, int NT, int P>
-GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, int lane, const double* buf, double* G) {
+// Consumer: the Gram updates of point base + lane (network sum arms; MASKED:
+// the arms whose bit is set in mask, a site with all arms set through its
+// sum arms, as fisher_event).
+template <int MODEL, int NT, int P, bool MASKED = false>
+GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, int lane, const double* buf, double* G,
+                       uint32_t mask = 0u) {
   typedef PcLayout<NT> L;
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
   __asm__ volatile("" ::: "memory");
@@ -756,23 +759,49 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
   int cur_site = -1, cur_psd = -1;
   double Sf = 1.0;
   Geo g;
-  for (int k = 0; k < net.n_sum_arms; ++k) {
-    const ArmK& arm = net.sum_arm[k];
-    const int s = arm.site;
-    const SiteK& S = net.site[s];
-    const double* sc = rec + REC_SITE + s * SC_COUNT;
-    if (s != cur_site) {
+  if constexpr (MASKED) {
+    for (int s = 0; s < net.n_sites; ++s) {
+      const SiteK& S = net.site[s];
+      const uint32_t all = (1u << S.narm) - 1u;
+      const uint32_t bits = (mask >> S.arm0) & all;
+      if (!bits) continue;
+      const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (S.psd != cur_psd) {
         Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
                         : (S.psd == 1 ? buf[L::S1 * 64 + lane] : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
         cur_psd = S.psd;
       }
       site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
-      cur_site = s;
+      const bool full = bits == all;
+      const int cnt = full ? S.nsum : S.narm;
+      for (int j = 0; j < cnt; ++j) {
+        if (!full && !((bits >> j) & 1u)) continue;
+        const ArmK& arm = full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j];
+        double re[P], im[P], Ap, Ac;
+        arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+        gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+      }
     }
-    double re[P], im[P], Ap, Ac;
-    arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
-    gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+  } else {
+    (void)mask;
+    for (int k = 0; k < net.n_sum_arms; ++k) {
+      const ArmK& arm = net.sum_arm[k];
+      const int s = arm.site;
+      const SiteK& S = net.site[s];
+      const double* sc = rec + REC_SITE + s * SC_COUNT;
+      if (s != cur_site) {
+        if (S.psd != cur_psd) {
+          Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
+                          : (S.psd == 1 ? buf[L::S1 * 64 + lane] : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
+          cur_psd = S.psd;
+        }
+        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
+        cur_site = s;
+      }
+      double re[P], im[P], Ap, Ac;
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
+      gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+    }
   }
 }
 

@@ -184,10 +184,11 @@ __device__ __forceinline__ void mfma_rows(const double* re, const double* im, do
 }
 
 // Consumer with every Gram update on the matrix cores (mfma_rows); dead lanes
-// (points past res) take part with zero rows.
-template <int MODEL, int NT, int P>
+// (points past res) take part with zero rows.  MASKED: the arms whose bit is
+// set in mask, a site with all arms set through its sum arms.
+template <int MODEL, int NT, int P, bool MASKED>
 __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec, int base, int res, int lane,
-                                              const double* buf, double* X, v4d* acc) {
+                                              const double* buf, double* X, v4d* acc, uint32_t mask) {
   typedef PcLayout<NT> L;
   __asm__ volatile("" ::: "memory");
   const bool live = base + lane < res;
@@ -214,11 +215,10 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
   int cur_site = -1, cur_psd = -1;
   double sw = 0.0;  // sqrt(wt / S) of the current PSD
   Geo g;
-  for (int k = 0; k < net.n_sum_arms; ++k) {
-    const ArmK& arm = net.sum_arm[k];
+  // the rows of one arm of site s into the Gram (zero rows on dead lanes)
+  auto arm_rows = [&](const ArmK& arm, int s) {
     double re[P], im[P];
     if (live) {
-      const int s = arm.site;
       const SiteK& S = net.site[s];
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (s != cur_site) {
@@ -245,6 +245,23 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
       for (int i = 0; i < P; ++i) re[i] = im[i] = 0.0;
     }
     mfma_rows<P>(re, im, X, lane, acc);
+  };
+  if constexpr (MASKED) {
+    for (int s = 0; s < net.n_sites; ++s) {
+      const SiteK& S = net.site[s];
+      const uint32_t all = (1u << S.narm) - 1u;
+      const uint32_t bits = (mask >> S.arm0) & all;
+      if (!bits) continue;
+      const bool full = bits == all;
+      const int cnt = full ? S.nsum : S.narm;
+      for (int j = 0; j < cnt; ++j) {
+        if (!full && !((bits >> j) & 1u)) continue;
+        arm_rows(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s);
+      }
+    }
+  } else {
+    (void)mask;
+    for (int k = 0; k < net.n_sum_arms; ++k) arm_rows(net.sum_arm[k], net.sum_arm[k].site);
   }
 }
 
@@ -278,10 +295,11 @@ __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F,
 #define GWF_PQ_MFMA_MIN_P 12
 #endif
 
-template <int MODEL, int NT, int P, bool LIN, bool RAGGED>
+template <int MODEL, int NT, int P, bool LIN, bool RAGGED, bool MASKED>
 __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_waveform w, NetK net,
                                                                       const double* __restrict__ rec, int64_t n_ev,
                                                                       const int* __restrict__ res_ev,
+                                                                      const uint32_t* __restrict__ arm_mask,
                                                                       unsigned int* counter, double* __restrict__ F) {
   typedef PcLayout<NT> L;
   constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P;  // Gram on the matrix cores (pc_consume_mx)
@@ -343,6 +361,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
     v4d acc[PQ_NACC];
+    uint32_t mask = 0u;
     for (int k = 0;; ++k) {
       const int s = k % NB;
       if (!pq_wait(&cnt[slot][0], k + 1)) {
@@ -356,14 +375,17 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       if (t == 0) {
 #pragma unroll
         for (int q = 0; q < PQ_NACC; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
       }
-      pc_consume_mx<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X, acc);
+      pc_consume_mx<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X,
+                                          acc, mask);
       pq_release(&cnt[slot][1], k + 1);
       if ((t + 1) * 64 >= res) store_mx<P>(acc, F, n_ev, e, lane);
     }
   } else {
     constexpr int NG = P * (P + 1) / 2;
     double G[NG];
+    uint32_t mask = 0u;
     for (int k = 0;; ++k) {
       const int s = k % NB;
       if (!pq_wait(&cnt[slot][0], k + 1)) {
@@ -377,8 +399,9 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       if (t == 0) {
 #pragma unroll
         for (int q = 0; q < NG; ++q) G[q] = 0.0;
+        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
       }
-      pc_consume<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], G);
+      pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], G, mask);
       pq_release(&cnt[slot][1], k + 1);
       if ((t + 1) * 64 >= res) store_packed<P>(G, F, n_ev, e, lane);
     }
@@ -1208,6 +1231,9 @@ int nt_of(const gwf_waveform* wf) {
 #ifndef GWF_RAGGED_PQ
 #define GWF_RAGGED_PQ 1
 #endif
+#ifndef GWF_MASKED_PQ
+#define GWF_MASKED_PQ 1  // duty-cycle (arm-mask) network sums on k_fisher_pq
+#endif
 
 template <bool LIN>
 void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st, const NetK& net, const double* rec,
@@ -1269,9 +1295,10 @@ void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
 // The network Fisher on producer / consumer pairs (k_fisher_pq): one
 // resident block per CU, then the NaN fill that reports an expired wait.
 // counter: two words, zeroed here.
-template <bool LIN, bool RAGGED>
+template <bool LIN, bool RAGGED, bool MASKED = false>
 int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, const double* rec, int64_t n_ev,
-                     const int32_t* res_ev, unsigned int* counter, double* out, int P) {
+                     const int32_t* res_ev, unsigned int* counter, double* out, int P,
+                     const uint32_t* arm_mask = nullptr) {
   int dev = 0, n_cu = 0;
   int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
   if (rc) return rc;
@@ -1281,20 +1308,20 @@ int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, co
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_ev + PC_EPB - 1) / PC_EPB, n_cu));
   const dim3 g((unsigned)blocks), b(128 * PC_EPB);
   if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD, 4, 11, LIN, RAGGED>), g, b, 0, st, *wf, net, rec, n_ev,
-                       res_ev, counter, out);
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD, 4, 11, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+                       n_ev, res_ev, arm_mask, counter, out);
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN, RAGGED>), g, b, 0, st, *wf, net, rec,
-                       n_ev, res_ev, counter, out);
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN, RAGGED, MASKED>), g, b, 0, st, *wf,
+                       net, rec, n_ev, res_ev, arm_mask, counter, out);
   else if (nt_of(wf) == 5)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 5, 12, LIN, RAGGED>), g, b, 0, st, *wf, net, rec, n_ev,
-                       res_ev, counter, out);
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 5, 12, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+                       n_ev, res_ev, arm_mask, counter, out);
   else if (nt_of(wf) == 6)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 6, 13, LIN, RAGGED>), g, b, 0, st, *wf, net, rec, n_ev,
-                       res_ev, counter, out);
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 6, 13, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+                       n_ev, res_ev, arm_mask, counter, out);
   else
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 4, 11, LIN, RAGGED>), g, b, 0, st, *wf, net, rec, n_ev,
-                       res_ev, counter, out);
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 4, 11, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+                       n_ev, res_ev, arm_mask, counter, out);
   const int64_t n_out = (int64_t)P * P * n_ev;
   hipLaunchKernelGGL(k_pq_poison, dim3((unsigned)std::min<int64_t>((n_out + 255) / 256, 1024)), dim3(256), 0, st,
                      counter, out, n_out);
@@ -1537,6 +1564,16 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)((n_ev + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK)), block(BLOCK);
   const double* rec = (const double*)workspace;
+  if (GWF_MASKED_PQ) {  // producer / consumer pairs (needs the counter words)
+    if (ws_bytes < gwf_workspace_bytes(n_ev)) return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+    unsigned int* counter = (unsigned int*)((char*)workspace + (size_t)n_ev * REC_DOUBLES * sizeof(double));
+    rc = net.lin ? launch_fisher_pq<true, false, true>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+                                                        7 + nt_of(wf), arm_mask)
+                 : launch_fisher_pq<false, false, true>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+                                                         7 + nt_of(wf), arm_mask);
+    if (rc) return rc;
+    return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
+  }
   if (net.lin)
     launch_fisher_masked<true>(wf, grid, block, st, net, rec, n_ev, arm_mask, fisher_out);
   else
