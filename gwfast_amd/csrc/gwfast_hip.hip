@@ -165,7 +165,7 @@ constexpr int PQ_XS = 17;
 #endif
 constexpr int PQ_NACC = GWF_PQ_NACC;
 
-template <int P>
+template <int P, int NA = PQ_NACC>
 __device__ __forceinline__ void mfma_rows(const double* re, const double* im, double* X, int lane, v4d* acc) {
   const int rd = (lane >> 4) * PQ_XS + (lane & 15);
 #pragma unroll
@@ -177,16 +177,20 @@ __device__ __forceinline__ void mfma_rows(const double* re, const double* im, do
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
       const double x = X[4 * t * PQ_XS + rd];
-      acc[t % PQ_NACC] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc[t % PQ_NACC], 0, 0, 0);
+      acc[t % NA] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc[t % NA], 0, 0, 0);
     }
     __builtin_amdgcn_wave_barrier();
   }
 }
 
 // Consumer with every Gram update on the matrix cores (mfma_rows); dead lanes
-// (points past res) take part with zero rows.  MASKED: the arms whose bit is
-// set in mask, a site with all arms set through its sum arms.
-template <int MODEL, int NT, int P, bool MASKED>
+// (points past res) take part with zero rows.  MODE PQ_SUM: the network-sum
+// arms into acc[0..PQ_NACC); PQ_MASKED: the arms whose bit is set in mask, a
+// site with all arms set through its sum arms; PQ_ARMS: physical arm a alone
+// into acc[a] (a < PQ_ARMS_MAX), for the per-arm Fisher matrices.
+constexpr int PQ_SUM = 0, PQ_MASKED = 1, PQ_ARMS = 2;
+constexpr int PQ_ARMS_MAX = 8;
+template <int MODEL, int NT, int P, int MODE>
 __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec, int base, int res, int lane,
                                               const double* buf, double* X, v4d* acc, uint32_t mask) {
   typedef PcLayout<NT> L;
@@ -215,9 +219,8 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
   int cur_site = -1, cur_psd = -1;
   double sw = 0.0;  // sqrt(wt / S) of the current PSD
   Geo g;
-  // the rows of one arm of site s into the Gram (zero rows on dead lanes)
-  auto arm_rows = [&](const ArmK& arm, int s) {
-    double re[P], im[P];
+  // the scaled rows of one arm of site s (zero rows on dead lanes)
+  auto arm_rows = [&](const ArmK& arm, int s, double* re, double* im) {
     if (live) {
       const SiteK& S = net.site[s];
       const double* sc = rec + REC_SITE + s * SC_COUNT;
@@ -244,9 +247,18 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
 #pragma unroll
       for (int i = 0; i < P; ++i) re[i] = im[i] = 0.0;
     }
-    mfma_rows<P>(re, im, X, lane, acc);
   };
-  if constexpr (MASKED) {
+  if constexpr (MODE == PQ_ARMS) {
+    (void)mask;
+#pragma unroll
+    for (int a = 0; a < PQ_ARMS_MAX; ++a) {
+      if (a < net.n_arms) {
+        double re[P], im[P];
+        arm_rows(net.arm[a], net.arm[a].site, re, im);
+        mfma_rows<P, 1>(re, im, X, lane, acc + a);
+      }
+    }
+  } else if constexpr (MODE == PQ_MASKED) {
     for (int s = 0; s < net.n_sites; ++s) {
       const SiteK& S = net.site[s];
       const uint32_t all = (1u << S.narm) - 1u;
@@ -256,19 +268,25 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
       const int cnt = full ? S.nsum : S.narm;
       for (int j = 0; j < cnt; ++j) {
         if (!full && !((bits >> j) & 1u)) continue;
-        arm_rows(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s);
+        double re[P], im[P];
+        arm_rows(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s, re, im);
+        mfma_rows<P>(re, im, X, lane, acc);
       }
     }
   } else {
     (void)mask;
-    for (int k = 0; k < net.n_sum_arms; ++k) arm_rows(net.sum_arm[k], net.sum_arm[k].site);
+    for (int k = 0; k < net.n_sum_arms; ++k) {
+      double re[P], im[P];
+      arm_rows(net.sum_arm[k], net.sum_arm[k].site, re, im);
+      mfma_rows<P>(re, im, X, lane, acc);
+    }
   }
 }
 
 // Stores the event's Fisher from the NACC matrix-core accumulators (element
 // (i, j) in lane j + 16 (i % 4), item i / 4): the upper-triangle element
 // (i <= j) goes to both F[i][j] and F[j][i].
-template <int P>
+template <int P, int NA = PQ_NACC>
 __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F, int64_t n_ev, int64_t e, int lane) {
   const int j = lane & 15;
 #pragma unroll
@@ -276,7 +294,7 @@ __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F,
     const int i = (lane >> 4) + 4 * r;
     double v = 0.0;
 #pragma unroll
-    for (int q = 0; q < PQ_NACC; ++q) v += acc[q][r];
+    for (int q = 0; q < NA; ++q) v += acc[q][r];
     if (i < P && j < P && i <= j) {
       F[(int64_t)(i * P + j) * n_ev + e] = v;
       if (i != j) F[(int64_t)(j * P + i) * n_ev + e] = v;
@@ -295,14 +313,15 @@ __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F,
 #define GWF_PQ_MFMA_MIN_P 12
 #endif
 
-template <int MODEL, int NT, int P, bool LIN, bool RAGGED, bool MASKED>
+template <int MODEL, int NT, int P, bool LIN, bool RAGGED, int MODE>
 __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_waveform w, NetK net,
                                                                       const double* __restrict__ rec, int64_t n_ev,
                                                                       const int* __restrict__ res_ev,
                                                                       const uint32_t* __restrict__ arm_mask,
                                                                       unsigned int* counter, double* __restrict__ F) {
   typedef PcLayout<NT> L;
-  constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P;  // Gram on the matrix cores (pc_consume_mx)
+  constexpr bool MASKED = MODE == PQ_MASKED, ARMS = MODE == PQ_ARMS;
+  constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P || ARMS;  // Gram on the matrix cores (pc_consume_mx)
   // ring depth: two buffers where three would not fit beside the MFMA tiles
   constexpr int NB = (MX && L::NV > 19) ? 2 : PQ_NBUF;
   __shared__ double buf[PC_EPB][NB][L::NV * 64];
@@ -357,6 +376,33 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       if (!RAGGED) e += stride;
     }
+  } else if constexpr (ARMS) {  // F: PQ_ARMS_MAX-bounded (n_arms, P, P, n_ev), one accumulator per arm
+    double* X = xt[slot];
+    for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;
+    v4d acc[PQ_ARMS_MAX];
+    for (int k = 0;; ++k) {
+      const int s = k % NB;
+      if (!pq_wait(&cnt[slot][0], k + 1)) {
+        pq_fail(counter, lane);
+        return;
+      }
+      const int e = __builtin_amdgcn_readfirstlane(head[slot][s][0]);
+      if (e < 0) return;
+      const int t = __builtin_amdgcn_readfirstlane(head[slot][s][1]);
+      const int res = __builtin_amdgcn_readfirstlane(head[slot][s][2]);
+      if (t == 0) {
+#pragma unroll
+        for (int a = 0; a < PQ_ARMS_MAX; ++a) acc[a] = v4d{0.0, 0.0, 0.0, 0.0};
+      }
+      pc_consume_mx<MODEL, NT, P, PQ_ARMS>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X,
+                                           acc, 0u);
+      pq_release(&cnt[slot][1], k + 1);
+      if ((t + 1) * 64 >= res) {
+#pragma unroll
+        for (int a = 0; a < PQ_ARMS_MAX; ++a)
+          if (a < net.n_arms) store_mx<P, 1>(acc + a, F + (int64_t)a * P * P * n_ev, n_ev, e, lane);
+      }
+    }
   } else if constexpr (MX) {
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
@@ -377,8 +423,8 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
         for (int q = 0; q < PQ_NACC; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
         if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
       }
-      pc_consume_mx<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X,
-                                          acc, mask);
+      pc_consume_mx<MODEL, NT, P, MODE>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X, acc,
+                                        mask);
       pq_release(&cnt[slot][1], k + 1);
       if ((t + 1) * 64 >= res) store_mx<P>(acc, F, n_ev, e, lane);
     }
@@ -1295,7 +1341,7 @@ void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
 // The network Fisher on producer / consumer pairs (k_fisher_pq): one
 // resident block per CU, then the NaN fill that reports an expired wait.
 // counter: two words, zeroed here.
-template <bool LIN, bool RAGGED, bool MASKED = false>
+template <bool LIN, bool RAGGED, int MODE = PQ_SUM>
 int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, const double* rec, int64_t n_ev,
                      const int32_t* res_ev, unsigned int* counter, double* out, int P,
                      const uint32_t* arm_mask = nullptr) {
@@ -1308,21 +1354,21 @@ int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, co
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_ev + PC_EPB - 1) / PC_EPB, n_cu));
   const dim3 g((unsigned)blocks), b(128 * PC_EPB);
   if (wf->model == GWF_MODEL_IMRPHENOMD)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD, 4, 11, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD, 4, 11, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN, RAGGED, MASKED>), g, b, 0, st, *wf,
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD_NRTIDALV2, 6, 13, LIN, RAGGED, MODE>), g, b, 0, st, *wf,
                        net, rec, n_ev, res_ev, arm_mask, counter, out);
   else if (nt_of(wf) == 5)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 5, 12, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 5, 12, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
   else if (nt_of(wf) == 6)
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 6, 13, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 6, 13, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
   else
-    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 4, 11, LIN, RAGGED, MASKED>), g, b, 0, st, *wf, net, rec,
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 4, 11, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
-  const int64_t n_out = (int64_t)P * P * n_ev;
+  const int64_t n_out = (int64_t)P * P * n_ev * (MODE == PQ_ARMS ? net.n_arms : 1);
   hipLaunchKernelGGL(k_pq_poison, dim3((unsigned)std::min<int64_t>((n_out + 255) / 256, 1024)), dim3(256), 0, st,
                      counter, out, n_out);
   return 0;
@@ -1567,9 +1613,9 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   if (GWF_MASKED_PQ) {  // producer / consumer pairs (needs the counter words)
     if (ws_bytes < gwf_workspace_bytes(n_ev)) return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
     unsigned int* counter = (unsigned int*)((char*)workspace + (size_t)n_ev * REC_DOUBLES * sizeof(double));
-    rc = net.lin ? launch_fisher_pq<true, false, true>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+    rc = net.lin ? launch_fisher_pq<true, false, PQ_MASKED>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
                                                         7 + nt_of(wf), arm_mask)
-                 : launch_fisher_pq<false, false, true>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+                 : launch_fisher_pq<false, false, PQ_MASKED>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
                                                          7 + nt_of(wf), arm_mask);
     if (rc) return rc;
     return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
@@ -1579,6 +1625,29 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   else
     launch_fisher_masked<false>(wf, grid, block, st, net, rec, n_ev, arm_mask, fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
+}
+
+int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                    int64_t ld, int64_t n_ev, int32_t res, double* fisher_out, void* workspace, size_t ws_bytes,
+                    void* stream) {
+  int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
+  if (rc) return rc;
+  NetK net;
+  rc = build_net_checked(dets, n_det, res, net);
+  if (rc) return rc;
+  if (net.n_arms > PQ_ARMS_MAX) return fail(-EINVAL, "gwf_fisher_arms takes at most 8 arms (gwf_fisher per arm)");
+  if (n_ev == 0) return 0;
+  if (!fisher_out) return fail(-EINVAL, "fisher_out is NULL");
+  if (ws_bytes < gwf_workspace_bytes(n_ev)) return fail(-EINVAL, "workspace too small (see gwf_workspace_bytes)");
+  hipStream_t st = (hipStream_t)stream;
+  const double* rec = (const double*)workspace;
+  unsigned int* counter = (unsigned int*)((char*)workspace + (size_t)n_ev * REC_DOUBLES * sizeof(double));
+  rc = net.lin ? launch_fisher_pq<true, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+                                                         7 + nt_of(wf))
+               : launch_fisher_pq<false, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
+                                                          7 + nt_of(wf));
+  if (rc) return rc;
+  return hip_check(hipGetLastError(), "gwf_fisher_arms launch");
 }
 
 int gwf_snr(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,

@@ -123,9 +123,27 @@ class DetNet(object):
             S.reparam_device(total, rows, use_m1m2, use_chi1chi2)
             return total.cpu().numpy()
         if return_all:
+            # per-arm matrices, one gwf_fisher_arms launch per group of detectors; the
+            # duty-cycle masks are drawn first, detector by detector, as the per-detector
+            # calls of the reference draw them
+            n = next(iter(self.signals.values()))._rows(evParams).shape[1]
+            masks = {name: sig._duty_masks(n) for name, sig in self.signals.items()}
+            per_det = {}
+            for g in self._groups():
+                sigs = [s for _, s in g]
+                r = S.resolve_res(sigs[0], evParams, res, df)
+                rows = sigs[0]._rows(evParams)
+                Fa = None
+                if np.ndim(r) == 0 and sum(s.n_arms for s in sigs) <= S.ARMS_PER_LAUNCH:
+                    Fa = S.fisher_arms_device(sigs, sigs[0].wf_model, rows, r, lin=lin)
+                a = 0
+                for name, sig in g:
+                    part = Fa[a:a + sig.n_arms] if Fa is not None else None
+                    per_det[name] = sig._fisher_per_arm(rows, r, lin, use_m1m2, use_chi1chi2, masks[name], Fa=part)
+                    a += sig.n_arms
             allF = {}
             for name, sig in self.signals.items():
-                F_ = sig.FisherMatr(evParams, res=res, return_all=True, **opts)
+                F_ = per_det[name]
                 if sig.detector_shape == 'T':
                     for i in range(3):
                         allF[name + '_%s' % i] = F_[i]

@@ -156,18 +156,27 @@ class GWSignal(object):
                     allD[k] = allD[k] * (masks[k][None, :, None] if allD[k].ndim == 3 else masks[k][None, :])
             return allF, allD
         if return_all:
-            Fs = []
-            for k in range(self.n_arms):
-                F = fisher_device([self], self.wf_model, rows, res, arm=k, lin=lin)
-                reparam_device(F, rows, use_m1m2, use_chi1chi2)
-                Fs.append(F.cpu().numpy() * (masks[k] if masks is not None else 1.0))
-            return Fs
+            return self._fisher_per_arm(rows, res, lin, use_m1m2, use_chi1chi2, masks)
         if masks is not None:
             F = fisher_masked_device([self], self.wf_model, rows, res, mask_words([masks]), lin=lin)
         else:
             F = fisher_device([self], self.wf_model, rows, res, lin=lin)
         reparam_device(F, rows, use_m1m2, use_chi1chi2)
         return F.cpu().numpy()
+
+    def _fisher_per_arm(self, rows, res, lin, use_m1m2, use_chi1chi2, masks, Fa=None):
+        """FisherMatr(return_all=True): the list of this detector's per-arm
+        matrices (host), each times its duty-cycle mask.  Fa: this detector's
+        (n_arms, P, P, N) slice of a gwf_fisher_arms launch, else one launch
+        here (uniform grids) or one per arm (ragged grids)."""
+        if Fa is None and np.ndim(res) == 0 and self.n_arms <= ARMS_PER_LAUNCH:
+            Fa = fisher_arms_device([self], self.wf_model, rows, res, lin=lin)
+        Fs = []
+        for k in range(self.n_arms):
+            F = Fa[k] if Fa is not None else fisher_device([self], self.wf_model, rows, res, arm=k, lin=lin)
+            reparam_device(F, rows, use_m1m2, use_chi1chi2)
+            Fs.append(F.cpu().numpy() * (masks[k] if masks is not None else 1.0))
+        return Fs
 
     def _fisher_with_derivs(self, evParams, rows, res, return_derivatives, lin=False):
         n = rows.shape[1]
@@ -430,6 +439,24 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None, lin=False):
         E.call('gwf_fisher_ragged', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n,
                N.ptr(r_ev), arm, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
+
+
+def fisher_arms_device(signals, wf, rows, res, lin=False):
+    """The Fisher matrix of every physical arm of signals (in network arm
+    order) from one gwf_fisher_arms launch: (n_arms, P, P, N_ev) device
+    tensor; res: one grid length for all events."""
+    t = E.torch()
+    n = rows.shape[1]
+    P = wf.nParams
+    n_arms = sum(s.n_arms for s in signals)
+    out = t.empty((n_arms, P, P, n), dtype=t.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    E.call('gwf_fisher_arms', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n,
+           int(res), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    return out
+
+
+ARMS_PER_LAUNCH = 8  # gwf_fisher_arms' limit
 
 
 def mask_words(masks_per_signal):

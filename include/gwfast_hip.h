@@ -141,6 +141,18 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
                       int64_t ld, int64_t n_ev, int32_t res, const uint32_t* arm_mask, double* fisher_out,
                       void* workspace, size_t ws_bytes, void* stream);
 
+/* GWSignal.FisherMatr / DetNet.FisherMatr with return_all=True
+ * (signal.py:1087-1098, network.py:125-135; the catalogue driver always asks
+ * for it, calculate_forecasts_from_catalog.py:507-513): the Fisher matrix of
+ * every physical arm of the network from one launch (the waveform and its
+ * tangents are evaluated once per frequency for all arms).  fisher_out:
+ * (n_arms, P, P, n_ev), arm a at fisher_out + a * P * P * n_ev, arms in the
+ * order of gwf_num_arms (detectors in order; a triangle's three arms, the
+ * third -(first + second)).  At most 8 arms. */
+int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                    int64_t ld, int64_t n_ev, int32_t res, double* fisher_out, void* workspace, size_t ws_bytes,
+                    void* stream);
+
 /* Ragged per-event grids (no reference counterpart: gwfast uses one res for
  * all events, signal.py:890-898): event e uses jnp.geomspace(fmin, fcut_e,
  * res_ev[e]) with res_ev a DEVICE int32 array (values < 2 are treated as 2).

@@ -390,3 +390,37 @@ def test_eccentric_taylorf2(gpu):
     assert norm_err(F0[:11, :11], Fc) <= 1e-13      # e0 = 0: the circular Fisher on the other rows
     with pytest.raises(ValueError):
         net.SNR({k: v for k, v in ev.items() if k != 'ecc'}, res=200)
+
+
+@pytest.mark.parametrize('model', ['PD', 'NRT'])
+def test_fisher_all_arms_one_launch(gpu, model):
+    """gwf_fisher_arms (FisherMatr(return_all=True), signal.py:1087-1098,
+    network.py:125-135): every physical arm's Fisher from one launch equals
+    the single-arm launches; their sum is the network Fisher; DetNet's
+    return_all dict carries them per detector and arm, and agrees with the
+    oracle per arm."""
+    from gwfast_amd import signal as S
+    from oracle import detector as OD
+    wf, owf = _models(model)
+    n, res = 40, 600
+    ev = (bbh if model == 'PD' else bns)(n, seed=61)
+    net = hip_net(wf, NET_C1, True)
+    sigs = list(net.signals.values())
+    sigs[0]._complete(ev)
+    rows = sigs[0]._rows(ev)
+    Fa = S.fisher_arms_device(sigs, wf, rows, res).cpu().numpy()
+    assert Fa.shape == (5, wf.nParams, wf.nParams, n)
+    for k in range(5):
+        Fk = S.fisher_device(sigs, wf, rows, res, arm=k).cpu().numpy()
+        e = norm_err(Fa[k], Fk)
+        print(model, 'arm', k, 'one-launch vs single-arm %.3e' % e, flush=True)
+        assert e <= 1e-13
+    full = S.fisher_device(sigs, wf, rows, res).cpu().numpy()
+    assert norm_err(Fa.sum(axis=0), full) <= 1e-13
+    allF = net.FisherMatr(dict(ev), res=res, return_all=True)
+    assert sorted(allF) == sorted(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM', 'net'])
+    for k, key in enumerate(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM']):
+        assert norm_err(allF[key], Fa[k]) <= 1e-15
+    odets = oracle_net(NET_C1[1:2], True)
+    Fo = OD.network_fisher(odets, owf, ev, res, mode='dual')
+    assert norm_err(allF['CE1Id'], Fo) <= 1e-10
