@@ -80,14 +80,66 @@ def _duty_snr_fisher(net, rows, res, snr_th, masks):
     return E.to_dev(snr), idx, F
 
 
+def _arm_keys(name, sig):
+    return [name] if sig.detector_shape == 'L' else [name + '_%s' % a for a in range(3)]
+
+
+def _all_arms(net, rows, res, snr_th, masks=None):
+    """--return_all (calculate_forecasts_from_catalog.py:421, 507-513): the
+    per-arm SNRs of every event and the per-arm Fisher matrices of the
+    detected events (gwf_fisher_arms, one launch per detector group), keyed as
+    DetNet.SNR / FisherMatr(return_all=True), times the duty-cycle masks when
+    given; 'net' holds the network SNR and the summed Fisher."""
+    from . import screen as SC
+    from . import signal as S
+    n = rows.shape[1]
+    groups = net._groups()
+    snrs = {}
+    net_sq = np.zeros(n)
+    for g in groups:
+        sq = S.snr_sq_device([s for _, s in g], rows, res).cpu().numpy()
+        k = 0
+        for name, s in g:
+            for key in _arm_keys(name, s):
+                snrs[key] = 2 * np.sqrt(sq[k]) * (masks[key] if masks is not None else 1.0)
+                net_sq += snrs[key] ** 2
+                k += 1
+    snrs['net'] = np.sqrt(net_sq)
+    det = np.nonzero(snrs['net'] > snr_th)[0]
+    idx = E.to_dev(det.astype(np.int64), E.torch().int64)
+    if not det.size:
+        return snrs, idx, None
+    rd = SC.gather(rows, idx)
+    fishers, total = {}, None
+    for g in groups:
+        sigs = [s for _, s in g]
+        n_arms = sum(s.n_arms for s in sigs)
+        if n_arms <= S.ARMS_PER_LAUNCH:
+            Fa = S.fisher_arms_device(sigs, sigs[0].wf_model, rd, res)
+            arms = [Fa[a] for a in range(n_arms)]
+        else:
+            arms = [S.fisher_device(sigs, sigs[0].wf_model, rd, res, arm=a) for a in range(n_arms)]
+        a = 0
+        for name, s in g:
+            for key in _arm_keys(name, s):
+                F = arms[a] if masks is None else arms[a] * E.to_dev(masks[key][det])
+                fishers[key] = F
+                total = F.clone() if total is None else total + F
+                a += 1
+    fishers['net'] = total
+    return snrs, idx, fishers
+
+
 def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None, band=(8.0, 16.0),
-                 duty_factor=None, seeds=None):
+                 duty_factor=None, seeds=None, return_all=False):
     """One chunk: returns a dict with 'snrs' (network SNR of every event;
     with a screen, NaN where the exact SNR was not evaluated), 'idxs_det',
     'fisher' (P', P', n_det), 'cov', 'eps', 'sky_area', 'errors' (P', n_det),
     'cond_numbers', 'parNums'.  Device-resident until the end.  duty_factor
     < 1 applies the driver's per-arm duty-cycle masks (seeds: one per
-    detector) to the SNRs and Fisher matrices."""
+    detector) to the SNRs and Fisher matrices.  return_all adds 'all_snrs'
+    and 'all_fishers': per-arm dicts as DetNet's return_all (with params_fix
+    applied to every matrix), the network Fisher being their sum."""
     t = E.torch()
     from . import screen as SC
     check_evparams(events)
@@ -96,7 +148,19 @@ def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None,
     wf = sig0.wf_model
     rows = E.to_dev(E.param_rows(events, wf.is_tidal))
     n = rows.shape[1]
-    if duty_factor is not None and duty_factor < 1:
+    all_snrs = all_fishers = None
+    if return_all:
+        if screen is not None:
+            raise NotImplementedError('return_all needs the exact per-arm SNRs; use it without the screen')
+        masks = None
+        if duty_factor is not None and duty_factor < 1:
+            if seeds is None or len(seeds) != len(net.signals):
+                raise ValueError('duty_factor needs one seed per detector')
+            masks = duty_masks(net, seeds, duty_factor, n)
+        all_snrs, idx, all_fishers = _all_arms(net, rows, res, snr_th, masks)
+        snrs = E.to_dev(all_snrs['net'])
+        F = all_fishers['net'] if all_fishers is not None else None
+    elif duty_factor is not None and duty_factor < 1:
         if screen is not None:
             raise NotImplementedError('the learned screen predicts duty-free SNRs; use it without duty_factor')
         if seeds is None or len(seeds) != len(net.signals):
@@ -114,12 +178,16 @@ def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None,
         snrs[idx] = r['snr']
     parNums = dict(wf.ParNums)
     out = {'snrs': snrs.cpu().numpy(), 'idxs_det': idx.cpu().numpy()}
+    if all_snrs is not None:
+        out['all_snrs'] = all_snrs
     if F is None:
         out.update(fisher=None, parNums=parNums)
         return out
     if len(params_fix):
         fix_idx = sorted(parNums[p] for p in params_fix)
         F = FT.fix_params_device(F, fix_idx)
+        if all_fishers is not None:
+            all_fishers = {k: FT.fix_params_device(v, fix_idx) for k, v in all_fishers.items()}
         for pm in params_fix:
             for k in parNums.keys():
                 if parNums[k] > parNums[pm]:
@@ -135,12 +203,20 @@ def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None,
     sig, area = FT.sigma_sky_area_device(C, theta, parNums['theta'], parNums['phi'], 90, 'SqDeg')
     out.update(fisher=F.cpu().numpy(), cov=C.cpu().numpy(), eps=eps.cpu().numpy(), sky_area=area.cpu().numpy(),
                errors=sig.cpu().numpy(), cond_numbers=cond.cpu().numpy(), parNums=parNums)
+    if all_fishers is not None:
+        out['all_fishers'] = {k: v.cpu().numpy() for k, v in all_fishers.items()}
     return out
 
 
 def to_file(res, out_path, suff=''):
     """calculate_forecasts_from_catalog.py:222-247 (file names and layouts)."""
     np.savetxt(os.path.join(out_path, 'snrs' + suff + '.txt'), res['snrs'])
+    # --return_all (to_file_all, 337-380): the per-arm SNRs and Fisher matrices,
+    # as .npz archives keyed like the driver's hdf5 groups (h5py is absent)
+    if res.get('all_snrs') is not None:
+        np.savez(os.path.join(out_path, 'all_snrs' + suff + '.npz'), **res['all_snrs'])
+    if res.get('all_fishers') is not None:
+        np.savez(os.path.join(out_path, 'all_fishers' + suff + '.npz'), **res['all_fishers'])
     if res.get('fisher') is None:
         return
     np.save(os.path.join(out_path, 'fishers' + suff + '.npy'), res['fisher'])
@@ -153,12 +229,15 @@ def to_file(res, out_path, suff=''):
 
 
 def run_catalog(events, net, out_path, batch_size=10000, snr_th=12.0, params_fix=(), res=1000, resume=False,
-                screen=None, band=(8.0, 16.0), idx_in=0, verbose=True, duty_factor=None, seeds=None):
+                screen=None, band=(8.0, 16.0), idx_in=0, verbose=True, duty_factor=None, seeds=None,
+                return_all=False):
     """Chunked forecast of a catalogue with per-chunk files (suffix
     _<i>_to_<j>, global event indices from idx_in).  With resume=True a chunk
     whose snrs file exists is skipped.  duty_factor (< 1) and seeds (one per
     detector; drawn when None, as the driver does) apply the duty cycle to
-    every chunk with the same seeds.  Returns the list of suffixes."""
+    every chunk with the same seeds; return_all also writes the per-arm
+    SNRs and Fisher matrices (all_snrs / all_fishers<suffix>.npz).  Returns
+    the list of suffixes."""
     os.makedirs(out_path, exist_ok=True)
     n = len(np.atleast_1d(events['dL']))
     if duty_factor is not None and duty_factor >= 1:
@@ -175,7 +254,8 @@ def run_catalog(events, net, out_path, batch_size=10000, snr_th=12.0, params_fix
                 print('Batch already present, skipping...', flush=True)
             continue
         t0 = time.perf_counter()
-        r = compute_errs(_subset(events, slice(i0, i1)), net, snr_th, params_fix, res, screen, band, duty_factor, seeds)
+        r = compute_errs(_subset(events, slice(i0, i1)), net, snr_th, params_fix, res, screen, band, duty_factor, seeds,
+                         return_all)
         r['idxs_det'] = r['idxs_det'] + idx_in + i0
         to_file(r, out_path, suff)
         if verbose:

@@ -326,6 +326,57 @@ def test_catalogue_forecast_files_and_resume(gpu, tmp_path):
     assert p.read_text() == 'sentinel\n'
 
 
+def test_catalogue_forecast_return_all(gpu, tmp_path):
+    """--return_all (calculate_forecasts_from_catalog.py:421, 507-513,
+    578-590, 337-380): per-arm SNRs of every event and per-arm Fisher
+    matrices (params_fix applied) of the detected ones, keyed as DetNet's
+    return_all; their network sums equal snrs / fishers of the same run and
+    DetNet.FisherMatr(return_all=True); with a duty factor the arms carry the
+    driver's masks."""
+    from gwfast_amd import forecast as FC, fisherTools as FT
+    from gwfast_amd.waveforms import IMRPhenomD
+    ev = bbh(120, seed=83)
+    ev['dL'] = ev['dL'] * 4.0
+    wf = IMRPhenomD()
+    net = hip_net(wf, NET_C1, True)
+    suffs = FC.run_catalog(ev, net, str(tmp_path), batch_size=60, params_fix=['Phicoal'], res=500, verbose=False,
+                           return_all=True)
+    keys = ['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM', 'net']
+    snr_all = net.SNR(dict(ev), res=500, return_all=True)
+    for s, (i0, i1) in zip(suffs, ((0, 60), (60, 120))):
+        A = np.load(tmp_path / ('all_snrs%s.npz' % s))
+        assert sorted(A.files) == sorted(keys)
+        for k in keys:
+            want = snr_all[k][i0:i1] if k != 'net' else snr_all['net'][i0:i1]
+            assert np.allclose(A[k], want, rtol=1e-13, atol=0), k
+        assert np.allclose(A['net'], np.loadtxt(tmp_path / ('snrs%s.txt' % s)), rtol=1e-15, atol=0)
+        fp = tmp_path / ('all_fishers%s.npz' % s)
+        if not fp.exists():
+            continue
+        B = np.load(fp)
+        det = np.atleast_1d(np.loadtxt(tmp_path / ('idxs_det%s.txt' % s))).astype(int)
+        assert B['net'].shape == (10, 10, len(det))
+        assert np.allclose(B['net'], sum(B[k] for k in keys[:-1]), rtol=1e-14, atol=0)
+        assert norm_err(B['net'], np.load(tmp_path / ('fishers%s.npy' % s))) == 0.0
+        Fd = net.FisherMatr({k: v[det] for k, v in ev.items()}, res=500, return_all=True)
+        for k in keys:
+            Fr, _ = FT.fixParams(Fd[k], wf.ParNums, ['Phicoal'])
+            assert norm_err(B[k], Fr) <= 1e-13, k
+    # duty cycle: masked arms
+    seeds = [11, 12, 13]
+    FC.run_catalog(ev, net, str(tmp_path / 'df'), batch_size=120, res=500, verbose=False, return_all=True,
+                   duty_factor=0.6, seeds=seeds)
+    masks = FC.duty_masks(net, seeds, 0.6, 120)
+    A = np.load(tmp_path / 'df' / 'all_snrs_0_to_120.npz')
+    for k in keys[:-1]:
+        assert np.allclose(A[k], snr_all[k] * masks[k], rtol=1e-13, atol=0), k
+    B = np.load(tmp_path / 'df' / 'all_fishers_0_to_120.npz')
+    det = np.atleast_1d(np.loadtxt(tmp_path / 'df' / 'idxs_det_0_to_120.txt')).astype(int)
+    for k in keys[:-1]:
+        zero = masks[k][det] == 0
+        assert np.all(B[k][:, :, zero] == 0), k
+
+
 def test_nrtidal_waveform_methods(gpu):
     wf, owf = _models('NRT')
     ev = bns(5, seed=13)
