@@ -131,6 +131,32 @@ def pcie_leg(net, ev, n, steps=5):
             'path': 'pinned host rows -> HBM, DetNet.fisher_device, (11, 11, N) -> pinned host'}
 
 
+def per_arm_leg(net, rows, n, steps=5):
+    """DetNet.FisherMatr(return_all=True), the call of the reference's
+    catalogue driver: the Fisher matrix of each of the 5 arms of the same
+    workload from one gwf_fisher_arms launch (setup included), against one
+    single-arm gwf_fisher launch per arm."""
+    import torch
+    from gwfast_amd import signal as S
+    sigs = list(net.signals.values())
+    wf = sigs[0].wf_model
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / steps
+
+    one = timed(lambda: S.fisher_arms_device(sigs, wf, rows, RES))
+    per = timed(lambda: [S.fisher_device(sigs, wf, rows, RES, arm=k) for k in range(5)])
+    return {'value': n / one, 'unit': 'per-arm Fisher sets/s (5 matrices each)', 'ms_per_step': 1e3 * one,
+            'single_arm_launches_ms': 1e3 * per, 'speedup_vs_single_arm_launches': per / one,
+            'path': 'gwf_fisher_arms (k_fisher_pq, matrix-core Gram per arm)'}
+
+
 def ragged_leg(n=100_000, steps=3, seed=11):
     """Config C3: TaylorF2 with tidal terms (13 parameters), the same network
     with Earth rotation, 10^5 BNS events each on its own grid length drawn
@@ -230,7 +256,7 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=512)
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
-    ap.add_argument('--no-extra', action='store_true', help='skip the PCIe-inclusive and ragged (C3) legs')
+    ap.add_argument('--no-extra', action='store_true', help='skip the PCIe-inclusive, per-arm and ragged (C3) legs')
     ap.add_argument('--screen-only', action='store_true', help='run only the learned-screen leg')
     ap.add_argument('--screen-events', type=int, default=1_000_000)
     ap.add_argument('--screen-train', type=int, default=1_000_000)
@@ -327,6 +353,7 @@ def main():
         del Fs
     if not args.no_extra:
         out['pcie_inclusive'] = pcie_leg(net, ev, n)
+        out['per_arm_return_all'] = per_arm_leg(net, rows, n)
         out['ragged_c3'] = ragged_leg()
     if not args.no_screen:
         del F, ws
