@@ -313,6 +313,28 @@ __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F,
 #define GWF_PQ_MFMA_MIN_P 12
 #endif
 
+// Consumer side of a pair: waits for each item in turn, runs item(e, t,
+// res, chunk) on it, releases its buffer and, after the last chunk of event
+// e, runs done(e); returns at the end marker or when a wait expires.
+template <int NB, int CH, class Item, class Done>
+__device__ __forceinline__ void pq_consumer(int* produced, int* consumed, int (*hd)[4], double (*bf)[CH],
+                                            unsigned int* counter, int lane, Item&& item, Done&& done) {
+  for (int k = 0;; ++k) {
+    const int s = k % NB;
+    if (!pq_wait(produced, k + 1)) {
+      pq_fail(counter, lane);
+      return;
+    }
+    const int e = __builtin_amdgcn_readfirstlane(hd[s][0]);
+    if (e < 0) return;
+    const int t = __builtin_amdgcn_readfirstlane(hd[s][1]);
+    const int res = __builtin_amdgcn_readfirstlane(hd[s][2]);
+    item(e, t, res, bf[s]);
+    pq_release(consumed, k + 1);
+    if ((t + 1) * 64 >= res) done(e);
+  }
+}
+
 template <int MODEL, int NT, int P, bool LIN, bool RAGGED, int MODE>
 __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_waveform w, NetK net,
                                                                       const double* __restrict__ rec, int64_t n_ev,
@@ -376,81 +398,46 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       if (!RAGGED) e += stride;
     }
-  } else if constexpr (ARMS) {  // F: PQ_ARMS_MAX-bounded (n_arms, P, P, n_ev), one accumulator per arm
-    double* X = xt[slot];
-    for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;
-    v4d acc[PQ_ARMS_MAX];
-    for (int k = 0;; ++k) {
-      const int s = k % NB;
-      if (!pq_wait(&cnt[slot][0], k + 1)) {
-        pq_fail(counter, lane);
-        return;
-      }
-      const int e = __builtin_amdgcn_readfirstlane(head[slot][s][0]);
-      if (e < 0) return;
-      const int t = __builtin_amdgcn_readfirstlane(head[slot][s][1]);
-      const int res = __builtin_amdgcn_readfirstlane(head[slot][s][2]);
-      if (t == 0) {
-#pragma unroll
-        for (int a = 0; a < PQ_ARMS_MAX; ++a) acc[a] = v4d{0.0, 0.0, 0.0, 0.0};
-      }
-      pc_consume_mx<MODEL, NT, P, PQ_ARMS>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X,
-                                           acc, 0u);
-      pq_release(&cnt[slot][1], k + 1);
-      if ((t + 1) * 64 >= res) {
-#pragma unroll
-        for (int a = 0; a < PQ_ARMS_MAX; ++a)
-          if (a < net.n_arms) store_mx<P, 1>(acc + a, F + (int64_t)a * P * P * n_ev, n_ev, e, lane);
-      }
-    }
-  } else if constexpr (MX) {
+  } else if constexpr (MX) {  // Gram on the matrix cores
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
-    v4d acc[PQ_NACC];
+    constexpr int NA = ARMS ? PQ_ARMS_MAX : PQ_NACC;        // ARMS: one accumulator per physical arm
+    v4d acc[NA];
     uint32_t mask = 0u;
-    for (int k = 0;; ++k) {
-      const int s = k % NB;
-      if (!pq_wait(&cnt[slot][0], k + 1)) {
-        pq_fail(counter, lane);
-        return;
-      }
-      const int e = __builtin_amdgcn_readfirstlane(head[slot][s][0]);
-      if (e < 0) return;
-      const int t = __builtin_amdgcn_readfirstlane(head[slot][s][1]);
-      const int res = __builtin_amdgcn_readfirstlane(head[slot][s][2]);
-      if (t == 0) {
+    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, lane,
+                    [&](int e, int t, int res, const double* chunk) {
+                      if (t == 0) {
 #pragma unroll
-        for (int q = 0; q < PQ_NACC; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
-        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
-      }
-      pc_consume_mx<MODEL, NT, P, MODE>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], X, acc,
-                                        mask);
-      pq_release(&cnt[slot][1], k + 1);
-      if ((t + 1) * 64 >= res) store_mx<P>(acc, F, n_ev, e, lane);
-    }
-  } else {
+                        for (int q = 0; q < NA; ++q) acc[q] = v4d{0.0, 0.0, 0.0, 0.0};
+                        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
+                      }
+                      pc_consume_mx<MODEL, NT, P, MODE>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
+                                                        X, acc, mask);
+                    },
+                    [&](int e) {
+                      if constexpr (ARMS) {  // F: (n_arms, P, P, n_ev)
+#pragma unroll
+                        for (int a = 0; a < PQ_ARMS_MAX; ++a)
+                          if (a < net.n_arms) store_mx<P, 1>(acc + a, F + (int64_t)a * P * P * n_ev, n_ev, e, lane);
+                      } else {
+                        store_mx<P>(acc, F, n_ev, e, lane);
+                      }
+                    });
+  } else {  // Gram on the VALU, packed upper triangle per lane
     constexpr int NG = P * (P + 1) / 2;
     double G[NG];
     uint32_t mask = 0u;
-    for (int k = 0;; ++k) {
-      const int s = k % NB;
-      if (!pq_wait(&cnt[slot][0], k + 1)) {
-        pq_fail(counter, lane);
-        return;
-      }
-      const int e = __builtin_amdgcn_readfirstlane(head[slot][s][0]);
-      if (e < 0) return;
-      const int t = __builtin_amdgcn_readfirstlane(head[slot][s][1]);
-      const int res = __builtin_amdgcn_readfirstlane(head[slot][s][2]);
-      if (t == 0) {
+    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, lane,
+                    [&](int e, int t, int res, const double* chunk) {
+                      if (t == 0) {
 #pragma unroll
-        for (int q = 0; q < NG; ++q) G[q] = 0.0;
-        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
-      }
-      pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, buf[slot][s], G, mask);
-      pq_release(&cnt[slot][1], k + 1);
-      if ((t + 1) * 64 >= res) store_packed<P>(G, F, n_ev, e, lane);
-    }
+                        for (int q = 0; q < NG; ++q) G[q] = 0.0;
+                        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
+                      }
+                      pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
+                                                       G, mask);
+                    },
+                    [&](int e) { store_packed<P>(G, F, n_ev, e, lane); });
   }
 }
 
