@@ -375,7 +375,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
         return;
       }
       int res = net.res;
-      if (RAGGED) res = max(__builtin_amdgcn_readfirstlane(res_ev[e]), 2);
+      if (RAGGED) res = min(max(__builtin_amdgcn_readfirstlane(res_ev[e]), 2), 1 << 30);  // chunk counts stay in int
       const int nch = (res + 63) / 64;
       const double* r = rec + e * (int64_t)REC_DOUBLES;
       double a, b;
@@ -485,8 +485,7 @@ __global__ __launch_bounds__(256) void k_fisher_ragged(gwf_waveform w, NetK net,
     double G[NG];
 #pragma unroll
     for (int k = 0; k < NG; ++k) G[k] = 0.0;
-    int res = res_ev[e];
-    res = res < 2 ? 2 : res;
+    const int res = min(max(res_ev[e], 2), 1 << 30);
     fisher_event<MODEL, NT, P, false, LIN>(w, net, rec + e * (int64_t)REC_DOUBLES, res, lane, 64, arm_sel, G);
     store_packed<P>(G, F, n_ev, e, lane);
   }
@@ -508,7 +507,7 @@ __global__ __launch_bounds__(256, 3) void k_snr(gwf_waveform w, NetK net, const 
   double acc[MAXA];
 #pragma unroll
   for (int k = 0; k < MAXA; ++k) acc[k] = 0.0;
-  snr_event<MODEL, MAXA>(w, net, r, res_ev ? (res_ev[e] < 2 ? 2 : res_ev[e]) : net.res, lane, 64, acc);
+  snr_event<MODEL, MAXA>(w, net, r, res_ev ? min(max(res_ev[e], 2), 1 << 30) : net.res, lane, 64, acc);
 #pragma unroll
   for (int k = 0; k < MAXA; ++k) {
     if (k < net.n_arms) {

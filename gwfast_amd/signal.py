@@ -396,7 +396,10 @@ def _res_arg(res, n, device):
     if np.ndim(res) == 0:
         return int(res), None
     t = E.torch()
-    r = t.as_tensor(np.asarray(res), dtype=t.int32, device=device).contiguous()
+    a = np.asarray(res)
+    if a.size and (a.min() < 2 or a.max() > 1 << 30):
+        raise ValueError('every grid length must be between 2 and 2**30')
+    r = t.as_tensor(a, dtype=t.int32, device=device).contiguous()
     if r.numel() != n:
         raise ValueError('res must be an int or have one entry per event')
     return 0, r

@@ -155,7 +155,7 @@ int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_
 
 /* Ragged per-event grids (no reference counterpart: gwfast uses one res for
  * all events, signal.py:890-898): event e uses jnp.geomspace(fmin, fcut_e,
- * res_ev[e]) with res_ev a DEVICE int32 array (values < 2 are treated as 2).
+ * res_ev[e]) with res_ev a DEVICE int32 array (values are clamped to [2, 2^30]).
  * gwf_fisher_ragged runs one persistent launch whose wavefronts pull events
  * from an atomic counter kept in the workspace, so uneven grids balance
  * across the GPU.  Outputs as gwf_snr / gwf_fisher. */

@@ -161,3 +161,17 @@ def test_phenomd_fit_tables_match_reference_text():
         assert m, n
         vals = [float(x) for x in m.group(1).split(',')]
         assert vals == nums, n
+
+
+def test_ragged_grid_length_validation():
+    """Per-event grid lengths are checked on the host before they reach the
+    ragged kernels: each in [2, 2**30], one per event."""
+    from gwfast_amd import signal as S
+    with pytest.raises(ValueError):
+        S._res_arg(np.array([1, 500]), 2, 'cpu')
+    with pytest.raises(ValueError):
+        S._res_arg(np.array([500, 2 ** 31 - 1]), 2, 'cpu')
+    with pytest.raises(ValueError):
+        S._res_arg(np.array([500, 600, 700]), 2, 'cpu')
+    r0, r = S._res_arg(np.array([2, 500]), 2, 'cpu')
+    assert r0 == 0 and r.tolist() == [2, 500]
