@@ -5,6 +5,7 @@
 // which are exactly the parameters gwfast differentiates with JAX
 // (gwfast/signal.py:1153-1167, derivargs = (1,2,10,11,16,17)).
 #pragma once
+#include <type_traits>
 #include <cmath>
 #include <cstdint>
 
@@ -142,6 +143,90 @@ GWF_HD D<N> operator/(double a, const D<N>& b) {
   const double s = -r.v * ib;
 #pragma unroll
   for (int k = 0; k < N; ++k) r.d[k] = b.d[k] * s;
+  return r;
+}
+
+// Mixed arithmetic: a D<M> with M < N stands for a D<N> whose last N - M
+// tangents are zero (functions of M G f depend on Mc and eta only, the first
+// two tangents); the result is a D<N> and the zero tangents are not
+// multiplied.  The values equal those of the full D<N> arithmetic.
+template <int N, int M>
+using MixedIf = typename std::enable_if<(M < N), int>::type;
+
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator+(const D<N>& a, const D<M>& b) {
+  D<N> r = a;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator+(const D<M>& a, const D<N>& b) {
+  D<N> r = b;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator-(const D<N>& a, const D<M>& b) {
+  D<N> r = a;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator-(const D<M>& a, const D<N>& b) {
+  D<N> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] - b.d[k];
+#pragma unroll
+  for (int k = M; k < N; ++k) r.d[k] = -b.d[k];
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator*(const D<N>& a, const D<M>& b) {
+  D<N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+#pragma unroll
+  for (int k = M; k < N; ++k) r.d[k] = a.d[k] * b.v;
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator*(const D<M>& a, const D<N>& b) {
+  D<N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+#pragma unroll
+  for (int k = M; k < N; ++k) r.d[k] = a.v * b.d[k];
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator/(const D<N>& a, const D<M>& b) {
+  D<N> r;
+  const double ib = 1.0 / b.v;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+#pragma unroll
+  for (int k = M; k < N; ++k) r.d[k] = a.d[k] * ib;
+  return r;
+}
+template <int N, int M, MixedIf<N, M> = 0>
+GWF_HD D<N> operator/(const D<M>& a, const D<N>& b) {
+  D<N> r;
+  const double ib = 1.0 / b.v;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int k = 0; k < M; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * ib;
+#pragma unroll
+  for (int k = M; k < N; ++k) r.d[k] = (-r.v * b.d[k]) * ib;
   return r;
 }
 

@@ -403,30 +403,24 @@ struct WfPoint {
   D<2> tau;   // tau_star over (Mc, eta)
 };
 
-template <int NT>
-GWF_HD D<2> first2(const D<NT>& x) {
-  D<2> r;
-  r.v = x.v;
-  r.d[0] = x.d[0];
-  r.d[1] = x.d[1];
-  return r;
-}
-
 // Waveform quantities at one frequency.  The powers and logarithm of the
 // scaled frequency are computed once and shared by phase, amplitude and tau:
 // IMRPhenomD: x = M G f, tau's v = (pi x)^(1/3) = pi^(1/3) x^(1/3);
 // TaylorF2:   v = (pi M G f)^(1/3).
+// M G (and so x, v and their powers and logarithms) depends on Mc and eta
+// only: it is carried as a D<2> over the first two tangents, and the mixed
+// dual arithmetic leaves the other tangents' zero products out.
 template <int MODEL, int NT>
 GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o) {
   const int smg = is_pd(MODEL) ? (int)PD_MG : (int)TF_MG;
   const int seta = is_pd(MODEL) ? (int)PD_ETA : (int)TF_ETA;
-  const D<NT> MG = ldd<NT>(rec + smg * SLOT);
-  D<NT> v, lv;
+  const D<2> MG = ldd<2>(rec + smg * SLOT);
+  D<2> v, lv;
   if (is_pd(MODEL)) {
     constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
-    const D<NT> x = MG * f;
-    const D<NT> x13 = dcbrt(x);
-    const D<NT> lx = dlog(x);
+    const D<2> x = MG * f;
+    const D<2> x13 = dcbrt(x);
+    const D<2> lx = dlog(x);
     o.A = pd_amp_at<NT, NRT>(rec, SLOT, x, x13, w.apply_fcut != 0 || NRT);
     o.Phi = pd_phi_at<NT, NRT>(rec, SLOT, x, x13, lx, w.apply_fcut != 0);
     if (need_tau) {
@@ -441,7 +435,7 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
     o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
   }
   if (need_tau) {
-    o.tau = tau_star_v(first2(MG), first2(ldd<NT>(rec + seta * SLOT)), first2(v), first2(lv));
+    o.tau = tau_star_v(MG, ldd<2>(rec + seta * SLOT), v, lv);
   } else {
     o.tau = mkc<2>(0.0);
   }

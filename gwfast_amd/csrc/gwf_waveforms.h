@@ -559,16 +559,15 @@ GWF_HD D<N> nrt_px23(const D<N>& x13) {
 }
 
 // tidal phase (arXiv:1905.06011 eq. 7) + 3.5PN spin-squared / spin-cubed terms
-template <int N>
-GWF_HD D<N> nrt_phase_extra(const double* rec, int stride, const D<N>& x, const D<N>& x13) {
-  typedef D<N> T;
-  const T px23 = nrt_px23(x13);
-  const T px = PI * x;
-  const T px43 = px * (x13 * 1.4645918875615231);
-  const T px53 = px * px23;
-  const T num = 1.0 + (-12.615214237993088 * px23) + (19.0537346970349 * px) + (-21.166863146081035 * px43) +
-                (90.55082156324926 * px53) + (-60.25357801943598 * px * px);
-  const T den = 1.0 + (-15.11120782773667 * px23) + (22.195327350624694 * px) + (8.064109635305156 * px43);
+template <int N, class X>
+GWF_HD D<N> nrt_phase_extra(const double* rec, int stride, const X& x, const X& x13) {
+  const auto px23 = nrt_px23(x13);
+  const auto px = PI * x;
+  const auto px43 = px * (x13 * 1.4645918875615231);
+  const auto px53 = px * px23;
+  const auto num = 1.0 + (-12.615214237993088 * px23) + (19.0537346970349 * px) + (-21.166863146081035 * px43) +
+                   (90.55082156324926 * px53) + (-60.25357801943598 * px * px);
+  const auto den = 1.0 + (-15.11120782773667 * px23) + (22.195327350624694 * px) + (8.064109635305156 * px43);
   return ldd<N>(rec + PD_TPH * stride) * px53 * num / den + ldd<N>(rec + PD_SSS * stride) * px23;
 }
 
@@ -608,14 +607,13 @@ GWF_HD void planck_taper(double x, double y, double& tv, double& dv) {
 }
 
 // tidal amplitude (arXiv:1905.06011 eq. 24) without 1/dL
-template <int N>
-GWF_HD D<N> nrt_amp_tidal(const double* rec, int stride, const D<N>& x13) {
-  typedef D<N> T;
-  const T xT = nrt_px23(x13);
-  const T xT2 = xT * xT;
-  const T xT325 = xT2 * xT * dsqrt(dsqrt(xT));
-  const T xT289 = dexp(2.89 * dlog(xT));
-  const T poly = (1.0 + 4.157407407407407 * xT + 2519.111111111111 * xT289) / (1. + 13477.8073677 * (xT2 * xT2));
+template <int N, class X>
+GWF_HD D<N> nrt_amp_tidal(const double* rec, int stride, const X& x13) {
+  const auto xT = nrt_px23(x13);
+  const auto xT2 = xT * xT;
+  const auto xT325 = xT2 * xT * dsqrt(dsqrt(xT));
+  const auto xT289 = dexp(2.89 * dlog(xT));
+  const auto poly = (1.0 + 4.157407407407407 * xT + 2519.111111111111 * xT289) / (1. + 13477.8073677 * (xT2 * xT2));
   return ldd<N>(rec + PD_TAMP * stride) * xT325 * poly;
 }
 
@@ -632,31 +630,33 @@ GWF_HD double nrt_amp_tidal_val(const double* rec, int stride, double x13) {
 // Per-frequency IMRPhenomD phase Phi(f) as a dual (waveforms.py:1139-1147).
 // Only the coefficients of the branch that x falls in are read.  NRT adds
 // the NRTidalv2 terms inside the Mf < 0.2 window (waveforms.py:1613-1616).
-template <int N, bool NRT = false>
-GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, const D<N>& lx, bool apply_fcut) {
+// x, x13, lx: duals over the first tangents only (D<2>, functions of Mc and
+// eta) or D<N>; the coefficients are D<N>.
+template <int N, bool NRT = false, class X = D<N>>
+GWF_HD D<N> pd_phi_at(const double* rec, int stride, const X& x, const X& x13, const X& lx, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
 #define GWF_L(slot) ldd<N>(rec + (slot) * stride)
   const T ieta = GWF_L(PD_IETA);
   T phis;
   if (xv < 0.018) {
-    const T lpx3 = (lx + log(PI)) * (1.0 / 3.0);
-    const T x23 = x13 * x13;
-    const T ix13 = 1.0 / x13;
-    const T ix23 = ix13 * ix13;
-    const T ix = ix13 * ix23;
+    const auto lpx3 = (lx + log(PI)) * (1.0 / 3.0);
+    const auto x23 = x13 * x13;
+    const auto ix13 = 1.0 / x13;
+    const auto ix23 = ix13 * ix13;
+    const auto ix = ix13 * ix23;
     phis = GWF_L(PD_PIP) + GWF_L(PD_P23) * x23 + GWF_L(PD_P13) * x13 + GWF_L(PD_P13L) * x13 * lpx3 +
            GWF_L(PD_PLOG) * lpx3 + GWF_L(PD_PM13) * ix13 + GWF_L(PD_PM23) * ix23 + GWF_L(PD_PM1) * ix +
            GWF_L(PD_PM53) * (ix * ix23) +
            (GWF_L(PD_S1) * x + GWF_L(PD_S43) * (x * x13) + GWF_L(PD_S53) * (x * x23) + GWF_L(PD_S2) * x * x) * ieta;
   } else if (xv < rec[PD_FMRDJ * stride]) {
-    const T ix = 1.0 / x;
+    const auto ix = 1.0 / x;
     phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) * (ix * ix * ix) * (1.0 / 3.0) + GWF_L(PD_B2) * lx) * ieta +
            GWF_L(PD_C1I) + GWF_L(PD_C2I) * x;
   } else if (apply_fcut && !(xv < 0.2)) {
-    phis = 0.0 * x;
+    phis = mkc<N>(0.0);
   } else {
-    const T x14 = dsqrt(dsqrt(x));
+    const auto x14 = dsqrt(dsqrt(x));
     phis = (-(GWF_L(PD_A2) * (1.0 / x)) + (4.0 / 3.0) * (GWF_L(PD_A3) * (x14 * x14 * x14)) + GWF_L(PD_A1) * x +
             GWF_L(PD_A4) * datan((x - GWF_L(PD_A5) * GWF_L(PD_FRING)) / GWF_L(PD_FDAMP))) *
                ieta +
@@ -665,27 +665,27 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const D<N>& x, const D<N>& 
   if (apply_fcut) {
     if (xv < 0.2) {
       if (NRT)
-        return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra(rec, stride, x, x13));
+        return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra<N>(rec, stride, x, x13));
       return phis + (-GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF));
     }
     return phis + 0.0;
   }
-  if (NRT) return phis - GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra(rec, stride, x, x13);
+  if (NRT) return phis - GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF) + nrt_phase_extra<N>(rec, stride, x, x13);
   return phis - GWF_L(PD_T0) * (x - GWF_L(PD_FREF)) - GWF_L(PD_PHIREF);
 #undef GWF_L
 }
 
-template <int N, bool NRT = false>
-GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& x13, bool apply_fcut) {
+template <int N, bool NRT = false, class X = D<N>>
+GWF_HD D<N> pd_amp_at(const double* rec, int stride, const X& x, const X& x13, bool apply_fcut) {
   typedef D<N> T;
   const double xv = x.v;
   T amp;
   if (xv < 0.014) {
-    const T x23 = x13 * x13;
-    const T x43 = x23 * x23;
-    const T x53 = x43 * x13;
-    const T x73 = x53 * x23;
-    const T x83 = x73 * x13;
+    const auto x23 = x13 * x13;
+    const auto x43 = x23 * x23;
+    const auto x53 = x43 * x13;
+    const auto x73 = x53 * x23;
+    const auto x83 = x73 * x13;
     amp = 1. + x23 * ldd<N>(rec + PD_AM23 * stride) + x43 * ldd<N>(rec + PD_AM43 * stride) +
           x53 * ldd<N>(rec + PD_AM53 * stride) + x73 * ldd<N>(rec + PD_AM73 * stride) +
           x83 * ldd<N>(rec + PD_AM83 * stride) +
@@ -694,7 +694,7 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& 
     amp = ldd<N>(rec + PD_DL0 * stride) + x * ldd<N>(rec + PD_DL1 * stride) +
           x * x * (ldd<N>(rec + PD_DL2 * stride) + x * ldd<N>(rec + PD_DL3 * stride) + x * x * ldd<N>(rec + PD_DL4 * stride));
   } else if (apply_fcut && !(xv < 0.2)) {
-    amp = 0.0 * x;
+    amp = mkc<N>(0.0);
   } else {
     const T fring = ldd<N>(rec + PD_FRING * stride);
     const T fdamp = ldd<N>(rec + PD_FDAMP * stride);
@@ -705,10 +705,10 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const D<N>& x, const D<N>& 
     amp = dexp(-(x - fring) * g2 / fdg3) * (fdg3 * g1) / ((x - fring) * (x - fring) + fdg3 * fdg3);
   }
   // x^(-7/6) = 1 / (x sqrt(x^(1/3)))
-  const T xm76 = 1.0 / (x * dsqrt(x13));
+  const auto xm76 = 1.0 / (x * dsqrt(x13));
   if (!NRT) return ldd<N>(rec + PD_OAMP * stride) * xm76 * amp;
   // NRTidalv2: (Overallamp amp0 x^-7/6 A_IMR + 2 sqrt(pi/5) Overallamp A_tidal) * taper
-  const T A = ldd<N>(rec + PD_OAMP * stride) * xm76 * amp + nrt_amp_tidal(rec, stride, x13);
+  const T A = ldd<N>(rec + PD_OAMP * stride) * xm76 * amp + nrt_amp_tidal<N>(rec, stride, x13);
   const T fm = ldd<N>(rec + PD_FMERG * stride);
   double tv, dv;
   planck_taper(xv, fm.v, tv, dv);
@@ -926,16 +926,17 @@ GWF_HD void tf2_setup(const gwf_waveform& w, const D<N>& Mc, const D<N>& eta, co
 }
 
 // Per-frequency TaylorF2 phase.  v is (pi MG f)^(1/3) as a dual.
-template <int N>
-GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v, const D<N>& lv) {
+// v, lv: duals over the first tangents only (D<2>) or D<N>.
+template <int N, class V = D<N>>
+GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const V& v, const V& lv) {
   typedef D<N> T;
-  const T iv = 1.0 / v;
-  const T iv2 = iv * iv;
-  const T iv3 = iv2 * iv;
-  const T iv5 = iv3 * iv2;
-  const T v2 = v * v;
-  const T v5 = v2 * v2 * v;
-  const T v7 = v5 * v2;
+  const auto iv = 1.0 / v;
+  const auto iv2 = iv * iv;
+  const auto iv3 = iv2 * iv;
+  const auto iv5 = iv3 * iv2;
+  const auto v2 = v * v;
+  const auto v5 = v2 * v2 * v;
+  const auto v7 = v5 * v2;
   T ph = ldd<N>(rec + TF_C0 * stride) * iv5 + ldd<N>(rec + TF_C2 * stride) * iv3 + ldd<N>(rec + TF_C3 * stride) * iv2 +
          ldd<N>(rec + TF_C4 * stride) * iv + ldd<N>(rec + TF_C5 * stride) + ldd<N>(rec + TF_C5L * stride) * lv +
          ldd<N>(rec + TF_C6 * stride) * v + ldd<N>(rec + TF_C6L * stride) * (v * lv) + ldd<N>(rec + TF_C7 * stride) * v2 +
@@ -943,8 +944,8 @@ GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const D<N>& v, const D<N>&
   if (tf2_eccentric(N)) {
     // v^(-34/3) (E0 + E2 v^2 + E3 v^3 + E4 v^4 + E5 v^5 + (E6 + E6L ln v) v^6)
     const double p = pow(v.v, -34. / 3.);
-    const T w = chain(v, p, (-34. / 3.) * p / v.v);
-    const T v3 = v2 * v, v4 = v2 * v2, v6 = v3 * v3;
+    const auto w = chain(v, p, (-34. / 3.) * p / v.v);
+    const auto v3 = v2 * v, v4 = v2 * v2, v6 = v3 * v3;
     ph = ph + w * (ldd<N>(rec + TF_E0 * stride) + ldd<N>(rec + TF_E2 * stride) * v2 + ldd<N>(rec + TF_E3 * stride) * v3 +
                    ldd<N>(rec + TF_E4 * stride) * v4 + ldd<N>(rec + TF_E5 * stride) * v5 +
                    (ldd<N>(rec + TF_E6 * stride) + ldd<N>(rec + TF_E6L * stride) * lv) * v6);
