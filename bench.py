@@ -31,10 +31,10 @@ SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
 GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_fisher_pq launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
-# (38,143 KiB, x2.0: a 256 MiB device copy reads 131,087 KiB by that counter)
-# and WRITE_SIZE (10,463 KiB, x1.0) runs of this script; summary in
+# (38,128 KiB, x2.0: a 256 MiB device copy reads 131,087 KiB by that counter)
+# and WRITE_SIZE (10,471 KiB, x1.0) runs of this script; summary in
 # profiles/k_fisher_imrphenomd_net_10k_2000f.md
-K_FISHER_TRAFFIC_10K = (38143.3 * 2.0 + 10462.7) * 1024
+K_FISHER_TRAFFIC_10K = (38127.8 * 2.0 + 10471.0) * 1024
 
 
 def build_net():
@@ -206,11 +206,12 @@ def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, thres
     sc = SC.SNRScreen(seed=7)
     hist = sc.train(rows_t, snr_t, epochs=epochs, batch=4096, lr=2e-3, seed=3)
     del rows_t, snr_t
-    rows, _ = SC.training_set(net, 1000, seed=7, res=res)    # warm-up of both pipelines
-    SC.screened_fisher(net, sc, rows, threshold, band, res)
-    SC.plain_fisher(net, rows, threshold, res)
     from gwfast_amd import catalog, _engine as E
     rows = E.to_dev(E.param_rows(catalog.draw_bbh(n_cat, seed=7), False))
+    # warm-up of both pipelines on the catalogue itself, so that neither timed
+    # run pays for first-time allocations (records workspace, output tensors)
+    SC.screened_fisher(net, sc, rows, threshold, band, res)
+    SC.plain_fisher(net, rows, threshold, res)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     a = SC.screened_fisher(net, sc, rows, threshold, band, res)
