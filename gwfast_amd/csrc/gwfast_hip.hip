@@ -1614,8 +1614,9 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
 }
 
 int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
-                    int64_t ld, int64_t n_ev, int32_t res, double* fisher_out, void* workspace, size_t ws_bytes,
-                    void* stream) {
+                    int64_t ld, int64_t n_ev, int32_t res, const int32_t* res_ev, double* fisher_out,
+                    void* workspace, size_t ws_bytes, void* stream) {
+  if (res_ev) res = 2;  // per-event grid lengths (the records do not depend on res)
   int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
   if (rc) return rc;
   NetK net;
@@ -1628,10 +1629,13 @@ int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_
   hipStream_t st = (hipStream_t)stream;
   const double* rec = (const double*)workspace;
   unsigned int* counter = (unsigned int*)((char*)workspace + (size_t)n_ev * REC_DOUBLES * sizeof(double));
-  rc = net.lin ? launch_fisher_pq<true, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
-                                                         7 + nt_of(wf))
-               : launch_fisher_pq<false, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out,
-                                                          7 + nt_of(wf));
+  const int P = 7 + nt_of(wf);
+  if (res_ev)
+    rc = net.lin ? launch_fisher_pq<true, true, PQ_ARMS>(wf, st, net, rec, n_ev, res_ev, counter, fisher_out, P)
+                 : launch_fisher_pq<false, true, PQ_ARMS>(wf, st, net, rec, n_ev, res_ev, counter, fisher_out, P);
+  else
+    rc = net.lin ? launch_fisher_pq<true, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out, P)
+                 : launch_fisher_pq<false, false, PQ_ARMS>(wf, st, net, rec, n_ev, nullptr, counter, fisher_out, P);
   if (rc) return rc;
   return hip_check(hipGetLastError(), "gwf_fisher_arms launch");
 }

@@ -134,7 +134,7 @@ class DetNet(object):
                 r = S.resolve_res(sigs[0], evParams, res, df)
                 rows = sigs[0]._rows(evParams)
                 Fa = None
-                if np.ndim(r) == 0 and sum(s.n_arms for s in sigs) <= S.ARMS_PER_LAUNCH:
+                if sum(s.n_arms for s in sigs) <= S.ARMS_PER_LAUNCH:
                     Fa = S.fisher_arms_device(sigs, sigs[0].wf_model, rows, r, lin=lin)
                 a = 0
                 for name, sig in g:

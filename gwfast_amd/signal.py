@@ -168,8 +168,8 @@ class GWSignal(object):
         """FisherMatr(return_all=True): the list of this detector's per-arm
         matrices (host), each times its duty-cycle mask.  Fa: this detector's
         (n_arms, P, P, N) slice of a gwf_fisher_arms launch, else one launch
-        here (uniform grids) or one per arm (ragged grids)."""
-        if Fa is None and np.ndim(res) == 0 and self.n_arms <= ARMS_PER_LAUNCH:
+        here."""
+        if Fa is None and self.n_arms <= ARMS_PER_LAUNCH:
             Fa = fisher_arms_device([self], self.wf_model, rows, res, lin=lin)
         Fs = []
         for k in range(self.n_arms):
@@ -447,15 +447,16 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None, lin=False):
 def fisher_arms_device(signals, wf, rows, res, lin=False):
     """The Fisher matrix of every physical arm of signals (in network arm
     order) from one gwf_fisher_arms launch: (n_arms, P, P, N_ev) device
-    tensor; res: one grid length for all events."""
+    tensor; res: int, or one grid length per event (ragged)."""
     t = E.torch()
     n = rows.shape[1]
     P = wf.nParams
     n_arms = sum(s.n_arms for s in signals)
     out = t.empty((n_arms, P, P, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    E.call('gwf_fisher_arms', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n,
-           int(res), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
+    r, r_ev = _res_arg(res, n, rows.device)
+    E.call('gwf_fisher_arms', wf.c_struct(), _dets(signals, lin), len(signals), N.ptr(rows), rows.stride(0), n, r,
+           N.ptr(r_ev), N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
 
 

@@ -148,10 +148,11 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
  * tangents are evaluated once per frequency for all arms).  fisher_out:
  * (n_arms, P, P, n_ev), arm a at fisher_out + a * P * P * n_ev, arms in the
  * order of gwf_num_arms (detectors in order; a triangle's three arms, the
- * third -(first + second)).  At most 8 arms. */
+ * third -(first + second)).  At most 8 arms.  res_ev: NULL for one grid
+ * length res, else per-event lengths as in gwf_fisher_ragged (res unused). */
 int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
-                    int64_t ld, int64_t n_ev, int32_t res, double* fisher_out, void* workspace, size_t ws_bytes,
-                    void* stream);
+                    int64_t ld, int64_t n_ev, int32_t res, const int32_t* res_ev, double* fisher_out,
+                    void* workspace, size_t ws_bytes, void* stream);
 
 /* Ragged per-event grids (no reference counterpart: gwfast uses one res for
  * all events, signal.py:890-898): event e uses jnp.geomspace(fmin, fcut_e,

@@ -424,3 +424,30 @@ def test_fisher_all_arms_one_launch(gpu, model):
     odets = oracle_net(NET_C1[1:2], True)
     Fo = OD.network_fisher(odets, owf, ev, res, mode='dual')
     assert norm_err(allF['CE1Id'], Fo) <= 1e-10
+
+
+def test_fisher_all_arms_ragged(gpu):
+    """gwf_fisher_arms with per-event grid lengths (the event queue of the
+    pair kernel): every event's per-arm matrices equal a uniform launch at
+    that event's length (the two are separately compiled kernels: equal to
+    rounding, measured 3e-15); DetNet's return_all takes the same path."""
+    from gwfast_amd import signal as S
+    from gwfast_amd.waveforms import IMRPhenomD
+    wf = IMRPhenomD()
+    n = 30
+    ev = bbh(n, seed=67)
+    lens = np.array([64, 65, 300, 777, 1000], dtype=np.int32)
+    res = lens[np.arange(n) % len(lens)]
+    net = hip_net(wf, NET_C1, True)
+    sigs = list(net.signals.values())
+    sigs[0]._complete(ev)
+    rows = sigs[0]._rows(ev)
+    Fa = S.fisher_arms_device(sigs, wf, rows, res).cpu().numpy()
+    for r in lens:
+        sel = np.where(res == r)[0]
+        Fu = S.fisher_arms_device(sigs, wf, rows[:, sel].contiguous(), int(r)).cpu().numpy()
+        for k in range(5):
+            assert norm_err(Fa[k][:, :, sel], Fu[k]) <= 1e-13, (r, k)
+    allF = net.FisherMatr(dict(ev), res=res, return_all=True)
+    for k, key in enumerate(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM']):
+        assert norm_err(allF[key], Fa[k]) <= 1e-15
