@@ -53,6 +53,21 @@ def test_library_binds_without_gpu(so_path):
     assert rc == -22 and b'P must be' in L.gwf_last_error()
 
 
+def test_ctypes_signatures_match_header_arity():
+    """Every entry point declared in include/gwfast_hip.h has a ctypes
+    binding with the same number of arguments (the Python layer calls the
+    library through these bindings only)."""
+    hdr = open(HEADER).read()
+    hdr = re.sub(r'/\*.*?\*/', '', hdr, flags=re.S)
+    decls = re.findall(r'\b(?:int|size_t|const char\*)\s+(gwf_\w+)\s*\(([^;]*?)\)\s*;', hdr, flags=re.S)
+    assert len(decls) >= 40
+    for name, args in decls:
+        n = 0 if args.strip() in ('', 'void') else len(args.split(','))
+        assert name in N.SIGNATURES, name
+        assert len(N.SIGNATURES[name][1]) == n, name
+    assert set(N.SIGNATURES) == {d[0] for d in decls}
+
+
 def test_struct_layouts_match_header():
     # field order and sizes of the ABI structs
     assert ctypes.sizeof(N.GwfWaveform) == 8 * 4 + 2 * 8 + 3 * 8 + 2 * 4 + 8
