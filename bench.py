@@ -266,8 +266,8 @@ def main():
                     help='first run a device copy of this many MiB (a known-size kernel for PMC byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path (events shard with no '
-              'exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
+        print('bench.py measures one GPU only: there is no multi-GPU path (events would shard across GPUs '
+              'with no exchange during compute). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
 
     import torch
