@@ -87,6 +87,7 @@ SIGNATURES = {
                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     'gwf_fisher_masked': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,
                                          c_int32, c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'gwf_set_queue_spin_limit': (ctypes.c_int, [ctypes.c_uint32]),
     'gwf_fisher_arms': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64, c_int32,
                                        c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     'gwf_fisher_block_transform': (ctypes.c_int, [c_void_p, c_void_p, c_int32, c_int64, c_int32, c_int32, c_void_p,

@@ -22,6 +22,7 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
   net.fmin = dets[0].fmin;
   net.fmax = dets[0].fmax;
   net.lin = dets[0].lin_spacing != 0;
+  net.spin_limit = 1u << 22;  // the producer / consumer launcher sets its own
   int na = 0, np = 0;
   for (int d = 0; d < n_det; ++d) {
     const gwf_detector& g = dets[d];

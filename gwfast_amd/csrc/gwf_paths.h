@@ -73,6 +73,7 @@ struct NetK {
   int n_sites, n_arms, res, any_earth;
   int n_sum_arms, n_psd;  // n_psd: distinct PSD tables
   int lin;                // Fisher grids: linspace instead of geomspace
+  unsigned spin_limit;    // polls per wait of the producer / consumer Fisher kernel
   double fmin, fmax;
   SiteK site[GWF_MAX_DETECTORS];
   ArmK arm[GWF_MAX_ARMS];

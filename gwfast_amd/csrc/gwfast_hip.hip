@@ -131,14 +131,15 @@ constexpr int PC_EPB = GWF_PC_EPB;
 #endif
 constexpr int PQ_NBUF = GWF_PQ_NBUF;
 constexpr unsigned PQ_SPIN_LIMIT = 1u << 22;  // >= 0.1 s of polling; a chunk takes ~10 us
+unsigned g_pq_spin_limit = PQ_SPIN_LIMIT;      // gwf_set_queue_spin_limit
 
 __device__ __forceinline__ void pq_release(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Waits (wave-uniformly) until *p >= target; false when the poll limit ran out.
-__device__ __forceinline__ bool pq_wait(int* p, int target) {
-  for (unsigned n = 0; n < PQ_SPIN_LIMIT; ++n) {
+// Waits (wave-uniformly) until *p >= target; false after limit polls.
+__device__ __forceinline__ bool pq_wait(int* p, int target, unsigned limit) {
+  for (unsigned n = 0; n < limit; ++n) {
     const int v = __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (__builtin_amdgcn_readfirstlane(v) >= target) return true;
     __builtin_amdgcn_s_sleep(1);
@@ -318,10 +319,11 @@ __device__ __forceinline__ void store_mx(const v4d* acc, double* __restrict__ F,
 // e, runs done(e); returns at the end marker or when a wait expires.
 template <int NB, int CH, class Item, class Done>
 __device__ __forceinline__ void pq_consumer(int* produced, int* consumed, int (*hd)[4], double (*bf)[CH],
-                                            unsigned int* counter, int lane, Item&& item, Done&& done) {
+                                            unsigned int* counter, unsigned limit, int lane, Item&& item,
+                                            Done&& done) {
   for (int k = 0;; ++k) {
     const int s = k % NB;
-    if (!pq_wait(produced, k + 1)) {
+    if (!pq_wait(produced, k + 1, limit)) {
       pq_fail(counter, lane);
       return;
     }
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       const int s0 = k % NB;
       if (e >= n_ev) {  // end marker
-        if (!pq_wait(&cnt[slot][1], k - NB + 1)) {
+        if (!pq_wait(&cnt[slot][1], k - NB + 1, net.spin_limit)) {
           pq_fail(counter, lane);
           return;
         }
@@ -384,7 +386,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       gw.init(a, b, res, lane, 64);
       for (int t = 0; t < nch; ++t, ++k) {
         const int s = k % NB;
-        if (!pq_wait(&cnt[slot][1], k - NB + 1)) {  // the buffer's previous item is consumed
+        if (!pq_wait(&cnt[slot][1], k - NB + 1, net.spin_limit)) {  // the buffer's previous item is consumed
           pq_fail(counter, lane);
           return;
         }
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
     constexpr int NA = ARMS ? PQ_ARMS_MAX : PQ_NACC;        // ARMS: one accumulator per physical arm
     v4d acc[NA];
     uint32_t mask = 0u;
-    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, lane,
+    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
                     [&](int e, int t, int res, const double* chunk) {
                       if (t == 0) {
 #pragma unroll
@@ -427,7 +429,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
     constexpr int NG = P * (P + 1) / 2;
     double G[NG];
     uint32_t mask = 0u;
-    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, lane,
+    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
                     [&](int e, int t, int res, const double* chunk) {
                       if (t == 0) {
 #pragma unroll
@@ -1328,7 +1330,7 @@ void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
 // resident block per CU, then the NaN fill that reports an expired wait.
 // counter: two words, zeroed here.
 template <bool LIN, bool RAGGED, int MODE = PQ_SUM>
-int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, const double* rec, int64_t n_ev,
+int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net_in, const double* rec, int64_t n_ev,
                      const int32_t* res_ev, unsigned int* counter, double* out, int P,
                      const uint32_t* arm_mask = nullptr) {
   int dev = 0, n_cu = 0;
@@ -1338,6 +1340,8 @@ int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net, co
   if (rc) return rc;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, counter, 2);
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_ev + PC_EPB - 1) / PC_EPB, n_cu));
+  NetK net = net_in;
+  net.spin_limit = g_pq_spin_limit;
   const dim3 g((unsigned)blocks), b(128 * PC_EPB);
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_IMRPHENOMD, 4, 11, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
@@ -1611,6 +1615,11 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
   else
     launch_fisher_masked<false>(wf, grid, block, st, net, rec, n_ev, arm_mask, fisher_out);
   return hip_check(hipGetLastError(), "gwf_fisher_masked launch");
+}
+
+int gwf_set_queue_spin_limit(uint32_t polls) {
+  g_pq_spin_limit = polls ? polls : PQ_SPIN_LIMIT;
+  return 0;
 }
 
 int gwf_fisher_arms(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,

@@ -141,6 +141,13 @@ int gwf_fisher_masked(const gwf_waveform* wf, const gwf_detector* dets, int32_t 
                       int64_t ld, int64_t n_ev, int32_t res, const uint32_t* arm_mask, double* fisher_out,
                       void* workspace, size_t ws_bytes, void* stream);
 
+/* No reference counterpart; for tests of a failure path.  Each wait of the
+ * producer / consumer Fisher kernels (gwf_fisher*, network sums and
+ * gwf_fisher_arms) polls at most this many times (0 restores the default,
+ * 2^22 polls, >= 0.1 s); a wait that runs out ends its wave and the launch
+ * writes NaN over its whole output instead of hanging.  Process-wide. */
+int gwf_set_queue_spin_limit(uint32_t polls);
+
 /* GWSignal.FisherMatr / DetNet.FisherMatr with return_all=True
  * (signal.py:1087-1098, network.py:125-135; the catalogue driver always asks
  * for it, calculate_forecasts_from_catalog.py:507-513): the Fisher matrix of

@@ -451,3 +451,29 @@ def test_fisher_all_arms_ragged(gpu):
     allF = net.FisherMatr(dict(ev), res=res, return_all=True)
     for k, key in enumerate(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM']):
         assert norm_err(allF[key], Fa[k]) <= 1e-15
+
+
+def test_pair_kernel_expired_wait_gives_nan(gpu):
+    """A wait of the producer / consumer kernel that runs out of polls ends
+    its wave and the launch reports NaN (forced from the host with a limit of
+    one poll, which the consumer's first wait cannot meet); the default limit
+    then gives the normal result again."""
+    from gwfast_amd import _native as N
+    from gwfast_amd import signal as S
+    from gwfast_amd.waveforms import IMRPhenomD
+    wf = IMRPhenomD()
+    ev = bbh(16, seed=71)
+    net = hip_net(wf, NET_C1, True)
+    sigs = list(net.signals.values())
+    sigs[0]._complete(ev)
+    rows = sigs[0]._rows(ev)
+    good = S.fisher_device(sigs, wf, rows, 800).cpu().numpy()
+    L = N.lib()
+    try:
+        N.check(L.gwf_set_queue_spin_limit(1), L)
+        bad = S.fisher_device(sigs, wf, rows, 800).cpu().numpy()
+    finally:
+        N.check(L.gwf_set_queue_spin_limit(0), L)
+    assert np.all(np.isnan(bad))
+    again = S.fisher_device(sigs, wf, rows, 800).cpu().numpy()
+    assert np.array_equal(again, good)
