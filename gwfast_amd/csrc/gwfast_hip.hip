@@ -123,7 +123,8 @@ constexpr int PC_EPB = GWF_PC_EPB;
 // one block never wait for each other: RAGGED (per-event grid lengths res_ev)
 // takes events from the device counter counter[0]; otherwise pair p of the
 // grid takes events p, p + n_pairs, ...  Every wait polls at most
-// PQ_SPIN_LIMIT times (with s_sleep); a wave whose wait runs out sets
+// net.spin_limit times (with s_sleep; PQ_SPIN_LIMIT unless
+// gwf_set_queue_spin_limit changed it); a wave whose wait runs out sets
 // counter[1] and leaves, and k_pq_poison then fills the whole output with
 // NaN, so the launch always ends and a failure cannot go unnoticed.
 #ifndef GWF_PQ_NBUF
