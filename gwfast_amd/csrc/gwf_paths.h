@@ -247,9 +247,14 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
     pd_setup<NT>(Mcs, etas, c1s, c2s, f0u, w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT,
                  QM1, QM2);
     nrt_setup<NT>(Mcs, etas, c1s, c2s, L1, L2, QM1, QM2, rec, SLOT);
-  } else if (tf2_eccentric(NT)) {  // eccentric TaylorF2: row 11 is the eccentricity (ParNums['ecc'] = 11)
-    tf2_setup<NT>(w, mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3), mkc<NT>(0.0),
-                  mkc<NT>(0.0), rec, SLOT, mkseed<NT>(par[11 * ld + e], 4), f0);
+  } else if (tf2_eccentric(NT)) {  // eccentric TaylorF2: ParNums['ecc'] = 11, or 13 after the tidal rows
+    if (NT == 7)
+      tf2_setup<NT>(w, mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3),
+                    mkseed<NT>(par[11 * ld + e], 4), mkseed<NT>(par[12 * ld + e], 5), rec, SLOT,
+                    mkseed<NT>(par[13 * ld + e], 6), f0);
+    else
+      tf2_setup<NT>(w, mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3), mkc<NT>(0.0),
+                    mkc<NT>(0.0), rec, SLOT, mkseed<NT>(par[11 * ld + e], 4), f0);
   } else {
     const double Lt = w.is_tidal ? par[11 * ld + e] : 0.0;
     const double dLm = w.is_tidal ? par[12 * ld + e] : 0.0;
@@ -552,6 +557,10 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   if (P > 12) {
     const double dA5 = wp.A.d[NT > 5 ? 5 : 0] * idL;
     GWF_SET(12, dA5 * Fp * cp, dA5 * Fc * cc, -wp.Phi.d[NT > 5 ? 5 : 0]);
+  }
+  if (P > 13) {  // eccentricity after the tidal rows
+    const double dA6 = wp.A.d[NT > 6 ? 6 : 0] * idL;
+    GWF_SET(13, dA6 * Fp * cp, dA6 * Fc * cc, -wp.Phi.d[NT > 6 ? 6 : 0]);
   }
 #undef GWF_SET
 }

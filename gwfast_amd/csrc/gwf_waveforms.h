@@ -57,7 +57,7 @@ enum TfSlot {
   TF_COUNT
 };
 constexpr int REC_SLOTS = PD_COUNT;  // >= TF_COUNT
-constexpr int NT_MAX = 6;
+constexpr int NT_MAX = 7;
 constexpr int REC_STRIDE = REC_SLOTS * (NT_MAX + 1);  // doubles per event record
 
 // IMRPhenomD phenomenological fits: (a0, a1, b0, b1, b2, c0, c1, c2, d0, d1, d2)
@@ -821,7 +821,7 @@ GWF_HD double tf2_fcut(const gwf_waveform& w, double Mc, double eta, double chi1
 // 3/(128 eta)), so Phi(f) = sum c_k b_k(v) + phiR - pi/4.
 // Eccentric TaylorF2 is the TaylorF2 instantiation with N = 5 tangents
 // (Mc, eta, chi1z, chi2z, ecc); the tidal one has N = 6.
-constexpr bool tf2_eccentric(int N) { return N == 5; }
+constexpr bool tf2_eccentric(int N) { return N == 5 || N == 7; }  // 7: tidal and eccentric
 
 template <int N>
 GWF_HD void tf2_setup(const gwf_waveform& w, const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2,

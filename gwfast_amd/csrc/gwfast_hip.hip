@@ -1242,15 +1242,15 @@ int check_wf(const gwf_waveform* wf) {
   if (is_pd(wf->model) && (!wf->qnm_a || !wf->qnm_fring || !wf->qnm_fdamp || wf->n_qnm < 2))
     return fail(-EINVAL, "IMRPhenomD needs the QNM tables");
   if (wf->model == GWF_MODEL_IMRPHENOMD && wf->is_tidal) return fail(-EINVAL, "IMRPhenomD is not tidal");
-  if (wf->is_eccentric && (wf->model != GWF_MODEL_TAYLORF2 || wf->is_tidal))
-    return fail(-EINVAL, "eccentricity is supported for non-tidal TaylorF2 only");
+  if (wf->is_eccentric && wf->model != GWF_MODEL_TAYLORF2)
+    return fail(-EINVAL, "eccentricity is supported for TaylorF2 only");
   if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2 && !wf->is_tidal)
     return fail(-EINVAL, "IMRPhenomD_NRTidalv2 is tidal (13 parameters)");
   return 0;
 }
 
 int nt_of(const gwf_waveform* wf) {
-  if (wf->model == GWF_MODEL_TAYLORF2 && wf->is_eccentric) return 5;  // eccentric TaylorF2, 12 parameters
+  if (wf->model == GWF_MODEL_TAYLORF2 && wf->is_eccentric) return wf->is_tidal ? 7 : 5;  // 14 / 12 parameters
   return ((wf->model == GWF_MODEL_TAYLORF2 && wf->is_tidal) || wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2) ? 6 : 4;
 }
 
@@ -1280,6 +1280,8 @@ void launch_fisher(const gwf_waveform* wf, dim3 grid, dim3 block, hipStream_t st
                        arm, out);
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 7, 14, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev, arm, out);
   else
@@ -1297,6 +1299,9 @@ void launch_fisher_masked(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
                        rec, n_ev, mask, out);
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       mask, out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 7, 14, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
                        mask, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_masked<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
@@ -1318,6 +1323,9 @@ void launch_fisher_ragged(const gwf_waveform* wf, dim3 grid, dim3 block, hipStre
                        rec, n_ev, res_ev, arm, counter, out);
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 5, 12, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
+                       res_ev, arm, counter, out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 7, 14, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
                        res_ev, arm, counter, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_ragged<GWF_MODEL_TAYLORF2, 6, 13, LIN>), grid, block, 0, st, *wf, net, rec, n_ev,
@@ -1353,6 +1361,9 @@ int launch_fisher_pq(const gwf_waveform* wf, hipStream_t st, const NetK& net_in,
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 5, 12, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 7, 14, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
+                       n_ev, res_ev, arm_mask, counter, out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_fisher_pq<GWF_MODEL_TAYLORF2, 6, 13, LIN, RAGGED, MODE>), g, b, 0, st, *wf, net, rec,
                        n_ev, res_ev, arm_mask, counter, out);
@@ -1375,6 +1386,8 @@ int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int
                        nf, rec);
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 5>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 7>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else
@@ -1500,6 +1513,9 @@ int gwf_waveform_eval(const gwf_waveform* wf, const double* params, int64_t ld, 
                        ampl_out, phi_out, tau_out);
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 5>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
+                       tau_out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 7>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
                        tau_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_wfeval<GWF_MODEL_TAYLORF2, 6>), grid, block, 0, st, *wf, rec, n_ev, f, n_f, ampl_out, phi_out,
@@ -1734,6 +1750,9 @@ static int strain_impl(const gwf_waveform* wf, const gwf_detector* det, int32_t 
   else if (nt_of(wf) == 5)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 5, 12>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
                        h_out, J_out, comp_out);
+  else if (nt_of(wf) == 7)  // tidal and eccentric TaylorF2, 14 parameters
+    hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 7, 14>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
+                       h_out, J_out, comp_out);
   else if (nt_of(wf) == 6)
     hipLaunchKernelGGL((k_strain<GWF_MODEL_TAYLORF2, 6, 13>), grid, block, 0, st, *wf, net, rec, n_ev, (int)arm, f, n_f,
                        h_out, J_out, comp_out);
@@ -1770,7 +1789,7 @@ int gwf_strain_vjp(const double* J, const double* gh, int32_t P, int64_t n_ev, i
 
 int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
              int32_t accumulate, double* fisher_out, void* stream) {
-  if (P < 11 || P > 13) return fail(-EINVAL, "gwf_gram supports P = 11, 12 or 13");
+  if (P < 11 || P > 14) return fail(-EINVAL, "gwf_gram supports P = 11 to 14");
   if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
   if (n_ev == 0) return 0;
   if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
@@ -1781,14 +1800,16 @@ int gwf_gram(const double* J, const double* f, const double* S, int32_t P, int64
     hipLaunchKernelGGL((k_gram<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   else if (P == 12)
     hipLaunchKernelGGL((k_gram<12>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
-  else
+  else if (P == 13)
     hipLaunchKernelGGL((k_gram<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else
+    hipLaunchKernelGGL((k_gram<14>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   return hip_check(hipGetLastError(), "gwf_gram launch");
 }
 
 int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P, int64_t n_ev, int64_t n_f,
                    int32_t accumulate, double* fisher_out, void* stream) {
-  if (P < 11 || P > 13) return fail(-EINVAL, "gwf_gram_split supports P = 11, 12 or 13");
+  if (P < 11 || P > 14) return fail(-EINVAL, "gwf_gram_split supports P = 11 to 14");
   if (n_ev < 0 || n_f < 0) return fail(-EINVAL, "negative sizes");
   if (n_ev == 0) return 0;
   if (!J || !f || !S || !fisher_out) return fail(-EINVAL, "NULL array");
@@ -1799,8 +1820,10 @@ int gwf_gram_split(const double* J, const double* f, const double* S, int32_t P,
     hipLaunchKernelGGL((k_gram_split<11>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   else if (P == 12)
     hipLaunchKernelGGL((k_gram_split<12>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
-  else
+  else if (P == 13)
     hipLaunchKernelGGL((k_gram_split<13>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
+  else
+    hipLaunchKernelGGL((k_gram_split<14>), grid, block, 0, st, J, f, S, n_ev, n_f, (int)accumulate, fisher_out);
   return hip_check(hipGetLastError(), "gwf_gram_split launch");
 }
 
@@ -1898,7 +1921,7 @@ int gwf_fisher_block_transform(const double* F, double* out, int32_t P, int64_t 
 
 int gwf_fisher_reparam(const double* params, int64_t ld, double* F, int32_t P, int64_t n_ev, int32_t use_m1m2,
                        int32_t use_chiSA, void* stream) {
-  if (P < 11 || P > 13) return fail(-EINVAL, "P must be 11, 12 or 13");
+  if (P < 11 || P > 14) return fail(-EINVAL, "P must be 11 to 14");
   if (n_ev < 0 || ld < n_ev) return fail(-EINVAL, "invalid sizes");
   if (n_ev == 0 || (!use_m1m2 && !use_chiSA)) return 0;
   if (!params || !F) return fail(-EINVAL, "NULL array");

@@ -19,15 +19,13 @@ from . import gwfastGlobals as glob
 
 class WaveFormModel(ABC):
     """gwfast/waveforms.py:49-199 (aligned-spin subset: no precession, no
-    higher modes; eccentricity for TaylorF2 without tidal terms)."""
+    higher modes; eccentricity for TaylorF2)."""
 
     def __init__(self, objType, fcutPar, is_newtonian=False, is_tidal=False, is_HigherModes=False,
                  is_chi1chi2=True, is_Precessing=False, is_LAL=False, is_prec_ang=False, is_eccentric=False,
                  is_holomorphic=False, apply_fcut=True):
         if is_newtonian or is_HigherModes or is_Precessing or is_LAL:
             raise NotImplementedError('only aligned-spin, dominant-mode models are on this path')
-        if is_eccentric and is_tidal:
-            raise NotImplementedError('eccentricity is supported without tidal terms (12 parameters)')
         self.objType = objType
         self.fcutPar = fcutPar
         self.is_newtonian = is_newtonian
@@ -48,9 +46,9 @@ class WaveFormModel(ABC):
             self.ParNums['LambdaTilde'] = 11
             self.ParNums['deltaLambda'] = 12
             self.nParams = 13
-        if is_eccentric:
-            self.ParNums['ecc'] = 11
-            self.nParams = 12
+        if is_eccentric:  # after the tidal rows when both (waveforms.py:106-115, 124-126)
+            self.ParNums['ecc'] = self.nParams
+            self.nParams += 1
         if is_chi1chi2:
             self.ParNums['chi1z'] = self.ParNums.pop('chiS')
             self.ParNums['chi2z'] = self.ParNums.pop('chiA')
@@ -103,7 +101,7 @@ class WaveFormModel(ABC):
 
     def _tidal_rows(self, ev):
         ev = {k: np.atleast_1d(v) for k, v in ev.items()}
-        return E.param_rows(ev, True)
+        return E.param_rows(ev, True, self.is_eccentric)
 
     def Phi(self, f, **kwargs):
         """GW phase on the grid f (waveforms.py:148-160)."""
@@ -139,7 +137,7 @@ class WaveFormModel(ABC):
 class TaylorF2_RestrictedPN(WaveFormModel):
     """TaylorF2 3.5PN restricted waveform (waveforms.py:697-953), optionally
     with 5 and 6PN tidal terms or with the low-eccentricity 3PN phase
-    (is_eccentric, fRef_ecc; waveforms.py:814-845), not both."""
+    (is_eccentric, fRef_ecc; waveforms.py:814-845), or both (14 parameters)."""
 
     def __init__(self, fHigh=None, is_tidal=False, use_3p5PN_SpinHO=False, phiref_vlso=False, is_eccentric=False,
                  fRef_ecc=None, which_ISCO='Schw', use_QuadMonTid=False, **kwargs):

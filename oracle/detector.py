@@ -162,11 +162,11 @@ class Detector:
         Returns complex array (n_AD, N_ev, N_f)."""
         tidal = wf.is_tidal
         ecc = getattr(wf, 'is_eccentric', False)
-        K = 6 if tidal else (5 if ecc else 4)
+        K = 4 + (2 if tidal else 0) + (1 if ecc else 0)
         evd = dict(ev)
         _seed_masses_spins(evd, ev, K, use_m1m2, use_chi1chi2)
-        if ecc:  # AD argument list [Mc, eta, chiS, chiA, ecc] (signal.py:1229-1233)
-            evd['ecc'] = D.Dual.seed(ev['ecc'], 4, K)
+        if ecc:  # AD argument list [Mc, eta, chiS, chiA, (LambdaTilde, deltaLambda,) ecc] (signal.py:1229-1233)
+            evd['ecc'] = D.Dual.seed(ev['ecc'], K - 1, K)
         if tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             Ltd = D.Dual.seed(Lt, 4, K)
@@ -299,8 +299,8 @@ class Detector:
         for k, key in enumerate(keys):
             evd[key] = D.Dual.seed(ev[key], k, P)
         _seed_masses_spins(evd, ev, P, use_m1m2, use_chi1chi2, slots=(0, 1, 9, 10))
-        if getattr(wf, 'is_eccentric', False):
-            evd['ecc'] = D.Dual.seed(ev['ecc'], 11, P)
+        if getattr(wf, 'is_eccentric', False):  # ParNums['ecc']: the last row
+            evd['ecc'] = D.Dual.seed(ev['ecc'], P - 1, P)
         if wf.is_tidal:
             Lt, dLm = lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
             L1, L2 = lam12_from_lamt_dellam(D.Dual.seed(Lt, 11, P), D.Dual.seed(dLm, 12, P), evd['eta'])

@@ -166,7 +166,7 @@ class TaylorF2:
         self.use_QuadMonTid = use_QuadMonTid
         self.is_holomorphic = True
         self.objType = 'BNS' if is_tidal else 'BBH'
-        self.nParams = 13 if is_tidal else (12 if is_eccentric else 11)
+        self.nParams = 11 + (2 if is_tidal else 0) + (1 if is_eccentric else 0)
 
     def phi_ecc(self, v, Mtot_sec, eta, ecc):
         """waveforms.py:814-845: the low-eccentricity phase to 3PN

@@ -87,8 +87,9 @@ def test_parnums_and_nparams():
     assert len(d.QNMgrid_a) == 1003
     e = TaylorF2_RestrictedPN(is_eccentric=True)
     assert e.nParams == 12 and e.ParNums['ecc'] == 11
-    with pytest.raises(NotImplementedError):
-        TaylorF2_RestrictedPN(is_eccentric=True, is_tidal=True)
+    te = TaylorF2_RestrictedPN(is_eccentric=True, is_tidal=True)   # waveforms.py:106-115
+    assert te.nParams == 14 and te.ParNums['LambdaTilde'] == 11 and te.ParNums['deltaLambda'] == 12
+    assert te.ParNums['ecc'] == 13 and list(te.ParNums.values()) == list(range(14))
 
 
 def test_gwsignal_validation():

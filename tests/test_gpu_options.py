@@ -477,3 +477,36 @@ def test_pair_kernel_expired_wait_gives_nan(gpu):
     assert np.all(np.isnan(bad))
     again = S.fisher_device(sigs, wf, rows, 800).cpu().numpy()
     assert np.array_equal(again, good)
+
+
+def test_tidal_eccentric_taylorf2(gpu):
+    """TaylorF2_RestrictedPN(is_tidal=True, is_eccentric=True): 14 parameters
+    (ParNums LambdaTilde 11, deltaLambda 12, ecc 13; waveforms.py:106-115):
+    phase, SNR and network Fisher against the oracle; e0 = 0 gives the tidal
+    Fisher on the first 13 rows."""
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    from oracle import detector as OD
+    from oracle import waveforms as OW
+    ev = bns(6, seed=99)
+    ev['ecc'] = np.linspace(0.01, 0.15, 6)
+    wf = TaylorF2_RestrictedPN(is_tidal=True, is_eccentric=True)
+    owf = OW.TaylorF2(is_tidal=True, is_eccentric=True)
+    assert wf.nParams == 14 and wf.ParNums['ecc'] == 13
+    f = np.geomspace(2.0, 0.9 * owf.fcut(ev), 500)
+    ph = wf.Phi(f, **ev)
+    pho = owf.Phi(f, ev)
+    assert np.max(np.abs(ph - pho)) <= 1e-13 * np.max(np.abs(pho)) + 1e-9
+    net = hip_net(wf, NET_C1, True)
+    odets = oracle_net(NET_C1, True)
+    snr = net.SNR(dict(ev), res=1000)
+    assert np.max(np.abs(snr / OD.network_snr(odets, owf, ev, 1000) - 1)) <= 1e-11
+    F = net.FisherMatr(dict(ev), res=1000)
+    assert F.shape == (14, 14, 6)
+    e = norm_err(F, OD.network_fisher(odets, owf, ev, 1000, mode='dual'))
+    print('tidal+eccentric TF2 Fisher err %.3e' % e, flush=True)
+    assert e <= 1e-10
+    ev0 = dict(ev)
+    ev0['ecc'] = np.zeros(6)
+    F0 = net.FisherMatr(dict(ev0), res=1000)
+    Ft = hip_net(TaylorF2_RestrictedPN(is_tidal=True), NET_C1, True).FisherMatr(dict(ev0), res=1000)
+    assert norm_err(F0[:13, :13], Ft) <= 1e-13

@@ -829,3 +829,20 @@ def test_eccentric_tf2_dual_vs_complex_step():
         evc[key] = ev[key] + 1j * h
         dph = np.imag(wf.Phi(f.astype(complex), evc)) / h
         assert np.max(np.abs(ph.d[k] - dph)) <= 1e-10 * np.max(np.abs(dph)), key
+
+
+def test_tidal_eccentric_tf2_is_additive():
+    """waveforms.py:814-862: the eccentric and the tidal phase terms enter
+    the TaylorF2 phase as independent additions, so with both on the phase is
+    Phi_tidal + (Phi_ecc - Phi_circ); the 14-parameter Fisher rows follow
+    ParNums (LambdaTilde 11, deltaLambda 12, ecc 13)."""
+    from gwfast_amd import catalog
+    ev = catalog.draw_bns(4, seed=21)
+    ev['ecc'] = np.array([0.01, 0.05, 0.1, 0.2])
+    f = np.geomspace(2.0, 0.9 * OW.TaylorF2(is_tidal=True).fcut(ev), 300)
+    both = OW.TaylorF2(is_tidal=True, is_eccentric=True).Phi(f, ev)
+    tid = OW.TaylorF2(is_tidal=True).Phi(f, ev)
+    ecc = OW.TaylorF2(is_eccentric=True).Phi(f, ev)
+    circ = OW.TaylorF2().Phi(f, ev)
+    assert np.max(np.abs(both - (tid + ecc - circ))) <= 1e-12 * np.max(np.abs(both))
+    assert OW.TaylorF2(is_tidal=True, is_eccentric=True).nParams == 14
