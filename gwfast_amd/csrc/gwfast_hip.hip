@@ -128,9 +128,10 @@ constexpr int PC_EPB = GWF_PC_EPB;
 // counter[1] and leaves, and k_pq_poison then fills the whole output with
 // NaN, so the launch always ends and a failure cannot go unnoticed.
 #ifndef GWF_PQ_NBUF
-#define GWF_PQ_NBUF 3  // measured (IMRPhenomD, 10^4 x 2000): 3 buffers 2.38 ms, 2 buffers 2.44 ms
+#define GWF_PQ_NBUF 3  // measured (IMRPhenomD, 10^4 x 2000): 3 buffers 2.38 ms, 2 buffers 2.44 ms, 4 as 3
 #endif
 constexpr int PQ_NBUF = GWF_PQ_NBUF;
+constexpr int PQ_LDS_BYTES = 160 * 1024;  // LDS of one CU (one resident block)
 constexpr unsigned PQ_SPIN_LIMIT = 1u << 22;  // >= 0.1 s of polling; a chunk takes ~10 us
 unsigned g_pq_spin_limit = PQ_SPIN_LIMIT;      // gwf_set_queue_spin_limit
 
@@ -162,6 +163,14 @@ __device__ __forceinline__ void pq_fail(unsigned int* counter, int lane) {
 // consecutive MFMAs do not wait for each other.
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PQ_XS = 17;
+
+// Deepest ring (<= PQ_NBUF buffers) whose chunks (nv doubles per point), the
+// headers and, with the matrix-core Gram, the transposition tiles fit the LDS.
+constexpr int pq_ring_depth(int nv, bool mx) {
+  int nb = PQ_NBUF;
+  while (nb > 2 && PC_EPB * (nb * (nv * 64 * 8 + 16) + 8 + (mx ? 64 * PQ_XS * 8 : 0)) > PQ_LDS_BYTES) --nb;
+  return nb;
+}
 #ifndef GWF_PQ_NACC
 #define GWF_PQ_NACC 2  // measured: 4 accumulators no faster
 #endif
@@ -347,8 +356,9 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   typedef PcLayout<NT> L;
   constexpr bool MASKED = MODE == PQ_MASKED, ARMS = MODE == PQ_ARMS;
   constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P || ARMS;  // Gram on the matrix cores (pc_consume_mx)
-  // ring depth: two buffers where three would not fit beside the MFMA tiles
-  constexpr int NB = (MX && L::NV > 19) ? 2 : PQ_NBUF;
+  // ring depth: PQ_NBUF buffers, fewer where they would not fit the CU's LDS
+  // beside the MFMA tiles
+  constexpr int NB = pq_ring_depth(L::NV, MX);
   __shared__ double buf[PC_EPB][NB][L::NV * 64];
   __shared__ int head[PC_EPB][NB][4];  // event, chunk, grid length of the item in each buffer
   __shared__ int cnt[PC_EPB][2];            // items produced, items consumed
