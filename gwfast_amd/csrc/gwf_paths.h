@@ -447,22 +447,27 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
   }
 }
 
-// Value-only amplitude without 1/dL (SNR path).
+// Value-only amplitude without 1/dL and, when need_tau, tau_star (SNR path).
+// IMRPhenomD shares x^(1/3) = (M G f)^(1/3) between them: tau's
+// v = (pi M G f)^(1/3) = pi^(1/3) x^(1/3), as in wf_point.
 template <int MODEL>
-GWF_HD double wf_amp_val(const gwf_waveform& w, const double* rec, double f) {
+GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f, bool need_tau, double* tau) {
   if (is_pd(MODEL)) {
     constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
-    const double x = rec[PD_MG * SLOT] * f;
-    return pd_amp_val(rec, SLOT, x, w.apply_fcut != 0 || NRT, NRT);
+    const double MG = rec[PD_MG * SLOT];
+    const double x = MG * f;
+    const double x13 = cbrt(x);
+    if (need_tau) {
+      const double v = x13 * cbrt(PI);
+      *tau = tau_star_v(MG, rec[PD_ETA * SLOT], v, log(v));
+    } else {
+      *tau = 0.0;
+    }
+    return pd_amp_val(rec, SLOT, x, x13, w.apply_fcut != 0 || NRT, NRT);
   }
+  *tau = need_tau ? tau_star(rec[TF_MG * SLOT], rec[TF_ETA * SLOT], f) : 0.0;
   const double f13 = cbrt(f);
   return rec[TF_AMP * SLOT] * (1.0 / (f * sqrt(f13)));
-}
-
-GWF_HD double tau_val(const double* rec, int model, double f) {
-  const int sm = is_pd(model) ? (int)PD_MG : (int)TF_MG;
-  const int se = is_pd(model) ? (int)PD_ETA : (int)TF_ETA;
-  return tau_star(rec[sm * SLOT], rec[se * SLOT], f);
 }
 
 // ------------------------------------------------------ strain derivatives
@@ -832,8 +837,8 @@ GWF_HD void snr_event(const gwf_waveform& w, const NetK& net, const double* rec,
     PsdPre pre0, pre1;
     psd_prefetch(net.psd[0], f, lnf, pre0);
     if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
-    const double Av = wf_amp_val<MODEL>(w, rec, f) * idL;
-    const double tau = earth ? tau_val(rec, MODEL, f) : 0.0;
+    double tau;
+    const double Av = wf_amp_tau_val<MODEL>(w, rec, f, earth, &tau) * idL;
     double cdel, sdel;
     earth_delta(earth, tau, &cdel, &sdel);
     int cur_site = -1, cur_psd = -1;

@@ -720,10 +720,10 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const X& x, const X& x13, b
 }
 
 // Value-only IMRPhenomD amplitude (the SNR path): reads only .v of the record.
-GWF_HD double pd_amp_val(const double* rec, int stride, double x, bool apply_fcut, bool nrt = false) {
+// x13 = cbrt(x), computed once by the caller (the SNR path reuses it for tau)
+GWF_HD double pd_amp_val(const double* rec, int stride, double x, double x13, bool apply_fcut, bool nrt = false) {
   double amp;
   if (x < 0.014) {
-    const double x13 = cbrt(x);
     const double x23 = x13 * x13;
     const double x43 = x23 * x23;
     const double x53 = x43 * x13;
@@ -743,7 +743,6 @@ GWF_HD double pd_amp_val(const double* rec, int stride, double x, bool apply_fcu
     const double fdg3 = fdamp * g3;
     amp = exp(-(x - fring) * g2 / fdg3) * (fdg3 * g1) / ((x - fring) * (x - fring) + fdg3 * fdg3);
   }
-  const double x13 = cbrt(x);
   // x^(-7/6) = 1 / (x * x^(1/6)), x^(1/6) = sqrt(x^(1/3))
   if (!nrt) return rec[PD_OAMP * stride] * (1.0 / (x * sqrt(x13))) * amp;
   const double A = rec[PD_OAMP * stride] * (1.0 / (x * sqrt(x13))) * amp + nrt_amp_tidal_val(rec, stride, x13);
