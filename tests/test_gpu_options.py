@@ -417,6 +417,10 @@ def test_fisher_all_arms_one_launch(gpu, model):
         assert e <= 1e-13
     full = S.fisher_device(sigs, wf, rows, res).cpu().numpy()
     assert norm_err(Fa.sum(axis=0), full) <= 1e-13
+    Fl = S.fisher_arms_device(sigs, wf, rows, res, lin=True).cpu().numpy()   # spacing='lin' grids
+    for k in (0, 3):
+        assert norm_err(Fl[k], S.fisher_device(sigs, wf, rows, res, arm=k, lin=True).cpu().numpy()) <= 1e-13
+    assert norm_err(Fl[0], Fa[0]) > 1e-6
     allF = net.FisherMatr(dict(ev), res=res, return_all=True)
     assert sorted(allF) == sorted(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM', 'net'])
     for k, key in enumerate(['ETS_0', 'ETS_1', 'ETS_2', 'CE1Id', 'CE2NM']):
