@@ -846,3 +846,20 @@ def test_tidal_eccentric_tf2_is_additive():
     circ = OW.TaylorF2().Phi(f, ev)
     assert np.max(np.abs(both - (tid + ecc - circ))) <= 1e-12 * np.max(np.abs(both))
     assert OW.TaylorF2(is_tidal=True, is_eccentric=True).nParams == 14
+
+
+def test_tidal_eccentric_faithful_equals_dual():
+    """The 14-parameter TaylorF2 Fisher built the reference's way (AD rows for
+    Mc, eta, chi1z, chi2z, LambdaTilde, deltaLambda, ecc spliced with the
+    closed-form rows, signal.py:1143-1299) equals the all-dual construction:
+    the ecc row lands at ParNums['ecc'] = 13."""
+    from tests._common import NET_C1, oracle_net, norm_err
+    from gwfast_amd import catalog
+    ev = catalog.draw_bns(3, seed=5)
+    ev['ecc'] = np.array([0.02, 0.08, 0.15])
+    owf = OW.TaylorF2(is_tidal=True, is_eccentric=True)
+    d = oracle_net(NET_C1[1:2], True)
+    Fd = OD.network_fisher(d, owf, ev, 400, mode='dual')
+    Ff = OD.network_fisher(d, owf, ev, 400, mode='faithful')
+    assert Fd.shape == (14, 14, 3)
+    assert norm_err(Ff, Fd) <= 1e-8
