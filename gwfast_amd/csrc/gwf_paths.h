@@ -22,7 +22,9 @@ constexpr double INV_DAY = 1.0 / (3600. * 24.);
 // -------------------------------------------------------------- event record
 // [waveform slots: REC_SLOTS * SLOT][event constants][site constants]
 enum EvConst {
-  EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_CRA, EV_SRA, EV_COUNT
+  EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_CRA, EV_SRA,
+  EV_LMG, EV_LMG_D0, EV_LMG_D1,  // ln(M G) and its (Mc, eta) tangents
+  EV_COUNT
 };
 enum SiteConst {
   SC_U1 = 0, SC_U2, SC_U3, SC_U1T, SC_U2T, SC_U3T,
@@ -275,6 +277,12 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   ev[EV_CRA] = cos(phi);
   ev[EV_SRA] = sin(phi);
   ev[EV_FCUT] = rec[(is_pd(MODEL) ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
+  {  // ln(M G f) = ln(M G) + ln f on the grids, whose ln f comes with f
+    const double* mg = rec + (is_pd(MODEL) ? (int)PD_MG : (int)TF_MG) * SLOT;
+    ev[EV_LMG] = log(mg[0]);
+    ev[EV_LMG_D0] = mg[1] / mg[0];
+    ev[EV_LMG_D1] = mg[2] / mg[0];
+  }
   // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
   const double ra = phi, dec = 0.5 * PI - theta;
   const double cd = cos(dec), sd = sin(dec), cr = cos(ra), sr = sin(ra);
@@ -416,17 +424,28 @@ struct WfPoint {
 // M G (and so x, v and their powers and logarithms) depends on Mc and eta
 // only: it is carried as a D<2> over the first two tangents, and the mixed
 // dual arithmetic leaves the other tangents' zero products out.
-template <int MODEL, int NT>
-GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o) {
+// HAVE_LNF: the caller (a grid walk) passes lnf = ln f.
+template <int MODEL, int NT, bool HAVE_LNF = false>
+GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o,
+                     double lnf = 0.0) {
   const int smg = is_pd(MODEL) ? (int)PD_MG : (int)TF_MG;
   const int seta = is_pd(MODEL) ? (int)PD_ETA : (int)TF_ETA;
   const D<2> MG = ldd<2>(rec + smg * SLOT);
+  // ln(M G f) from the event's ln(M G) and ln f (its tangents are those of ln M G)
+  constexpr bool have_lnf = HAVE_LNF;
+  D<2> lmgf;
+  if (have_lnf) {
+    const double* ev = rec + REC_EV;
+    lmgf.v = ev[EV_LMG] + lnf;
+    lmgf.d[0] = ev[EV_LMG_D0];
+    lmgf.d[1] = ev[EV_LMG_D1];
+  }
   D<2> v, lv;
   if (is_pd(MODEL)) {
     constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
     const D<2> x = MG * f;
     const D<2> x13 = dcbrt(x);
-    const D<2> lx = dlog(x);
+    const D<2> lx = have_lnf ? lmgf : dlog(x);
     o.A = pd_amp_at<NT, NRT>(rec, SLOT, x, x13, w.apply_fcut != 0 || NRT);
     o.Phi = pd_phi_at<NT, NRT>(rec, SLOT, x, x13, lx, w.apply_fcut != 0);
     if (need_tau) {
@@ -435,7 +454,7 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
     }
   } else {
     v = dcbrt(PI * MG * f);
-    lv = dlog(v);
+    lv = have_lnf ? (lmgf + log(PI)) * (1.0 / 3.0) : dlog(v);
     o.Phi = tf2_phi_at<NT>(rec, SLOT, v, lv);
     const double f13 = cbrt(f);
     o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
@@ -619,7 +638,7 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
     if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
 #endif
     WfPoint<MODEL, NT> wp;
-    wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
+    wf_point<MODEL, NT, true>(w, rec, f, net.any_earth != 0, wp, lnf);
     const double tc = ev[EV_TC];
     double cdel, sdel;
     earth_delta(net.any_earth != 0, wp.tau.v, &cdel, &sdel);
@@ -716,7 +735,7 @@ GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec
   psd_prefetch(net.psd[0], f, lnf, pre0);
   if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
   WfPoint<MODEL, NT> wp;
-  wf_point<MODEL, NT>(w, rec, f, net.any_earth != 0, wp);
+  wf_point<MODEL, NT, true>(w, rec, f, net.any_earth != 0, wp, lnf);
   double cdel, sdel;
   earth_delta(net.any_earth != 0, wp.tau.v, &cdel, &sdel);
   const double s0 = psd_finish(net.psd[0], f, lnf, pre0);
