@@ -329,7 +329,7 @@ def main():
         'ms_per_step': ms_per_step, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f64', 'data': 'synthetic (seeded BBH catalogue)',
         'config': {'workload': 'IMRPhenomD network Fisher: ETS triangle + CE1Id + CE2NM (5 IFOs), Earth rotation, '
-                               '11 params, %d events x %d freqs, FP64 fused plain path' % (n, RES),
+                               '11 params, %d events x %d freqs, FP64, fused path (gwf_prepare + gwf_fisher_prepared)' % (n, RES),
                    'n_events': n, 'n_freqs': RES, 'n_params': 11, 'n_ifo': 5, 'parallelism': 'single GPU'},
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': achieved / FP64_PEAK_TFLOPS,
