@@ -380,6 +380,22 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   const T fRef = MG * f0;
   const T phiRef = pd_phi_piecewise(c, fRef, apply_fcut);
 
+  // The phase coefficients go to the record before the amplitude section
+  // starts: none of them is live past this point but fring and fdamp, which
+  // shortens the live ranges through the collocation solve below.
+  double* r = rec;
+  auto put = [&](int slot, const T& x) { std_(r + slot * stride, x); };
+  put(PD_MG, MG);
+  put(PD_ETA, eta);
+  put(PD_PIP, c.pip); put(PD_P23, c.p23); put(PD_P13, c.p13); put(PD_P13L, c.p13l); put(PD_PLOG, c.plog);
+  put(PD_PM13, c.pm13); put(PD_PM23, c.pm23); put(PD_PM1, c.pm1); put(PD_PM53, c.pm53);
+  put(PD_S1, c.s1); put(PD_S43, c.s43); put(PD_S53, c.s53); put(PD_S2, c.s2);
+  put(PD_B1, beta1); put(PD_B2, beta2); put(PD_B3, beta3); put(PD_C1I, C1Int); put(PD_C2I, C2Int);
+  put(PD_A1, alpha1); put(PD_A2, alpha2); put(PD_A3, alpha3); put(PD_A4, alpha4); put(PD_A5, alpha5);
+  put(PD_C1M, C1MRD); put(PD_C2M, C2MRD);
+  put(PD_FRING, fring); put(PD_FDAMP, fdamp); put(PD_FMRDJ, fMRDJoin); put(PD_T0, t0); put(PD_FREF, fRef);
+  put(PD_PHIREF, phiRef);
+
   // Amplitude (waveforms.py:1188-1248); fpeak with fabs on the first branch only (1193)
   const T fpeakA = (val(gamma2) >= 1.0) ? dfabs(fring - (fdamp * gamma3) / gamma2)
                                         : fring + (fdamp * (-1.0 + dsqrt(1.0 - gamma2 * gamma2)) * gamma3) / gamma2;
@@ -510,18 +526,6 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   // Overallamp * amp0 without the 1/dL factor (waveforms.py:1248, 1254)
   const T OAmp = 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3 * amp0;
 
-  double* r = rec;
-  auto put = [&](int slot, const T& x) { std_(r + slot * stride, x); };
-  put(PD_MG, MG);
-  put(PD_ETA, eta);
-  put(PD_PIP, c.pip); put(PD_P23, c.p23); put(PD_P13, c.p13); put(PD_P13L, c.p13l); put(PD_PLOG, c.plog);
-  put(PD_PM13, c.pm13); put(PD_PM23, c.pm23); put(PD_PM1, c.pm1); put(PD_PM53, c.pm53);
-  put(PD_S1, c.s1); put(PD_S43, c.s43); put(PD_S53, c.s53); put(PD_S2, c.s2);
-  put(PD_B1, beta1); put(PD_B2, beta2); put(PD_B3, beta3); put(PD_C1I, C1Int); put(PD_C2I, C2Int);
-  put(PD_A1, alpha1); put(PD_A2, alpha2); put(PD_A3, alpha3); put(PD_A4, alpha4); put(PD_A5, alpha5);
-  put(PD_C1M, C1MRD); put(PD_C2M, C2MRD);
-  put(PD_FRING, fring); put(PD_FDAMP, fdamp); put(PD_FMRDJ, fMRDJoin); put(PD_T0, t0); put(PD_FREF, fRef);
-  put(PD_PHIREF, phiRef);
   put(PD_AM23, A23); put(PD_AM43, A43); put(PD_AM53, A53); put(PD_AM73, rho1); put(PD_AM83, rho2);
   put(PD_AM1, A1); put(PD_AM2, A2); put(PD_AM3, rho3);
   put(PD_DL0, delta0); put(PD_DL1, delta1); put(PD_DL2, delta2); put(PD_DL3, delta3); put(PD_DL4, delta4);
