@@ -459,6 +459,12 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
                (dexp((f3 - fring) * gamma2 / (fdamp * gamma3)) * ((f3 - fring) * (f3 - fring) + fdg3 * fdamp * gamma3));
   const T v2 = fit_eval(FIT_V2, eta, eta2, xi);
 
+  // The amplitude coefficients, too, go out before the collocation solve,
+  // which needs only f1, f2, f3, v1, v2, v3, d1 and d2.
+  put(PD_AM23, A23); put(PD_AM43, A43); put(PD_AM53, A53); put(PD_AM73, rho1); put(PD_AM83, rho2);
+  put(PD_AM1, A1); put(PD_AM2, A2); put(PD_AM3, rho3);
+  put(PD_G1, gamma1); put(PD_G2, gamma2); put(PD_G3, gamma3); put(PD_FPEAKA, fpeakA);
+
   // Collocation solution for the intermediate amplitude polynomial
   // (waveforms.py:1229-1245): the unique quartic with p(f1)=v1, p'(f1)=d1,
   // p(f2)=v2, p(f3)=v3, p'(f3)=d2, written out as in the reference.
@@ -526,10 +532,8 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   // Overallamp * amp0 without the 1/dL factor (waveforms.py:1248, 1254)
   const T OAmp = 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3 * amp0;
 
-  put(PD_AM23, A23); put(PD_AM43, A43); put(PD_AM53, A53); put(PD_AM73, rho1); put(PD_AM83, rho2);
-  put(PD_AM1, A1); put(PD_AM2, A2); put(PD_AM3, rho3);
   put(PD_DL0, delta0); put(PD_DL1, delta1); put(PD_DL2, delta2); put(PD_DL3, delta3); put(PD_DL4, delta4);
-  put(PD_G1, gamma1); put(PD_G2, gamma2); put(PD_G3, gamma3); put(PD_FPEAKA, fpeakA); put(PD_OAMP, OAmp);
+  put(PD_OAMP, OAmp);
   // fcut = 0.2 / (Mc G / eta^(3/5)) (waveforms.py:1333)
   put(PD_FCUT, mkc<N>(0.2 / (Mc.v * GMSUN_C3 / pow(eta.v, 3. / 5.))));
   put(PD_IETA, 1.0 / eta);
