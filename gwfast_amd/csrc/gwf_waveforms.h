@@ -464,6 +464,8 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   put(PD_AM23, A23); put(PD_AM43, A43); put(PD_AM53, A53); put(PD_AM73, rho1); put(PD_AM83, rho2);
   put(PD_AM1, A1); put(PD_AM2, A2); put(PD_AM3, rho3);
   put(PD_G1, gamma1); put(PD_G2, gamma2); put(PD_G3, gamma3); put(PD_FPEAKA, fpeakA);
+  // Overallamp * amp0 without the 1/dL factor (waveforms.py:1248, 1254)
+  put(PD_OAMP, 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3 * amp0);
 
   // Collocation solution for the intermediate amplitude polynomial
   // (waveforms.py:1229-1245): the unique quartic with p(f1)=v1, p'(f1)=d1,
@@ -485,6 +487,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
          2 * f14 * f23 * v3 - f13 * f24 * v3 + 2 * f15 * f2 * f3 * v3 - f14 * f22 * f3 * v3 - 4 * f13 * f23 * f3 * v3 +
          3 * f12 * f24 * f3 * v3 - 4 * f14 * f2 * f32 * v3 + 8 * f13 * f22 * f32 * v3 - 4 * f12 * f23 * f32 * v3) /
         den);
+  put(PD_DL0, delta0);
   const T delta1 =
       -((-(d2 * f15 * f22) + 2. * d2 * f14 * f23 - d2 * f13 * f24 - d2 * f14 * f22 * f3 + 2. * d1 * f13 * f23 * f3 +
          2. * d2 * f13 * f23 * f3 - 2 * d1 * f12 * f24 * f3 - d2 * f12 * f24 * f3 + d2 * f15 * f32 - 3 * d1 * f13 * f22 * f32 -
@@ -496,6 +499,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
          4 * f14 * f32 * v2 + 4 * f12 * f34 * v2 - 2 * F1 * f35 * v2 - 2 * f15 * f3 * v3 + 8 * f12 * f23 * f3 * v3 -
          6 * F1 * f24 * f3 * v3 + 4 * f14 * f32 * v3 - 12 * f12 * f22 * f32 * v3 + 8 * F1 * f23 * f32 * v3) /
         den);
+  put(PD_DL1, delta1);
   const T delta2 =
       -((d2 * f15 * f2 - d1 * f13 * f23 - 3 * d2 * f13 * f23 + d1 * f12 * f24 + 2. * d2 * f12 * f24 - d2 * f15 * f3 +
          d2 * f14 * f2 * f3 - d1 * f12 * f23 * f3 + d2 * f12 * f23 * f3 + d1 * F1 * f24 * f3 - d2 * F1 * f24 * f3 -
@@ -508,6 +512,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
          f15 * v3 - 4 * f12 * f23 * v3 + 3 * F1 * f24 * v3 + f14 * f3 * v3 - 4 * F1 * f23 * f3 * v3 + 3 * f24 * f3 * v3 -
          8 * f13 * f32 * v3 + 12 * f12 * f2 * f32 * v3 - 4 * f23 * f32 * v3) /
         den);
+  put(PD_DL2, delta2);
   const T delta3 =
       -((-2. * d2 * f14 * f2 + d1 * f13 * f22 + 3 * d2 * f13 * f22 - d1 * F1 * f24 - d2 * F1 * f24 + 2 * d2 * f14 * f3 -
          2. * d1 * f13 * f2 * f3 - 2 * d2 * f13 * f2 * f3 + d1 * f12 * f22 * f3 - d2 * f12 * f22 * f3 + d1 * f24 * f3 +
@@ -520,6 +525,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
          4 * f13 * f3 * v3 - 8 * f12 * f2 * f3 * v3 + 4 * F1 * f22 * f3 * v3 + 4 * f12 * f32 * v3 -
          8 * F1 * f2 * f32 * v3 + 4 * f22 * f32 * v3) /
         den);
+  put(PD_DL3, delta3);
   const T delta4 =
       -((d2 * f13 * f2 - d1 * f12 * f22 - 2 * d2 * f12 * f22 + d1 * F1 * f23 + d2 * F1 * f23 - d2 * f13 * f3 +
          2. * d1 * f12 * f2 * f3 + d2 * f12 * f2 * f3 - d1 * F1 * f22 * f3 + d2 * F1 * f22 * f3 - d1 * f23 * f3 -
@@ -529,11 +535,8 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
          3 * F1 * f32 * v2 + f33 * v2 + f13 * v3 - 3 * F1 * f22 * v3 + 2 * f23 * v3 - 3 * f12 * f3 * v3 +
          6 * F1 * f2 * f3 * v3 - 3 * f22 * f3 * v3) /
         den);
-  // Overallamp * amp0 without the 1/dL factor (waveforms.py:1248, 1254)
-  const T OAmp = 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3 * amp0;
+  put(PD_DL4, delta4);
 
-  put(PD_DL0, delta0); put(PD_DL1, delta1); put(PD_DL2, delta2); put(PD_DL3, delta3); put(PD_DL4, delta4);
-  put(PD_OAMP, OAmp);
   // fcut = 0.2 / (Mc G / eta^(3/5)) (waveforms.py:1333)
   put(PD_FCUT, mkc<N>(0.2 / (Mc.v * GMSUN_C3 / pow(eta.v, 3. / 5.))));
   put(PD_IETA, 1.0 / eta);
