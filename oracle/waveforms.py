@@ -350,6 +350,20 @@ class IMRPhenomD:
 
     def Phi(self, f, ev):
         """waveforms.py:992-1153."""
+        co = self.phase_coeffs(f, ev)
+        phis = self._phi_piecewise(co['fgrid'], co)
+        extra = self._phase_extra(co['fgrid'], ev, co['eta'], co['Seta'], co['QuadMon1'], co['QuadMon2'],
+                                  co['TF2OverallAmpl'])
+        t0, fRef, phiRef = co['t0'], co['fRef'], co['phiRef']
+        if self.apply_fcut:
+            return phis + D.where(D.value(co['fgrid']) < self.fcutPar, -t0 * (co['fgrid'] - fRef) - phiRef + extra, 0.)
+        return phis - t0 * (co['fgrid'] - fRef) - phiRef + extra
+
+    def phase_coeffs(self, f, ev):
+        """The per-event constants of the phase (waveforms.py:992-1143): the
+        inspiral / intermediate / merger-ringdown coefficients, the C1 join
+        constants, fpeak, the time shift t0, fRef (in Mf) and phiRef, plus
+        fgrid = M G f."""
         M = ev['Mc'] / (ev['eta'] ** (3. / 5.))
         eta = ev['eta']
         eta2 = eta * eta
@@ -443,13 +457,11 @@ class IMRPhenomD:
             fRef = M * glob.GMsun_over_c3 * self.fRef
         co = dict(P=P, eta=eta, beta1=beta1, beta2=beta2, beta3=beta3, C1Int=C1Int, C2Int=C2Int, alpha1=alpha1,
                   alpha2=alpha2, alpha3=alpha3, alpha4=alpha4, alpha5=alpha5, fring=fring, fdamp=fdamp,
-                  C1MRD=C1MRD, C2MRD=C2MRD, fMRDJoin=fMRDJoin)
-        phiRef = self._phi_piecewise(fRef, co)
-        phis = self._phi_piecewise(fgrid, co)
-        extra = self._phase_extra(fgrid, ev, eta, Seta, QuadMon1, QuadMon2, TF2OverallAmpl)
-        if self.apply_fcut:
-            return phis + D.where(D.value(fgrid) < self.fcutPar, -t0 * (fgrid - fRef) - phiRef + extra, 0.)
-        return phis - t0 * (fgrid - fRef) - phiRef + extra
+                  C1MRD=C1MRD, C2MRD=C2MRD, fMRDJoin=fMRDJoin, gamma2=gamma2, gamma3=gamma3, fpeak=fpeak, t0=t0,
+                  fRef=fRef, fgrid=fgrid, Seta=Seta, QuadMon1=QuadMon1, QuadMon2=QuadMon2,
+                  TF2OverallAmpl=TF2OverallAmpl)
+        co['phiRef'] = self._phi_piecewise(fRef, co)
+        return co
 
     def _quadmons(self, ev):
         return 1.0, 1.0

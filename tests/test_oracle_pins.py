@@ -863,3 +863,136 @@ def test_tidal_eccentric_faithful_equals_dual():
     Ff = OD.network_fisher(d, owf, ev, 400, mode='faithful')
     assert Fd.shape == (14, 14, 3)
     assert norm_err(Ff, Fd) <= 1e-8
+
+
+# ------------------------------------------- tidal TaylorF2, t0 and phiRef
+def _vfh_tidal_phase(v, eta, m1, m2, L1, L2):
+    """Tidal phase of a binary summed body by body, in the per-body form of
+    Vines, Flanagan & Hinderer (2011, PRD 83, 084051, eq. 3.5): with
+    X_A = m_A / M, 5PN -24 (12 - 11 X_A) X_A^4 Lambda_A and 6PN
+    -(5/28) (3179 - 919 X_A - 2286 X_A^2 + 260 X_A^3) X_A^4 Lambda_A, times
+    3 / (128 eta v^5).  Wade et al. (2014, PRD 89, 103012; cited at
+    waveforms.py:851) rewrite exactly this sum with LambdaTilde and
+    deltaLambda."""
+    M = m1 + m2
+    s = 0.
+    for L, X in ((L1, m1 / M), (L2, m2 / M)):
+        s = s + L * X ** 4 * (-24. * (12. - 11. * X) * v ** 10
+                              - (5. / 28.) * (3179. - 919. * X - 2286. * X * X + 260. * X ** 3) * v ** 12)
+    return 3. / (128. * eta * v ** 5) * s
+
+
+def test_tf2_tidal_phase_is_per_body_vfh_sum():
+    """TaylorF2's tidal phase (waveforms.py:850-858) minus the point-particle
+    phase equals the per-body 5PN + 6PN tidal phase written in Lambda1,
+    Lambda2 and the component masses.  This pins the -39/2 and -3115/64
+    coefficients, the 6595/364 sqrt(1 - 4 eta) deltaLambda term and the
+    LambdaTilde / deltaLambda combinations of gwfastUtils.py:398-417 at
+    once; the inverse map (gwfastUtils.py:419-444) must return the inputs."""
+    from gwfast_amd import catalog
+    ev = catalog.draw_bns(8, seed=21)
+    ev['eta'][0] = 0.25 - 1e-9           # near-equal masses
+    ev['eta'][1] = 0.20                  # mass ratio ~ 0.38
+    M = ev['Mc'] / ev['eta'] ** 0.6
+    Seta = np.sqrt(1. - 4. * ev['eta'])
+    m1, m2 = 0.5 * M * (1. + Seta), 0.5 * M * (1. - Seta)
+    fc = OW.TaylorF2(is_tidal=True).fcut(ev)
+    f = np.geomspace(20., 1., 60)[:, None] * fc[None, :] / 21.   # v from ~0.1 to ~0.4
+    d = OW.TaylorF2(is_tidal=True).Phi(f, ev) - OW.TaylorF2().Phi(f, ev)
+    v = (np.pi * M * glob.GMsun_over_c3 * f) ** (1. / 3.)
+    want = _vfh_tidal_phase(v, ev['eta'], m1, m2, ev['Lambda1'], ev['Lambda2'])
+    scale = np.max(np.abs(want), axis=0)
+    assert np.all(scale > 1.)                # the tidal phase is radians-sized here
+    # the difference of two ~1e2-1e4 rad phases carries ~1e-12 rad of rounding
+    assert np.max(np.abs(d - want) / scale) < 1e-10
+    Lt, dLm = OW.lamt_dellam_from_lam12(ev['Lambda1'], ev['Lambda2'], ev['eta'])
+    L1, L2 = OW.lam12_from_lamt_dellam(Lt, dLm, ev['eta'])
+    assert np.allclose(L1, ev['Lambda1'], rtol=1e-9, atol=1e-9)
+    assert np.allclose(L2, ev['Lambda2'], rtol=1e-9, atol=1e-9)
+    # equal masses: LambdaTilde = Lambda, deltaLambda = 0 (Wade et al. 2014)
+    Lt, dLm = OW.lamt_dellam_from_lam12(np.array([400.]), np.array([400.]), np.array([0.25]))
+    assert abs(Lt[0] - 400.) < 1e-12 and abs(dLm[0]) < 1e-12
+
+
+def _pd_branch_events():
+    """Events on both sides of IMRPhenomD's gamma2 = 1 switch (gamma2 >= 1
+    only at small eta and spins near +1)."""
+    n = 6
+    ev = {'Mc': np.array([12., 8., 30., 25., 40., 20.]), 'eta': np.array([0.05, 0.055, 0.2, 0.2499, 0.12, 0.05]),
+          'dL': np.ones(n), 'chi1z': np.array([0.99, 0.99, 0.3, -0.6, 0.8, -0.9]),
+          'chi2z': np.array([0.99, 0.99, -0.2, -0.6, 0.1, -0.9])}
+    return ev
+
+
+def _ansatz_mrd(x, co):
+    """The merger-ringdown phase ansatz of Khan et al. (2016, PRD 93 044007,
+    eq. 14): (alpha1 x - alpha2 / x + (4/3) alpha3 x^(3/4)
+    + alpha4 arctan((x - alpha5 fring) / fdamp)) / eta."""
+    a1, a2, a3, a4, a5 = (D.value(co[k]) for k in ('alpha1', 'alpha2', 'alpha3', 'alpha4', 'alpha5'))
+    fr, fd = D.value(co['fring']), D.value(co['fdamp'])
+    return (a1 * x - a2 / x + (4. / 3.) * a3 * x ** 0.75 + a4 * np.arctan((x - a5 * fr) / fd)) / D.value(co['eta'])
+
+
+def test_phenomd_t0_is_mrd_time_at_fpeak():
+    """IMRPhenomD's time shift t0 (waveforms.py:1132-1136) is the derivative
+    of the merger-ringdown phase ansatz at the amplitude peak fpeak, on both
+    gamma2 branches; fpeak maximises the Lorentzian ringdown amplitude
+    exp(-gamma2 (x - fring) / (gamma3 fdamp)) / ((x - fring)^2
+    + (gamma3 fdamp)^2) where gamma2 < 1.  End to end, the final phase's slope at fpeak is C2MRD."""
+    from scipy.optimize import minimize_scalar
+    wf = OW.IMRPhenomD()
+    ev = _pd_branch_events()
+    MG = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    f = np.geomspace(2., 1000., 16)[:, None] / (MG[None, :] / MG.min())
+    co = wf.phase_coeffs(f, ev)
+    g2 = D.value(co['gamma2'])
+    assert np.any(g2 >= 1.) and np.any(g2 < 1.)
+    fpk, t0 = D.value(co['fpeak']), D.value(co['t0'])
+    h = 1e-5 * fpk
+    fd_t0 = (_ansatz_mrd(fpk + h, co) - _ansatz_mrd(fpk - h, co)) / (2. * h)
+    assert np.max(np.abs(fd_t0 / t0 - 1.)) < 1e-8
+    fr, fdm, g3 = D.value(co['fring']), D.value(co['fdamp']), D.value(co['gamma3'])
+    for e in np.nonzero(g2 < 1.)[0]:
+        a = fdm[e] * g3[e]
+        r = minimize_scalar(lambda x: g2[e] * (x - fr[e]) / a + np.log((x - fr[e]) ** 2 + a * a),
+                            bounds=(fr[e] - a, fr[e] + 5 * a), method='bounded', options={'xatol': 1e-14})
+        assert abs(r.x / fpk[e] - 1.) < 1e-7
+    # (the ansatz also has a local minimum below fring - gamma3 fdamp, where
+    # it rises again towards low frequency; the peak is the local maximum.
+    # For gamma2 >= 1 the ansatz has no maximum and fpeak = |fring - gamma3
+    # fdamp / gamma2| is the model's convention, the same point as the
+    # amplitude's collocation peak.)
+    # the total phase's slope at fpeak (Mf units) is C2MRD
+    evf = dict(ev)
+    fp_hz = fpk / MG
+    ff = np.array([fp_hz * (1. - 1e-5), fp_hz * (1. + 1e-5)])
+    c2 = D.value(co['C2MRD'])
+    assert np.all(fpk < wf.fcutPar)
+    for cut in (True, False):
+        p = OW.IMRPhenomD(apply_fcut=cut).Phi(ff, evf)
+        slope = (p[1] - p[0]) / (MG * (ff[1] - ff[0]))
+        assert np.max(np.abs(slope - c2) / np.abs(t0)) < 1e-6
+
+
+def test_phenomd_phase_vanishes_at_fref():
+    """The reference phase phiRef (waveforms.py:1139-1147) is the phase at
+    fRef: the returned phase is exactly zero there, with fRef the grid
+    minimum by default or a given frequency; t0 (f - fRef) leaves the
+    phase's slope unchanged by the choice of fRef."""
+    ev = _pd_ev(4, seed=5)
+    MG = ev['Mc'] / ev['eta'] ** 0.6 * glob.GMsun_over_c3
+    fc = OW.IMRPhenomD().fcut(ev)
+    f = np.geomspace(2., 1., 300)[:, None] * fc[None, :] * (np.array([1.0, 0.9, 0.5, 0.2])[None, :] + 0.)
+    f = np.sort(f, axis=0)
+    p = OW.IMRPhenomD().Phi(f, ev)
+    assert np.all(np.abs(p[0]) <= 1e-12 * np.max(np.abs(p), axis=0))
+    assert np.max(np.abs(p[-5])) > 1.
+    wf10 = OW.IMRPhenomD(fRef=10.)
+    f10 = np.concatenate([np.full((1, 4), 10.), f], axis=0)
+    p10 = wf10.Phi(f10, ev)
+    assert np.all(np.abs(p10[0]) <= 1e-12 * np.max(np.abs(p10), axis=0))
+    # changing fRef adds a constant: the slopes agree
+    p_default = OW.IMRPhenomD().Phi(f10, ev)
+    dd = (p10 - p_default) - (p10 - p_default)[0]
+    assert np.max(np.abs(dd)) <= 1e-9 * np.max(np.abs(p_default))
+    del MG
