@@ -52,28 +52,58 @@ def build_net():
     return wf, DetNet(sigs, verbose=False)
 
 
-def cpu_baseline(n_sample=12, res=RES, seed=0):
+def _cpu_chunk(args):
+    """Worker of cpu_baseline: the oracle's network Fisher of one chunk of
+    events on one core."""
+    ev, res = args
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):
+        from gwfast_amd import gwfastGlobals as glob
+        from oracle import detector as OD, waveforms as OW
+        psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
+        dets = []
+        for name, p in SITES:
+            d = glob.detectors[name]
+            dets.append(OD.Detector(psd[p], d['shape'], d['lat'], d['long'], d['xax'], useEarthMotion=True, fmin=2.))
+        OD.network_fisher(dets, OW.IMRPhenomD(), ev, res, mode='faithful')
+    return len(ev['Mc'])
+
+
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as fh:
+            for line in fh:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def cpu_baseline(per_core=128, res=RES, seed=0, max_cores=16):
     """The oracle (reference-faithful construction: AD-equivalent rows through
-    exp(i Psi), closed-form rows, trapz Gram) on n_sample events of the same
-    workload, single-threaded NumPy."""
-    from gwfast_amd import catalog, gwfastGlobals as glob
-    from oracle import detector as OD, waveforms as OW
+    exp(i Psi), closed-form rows, trapz Gram) on per_core events per core of
+    the same workload, one single-threaded process per core (the reference's
+    catalogue driver parallelises over a process pool the same way,
+    run/calculate_forecasts_from_catalog.py:152-186), two chunks per core.
+    Runs before the GPU is initialised, so the pool forks a clean process."""
+    import multiprocessing as mp
+    from gwfast_amd import catalog
+    avail = len(os.sched_getaffinity(0))
+    cores = max(1, min(avail, max_cores))
+    n_sample = per_core * cores
     ev = catalog.draw_bbh(n_sample, seed=seed)
-    psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
-    dets = []
-    for name, p in SITES:
-        d = glob.detectors[name]
-        dets.append(OD.Detector(psd[p], d['shape'], d['lat'], d['long'], d['xax'], useEarthMotion=True, fmin=2.))
-    wf = OW.IMRPhenomD()
+    chunk = max(1, min(128, per_core // 2))  # bounds host memory of the broadcast derivative arrays
+    jobs = [({k: v[s:s + chunk] for k, v in ev.items()}, res) for s in range(0, n_sample, chunk)]
     t = time.perf_counter()
-    chunk = 128  # bounds host memory of the broadcast derivative arrays
-    for s in range(0, n_sample, chunk):
-        evc = {k: v[s:s + chunk] for k, v in ev.items()}
-        OD.network_fisher(dets, wf, evc, res, mode='faithful')
+    with mp.get_context('fork').Pool(cores) as pool:
+        done = sum(pool.map(_cpu_chunk, jobs, chunksize=1))
     dt = time.perf_counter() - t
-    return {'value': n_sample / dt, 'unit': 'Fisher matr/s', 'cores': 1, 'kind': 'port',
-            'sample': '%d events of the same workload (IMRPhenomD, ETS+CE1Id+CE2NM, Earth rotation, %d freqs), '
-                      'NumPy FP64 restatement of gwfast\'s algorithm, %.1f s' % (n_sample, res, dt)}
+    return {'value': done / dt, 'unit': 'Fisher matr/s', 'cores': cores, 'kind': 'port',
+            'cores_available': avail, 'cores_used': cores, 'cpu_model': _cpu_model(),
+            'sample': '%d events of the same workload (IMRPhenomD, ETS+CE1Id+CE2NM, Earth rotation, %d freqs): '
+                      'the oracle (CPU restatement of gwfast\'s algorithm in NumPy FP64, not gwfast), one process '
+                      'per core, %.1f s wall' % (done, res, dt)}
 
 
 def plain_path(net, rows, n, steps=3, mode='fp64'):
@@ -158,7 +188,8 @@ def per_arm_leg(net, rows, n, steps=5):
 
 
 def ragged_leg(n=100_000, steps=3, seed=11):
-    """Config C3: TaylorF2 with tidal terms (13 parameters), the same network
+    """Config C3: TaylorF2 with tidal terms (13 parameters, cut at the Kerr
+    ISCO of the remnant: 1.3-4.2 kHz for this population), the same network
     with Earth rotation, 10^5 BNS events each on its own grid length drawn
     uniformly in [500, 10^4] (gwf_fisher_ragged, one persistent launch)."""
     import torch
@@ -166,7 +197,7 @@ def ragged_leg(n=100_000, steps=3, seed=11):
     from gwfast_amd.network import DetNet
     from gwfast_amd.signal import GWSignal
     from gwfast_amd.waveforms import TaylorF2_RestrictedPN
-    wf = TaylorF2_RestrictedPN(is_tidal=True)
+    wf = TaylorF2_RestrictedPN(is_tidal=True, which_ISCO='Kerr')
     psd = {'ET': glob.PSD_ET, 'CE': glob.PSD_CE}
     sigs = {}
     for name, p in SITES:
@@ -185,10 +216,56 @@ def ragged_leg(n=100_000, steps=3, seed=11):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / steps
     Fh = F.cpu().numpy()
+    fc = np.asarray(wf.fcut(**{k: ev[k] for k in ('Mc', 'eta', 'chi1z', 'chi2z')}))
     return {'value': n / dt, 'unit': 'Fisher matr/s', 'ms_per_step': 1e3 * dt, 'n_events': n, 'n_params': 13,
             'n_freqs': '500..10000 per event (uniform), %d points in total' % int(res.sum()),
             'points_per_s': float(res.sum()) / dt, 'finite': bool(np.all(np.isfinite(Fh))),
-            'path': 'TaylorF2 tidal, ETS+CE1Id+CE2NM, Earth rotation, gwf_fisher_ragged'}
+            'cutoffs_hz': [float(fc.min()), float(fc.max())],
+            'path': 'TaylorF2 tidal (Kerr ISCO cut), ETS+CE1Id+CE2NM, Earth rotation, gwf_fisher_ragged'}
+
+
+def cov_leg(F, steps=10):
+    """CovMatr (default 'cho' with the 'svd' fallback, inversion error and
+    condition number) and CheckFisher (eigenvalues, eigenvectors) on the
+    step's Fisher matrices, timed with HIP events on the library's stream."""
+    import torch
+    from gwfast_amd import _native as N
+    L = N.lib()
+    P, n = F.shape[0], F.shape[2]
+    C = torch.empty_like(F)
+    eps = torch.empty(n, dtype=torch.float64, device=F.device)
+    cond = torch.empty(n, dtype=torch.float64, device=F.device)
+    meth = torch.empty(n, dtype=torch.int32, device=F.device)
+    ev = torch.empty((n, P), dtype=torch.float64, device=F.device)
+    vec = torch.empty((n, P, P), dtype=torch.float64, device=F.device)
+    st = N.stream_ptr()
+
+    def cov():
+        N.check(L.gwf_cov_invert(N.ptr(F), P, n, N.ptr(C), N.ptr(eps), N.ptr(cond), N.ptr(meth), st), L)
+
+    def check():
+        N.check(L.gwf_check_fisher(N.ptr(F), P, n, N.ptr(ev), N.ptr(vec), N.ptr(cond), st), L)
+
+    out = {}
+    for name, fn in (('cov_invert', cov), ('check_fisher', check)):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        out[name + '_ms'] = a.elapsed_time(b) / steps
+    cov()
+    torch.cuda.synchronize()
+    m = meth.cpu().numpy()
+    e = eps.cpu().numpy()
+    out.update({'n_events': n, 'n_params': P, 'events_per_s': n / (out['cov_invert_ms'] * 1e-3),
+                'methods': {'cho': int((m == 0).sum()), 'svd': int((m == 1).sum()), 'nan': int((m == -1).sum())},
+                'median_inversion_error': float(np.nanmedian(e)),
+                'path': 'gwf_cov_invert (k_cov) and gwf_check_fisher (k_check): one wavefront per event, LDS matrices'})
+    return out
 
 
 def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, threshold=12.0, band=(8.0, 16.0)):
@@ -229,6 +306,7 @@ def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, thres
     torch.cuda.synchronize()
     t_mlp = ev0.elapsed_time(ev1) * 1e-3
     same = bool(torch.equal(a['idx'], b['idx']) and torch.equal(a['F'], b['F']))
+    cov_confirmed = cov_leg(a['F'], steps=2)
     ts, tp = t2 - t1, t3 - t2
     mlp_tflops = SC.FLOP_PER_INFERENCE * n_cat / t_mlp / 1e12
     return {'n_catalogue': n_cat, 'threshold': threshold, 'band': list(band),
@@ -242,9 +320,60 @@ def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, thres
                          'confirmed_fisher_per_s': int(a['idx'].numel()) / ts},
             'plain': {'time_s': tp, 'n_exact_snr': b['n_exact_snr'], 'n_fisher': b['n_fisher'],
                       'confirmed_fisher_per_s': int(b['idx'].numel()) / tp},
-            'gain_end_to_end': tp / ts, 'fisher_identical_to_plain': same,
+            'gain_end_to_end': tp / ts, 'fisher_identical_to_plain': same, 'cov_confirmed': cov_confirmed,
             'mlp_inference': {'kernel_s': t_mlp, 'events_per_s': n_cat / t_mlp, 'bf16_tflops': mlp_tflops,
                               'frac_of_bf16_dense_peak': mlp_tflops / 2500.0}}
+
+
+def fisher_steps(wf, net, n, steps, warmup, seed=0):
+    """The timed step on n events: gwf_prepare (per-event records) then
+    gwf_fisher_prepared (the fused Fisher kernel), parameters resident in
+    HBM.  Returns (F, rows, wall seconds for the steps, mean Fisher-kernel
+    milliseconds from HIP events on the library's stream)."""
+    import torch
+    from gwfast_amd import _engine as E, _native as N, catalog
+    from gwfast_amd import signal as S
+    ev = catalog.draw_bbh(n, seed=seed)
+    rows = E.to_dev(E.param_rows(ev, False))
+    sigs = list(net.signals.values())
+    F = torch.empty((11, 11, n), dtype=torch.float64, device=rows.device)
+    ws, nb = E.workspace(n)
+    dets = S._dets(sigs)
+    wfc = wf.c_struct()
+    st = N.stream_ptr()
+    L = N.lib()
+
+    def step(t_fisher=None):
+        N.check(L.gwf_prepare(wfc, dets, len(sigs), N.ptr(rows), rows.stride(0), n, RES, N.ptr(ws), nb, st), L)
+        if t_fisher is not None:
+            t_fisher[0].record()
+        N.check(L.gwf_fisher_prepared(wfc, dets, len(sigs), n, RES, -1, N.ptr(F), N.ptr(ws), nb, st), L)
+        if t_fisher is not None:
+            t_fisher[1].record()
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    return F, rows, wall, kern_ms
+
+
+def c2_leg(wf, net, n=100_000, steps=5, warmup=2):
+    """Config C2: the same step on 10^5 events."""
+    F, _, wall, kern_ms = fisher_steps(wf, net, n, steps, warmup)
+    ok = bool(np.all(np.isfinite(F.cpu().numpy())))
+    achieved = GRAM_FLOP_PER_FISHER * n / (kern_ms * 1e-3) / 1e12
+    return {'value': n * steps / wall, 'unit': 'Fisher matr/s', 'n_events': n, 'steps': steps,
+            'ms_per_step': 1e3 * wall / steps, 'kernel_ms': kern_ms, 'achieved_tflops': achieved,
+            'frac_of_fp64_peak': achieved / FP64_PEAK_TFLOPS, 'finite': ok,
+            'path': 'gwf_prepare + gwf_fisher_prepared, as the main step'}
 
 
 def main():
@@ -254,7 +383,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n-events', type=int, default=N_EV)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-sample', type=int, default=512)
+    ap.add_argument('--cpu-per-core', type=int, default=128, help='CPU baseline events per core')
+    ap.add_argument('--no-c2', action='store_true', help='skip the 10^5-event (C2) leg')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
     ap.add_argument('--no-extra', action='store_true', help='skip the PCIe-inclusive, per-arm and ragged (C3) legs')
@@ -269,6 +399,9 @@ def main():
         print('bench.py measures one GPU only: there is no multi-GPU path (events would shard across GPUs '
               'with no exchange during compute). Run with --gpus 1.', file=sys.stderr)
         sys.exit(2)
+
+    # the CPU baseline first: its process pool forks before the GPU is touched
+    cpu = None if (args.no_cpu_baseline or args.screen_only) else cpu_baseline(args.cpu_per_core)
 
     import torch
     from gwfast_amd import _engine as E, _native as N, catalog
@@ -289,35 +422,8 @@ def main():
         del src, dst
     wf, net = build_net()
     n = args.n_events
+    F, rows, wall, kern_ms = fisher_steps(wf, net, n, args.steps, args.warmup)
     ev = catalog.draw_bbh(n, seed=0)
-    rows = E.to_dev(E.param_rows(ev, False))
-    sigs = list(net.signals.values())
-    F = torch.empty((11, 11, n), dtype=torch.float64, device=rows.device)
-    ws, nb = E.workspace(n)
-    dets = S._dets(sigs)
-    wfc = wf.c_struct()
-    st = N.stream_ptr()
-    L = N.lib()
-
-    def step(t_fisher=None):
-        N.check(L.gwf_prepare(wfc, dets, len(sigs), N.ptr(rows), rows.stride(0), n, RES, N.ptr(ws), nb, st), L)
-        if t_fisher is not None:
-            t_fisher[0].record()
-        N.check(L.gwf_fisher_prepared(wfc, dets, len(sigs), n, RES, -1, N.ptr(F), N.ptr(ws), nb, st), L)
-        if t_fisher is not None:
-            t_fisher[1].record()
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
     ms_per_step = 1e3 * wall / args.steps
     value = n * args.steps / wall
     achieved = GRAM_FLOP_PER_FISHER * n / (kern_ms * 1e-3) / 1e12
@@ -337,6 +443,10 @@ def main():
                      'kernel': 'k_fisher_pq<IMRPhenomD,4,11>', 'kernel_ms': kern_ms,
                      'note': 'FP64 dense peak; algorithmic flops = Gram 4 P^2 N_f N_ifo per Fisher'},
     }
+    if not args.no_c2 and n != 100_000:
+        out['c2_1e5'] = c2_leg(wf, net)
+    out['cov'] = cov_leg(F)
+    out['cov']['share_of_fisher_step'] = out['cov']['cov_invert_ms'] / ms_per_step
     if not args.no_plain:
         pv, Fp = plain_path(net, rows, n)
         Fp = Fp.cpu().numpy()
@@ -357,11 +467,11 @@ def main():
         out['per_arm_return_all'] = per_arm_leg(net, rows, n)
         out['ragged_c3'] = ragged_leg()
     if not args.no_screen:
-        del F, ws
+        del F
         out['screen_1e6'] = screen_leg(net, n_cat=args.screen_events, n_train=args.screen_train,
                                        epochs=args.screen_epochs)
-    if not args.no_cpu_baseline:
-        out['cpu_baseline'] = cpu_baseline(args.cpu_sample)
+    if cpu is not None:
+        out['cpu_baseline'] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -84,6 +84,30 @@ def param_rows(evParams, is_tidal, is_eccentric=False):
     return np.ascontiguousarray(np.array(rows, dtype=np.float64))
 
 
+def n_rows(wf):
+    """Parameter rows a model's kernels read: 11, + 2 tidal, + 1 eccentric."""
+    return 11 + 2 * int(bool(wf.is_tidal)) + int(bool(getattr(wf, 'is_eccentric', False)))
+
+
+def check_rows(rows, wf=None, need=None):
+    """Rejects parameter rows the kernels would read out of bounds: a 2-D
+    float64 tensor with unit stride along events, at least n_rows(wf) (or
+    need) rows, on the GPU.  The C ABI takes the row stride but not the row
+    count, so this is the check."""
+    t = torch()
+    if not isinstance(rows, t.Tensor) or rows.dim() != 2:
+        raise ValueError('parameter rows must be a 2-D (P_rows, N_ev) tensor')
+    if rows.dtype != t.float64:
+        raise ValueError('parameter rows must be float64, got %s' % rows.dtype)
+    need = n_rows(wf) if need is None else need
+    if rows.shape[0] < need:
+        raise ValueError('the model needs %d parameter rows, got %d' % (need, rows.shape[0]))
+    if rows.shape[1] > 1 and rows.stride(1) != 1:
+        raise ValueError('parameter rows must be contiguous along events')
+    if rows.device.type != 'cuda':
+        raise ValueError('parameter rows must be on the GPU')
+
+
 def call(fn, *args):
     L = N.lib()
     N.check(getattr(L, fn)(*args), L)

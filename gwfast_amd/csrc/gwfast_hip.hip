@@ -765,354 +765,463 @@ __global__ __launch_bounds__(256) void k_vjp(const double* __restrict__ J, const
   if (lane == 0) g[p * n_ev + e] = (p == 7) ? acc * tc_scale : acc;
 }
 
-// ------------------------------------------------- covariance (per thread)
+// ------------------------------------------- covariance (one wave per event)
+// fisherTools' per-event loops (CovMatr, CheckFisher) run one 64-lane wave
+// per event.  The event's P x P matrices live in LDS (row pitch CLD, rows and
+// columns P..PMAX-1 kept zero); lanes work on matrix entries, rows or
+// columns, and the workgroup is that one wave, so __syncthreads() orders
+// the LDS steps without tying events together.  No private arrays: nothing
+// goes to scratch memory.
 constexpr int PMAX = 16;
+constexpr int CLD = PMAX + 1;
+constexpr int CMAT = PMAX * CLD;
+constexpr double MP_TOL = 2.220446049250313e-16;  // 2^-52: mpmath's tolerance at 53 bits
 
-// Cyclic Jacobi eigen-decomposition of a symmetric P x P matrix (row-major
-// in A, destroyed); eigenvalues to w, eigenvectors to the columns of V.
-__device__ void jacobi_eig(double* A, int P, double* w, double* V) {
-  for (int i = 0; i < P; ++i)
-    for (int j = 0; j < P; ++j) V[i * P + j] = (i == j) ? 1.0 : 0.0;
+#define AT(M, i, j) (M)[(i) * CLD + (j)]
+
+// Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8): this
+// mapping gives each XCD one contiguous range of events, so the (P, P, N)
+// loads and stores of neighbouring events meet in that XCD's L2.  The grid
+// is a multiple of 8 blocks.
+__device__ __forceinline__ int64_t xcd_event() {
+  const int64_t b = blockIdx.x;
+  return (b & 7) * (int64_t)(gridDim.x >> 3) + (b >> 3);
+}
+
+inline unsigned xcd_grid(int64_t n_ev) { return (unsigned)(((n_ev + 7) / 8) * 8); }
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Round-robin pairing of the indices 0..m (m odd): in round r index m meets
+// r and every other index i meets (2r - i) mod m; m rounds pair every index
+// with every other one once.
+__device__ __forceinline__ int rr_partner(int i, int r, int m) {
+  if (i == m) return r;
+  if (i == r) return m;
+  return (2 * r - i + 2 * m) % m;
+}
+
+// Jacobi eigen-decomposition of the symmetric P x P matrix in E (destroyed):
+// cyclic Jacobi in round-robin order, the P/2 disjoint rotations of a round
+// applied together as E <- J^T E J and V <- V J (each rotation the classical
+// one that zeroes E_pq: theta = (E_qq - E_pp) / (2 E_pq),
+// t = sgn(theta) / (|theta| + sqrt(theta^2 + 1))), sweeps until the
+// off-diagonal mass is below 1e-36 of the diagonal's (at most 60).  On
+// return w[0..P) holds the eigenvalues in ascending order (mpmath.eigh's
+// order; ties and NaNs keep index order, NaNs last) and the columns of V the
+// eigenvectors.  E, E2, V, V2: LDS matrices with zero rows / columns beyond
+// P (an odd P pairs with the zero index P); the four pointers are swapped as
+// the rounds alternate between them.  cc, ss: PMAX doubles of LDS.
+__device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, int P, double* w, double* cc,
+                            double* ss, int lane) {
+  for (int t = lane; t < P * P; t += 64) {
+    const int i = t / P, j = t - (t / P) * P;
+    AT(V, i, j) = (i == j) ? 1.0 : 0.0;
+  }
+  const int m = P + (P & 1) - 1;
+  __syncthreads();
   for (int sweep = 0; sweep < 60; ++sweep) {
     double off = 0.0, diag = 0.0;
-    for (int i = 0; i < P; ++i) {
-      diag += A[i * P + i] * A[i * P + i];
-      for (int j = i + 1; j < P; ++j) off += A[i * P + j] * A[i * P + j];
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      const double a = AT(E, i, j);
+      if (i == j) diag += a * a;
+      else if (i < j) off += a * a;
     }
-    if (!(off > 1e-36 * diag)) break;
-    for (int p = 0; p < P - 1; ++p) {
-      for (int q = p + 1; q < P; ++q) {
-        const double apq = A[p * P + q];
-        if (apq == 0.0) continue;
-        const double app = A[p * P + p], aqq = A[q * P + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < P; ++k) {
-          const double akp = A[k * P + p], akq = A[k * P + q];
-          A[k * P + p] = c * akp - s * akq;
-          A[k * P + q] = s * akp + c * akq;
+    off = wave_sum(off);
+    diag = wave_sum(diag);
+    if (!(off > 1e-36 * diag)) break;  // the same sums on every lane: uniform
+    for (int r = 0; r < m; ++r) {
+      if (lane < P) {
+        const int j = rr_partner(lane, r, m);
+        double c = 1.0, s = 0.0;
+        if (j < P) {
+          const int p = min(lane, j), q = max(lane, j);
+          const double apq = AT(E, p, q);
+          if (apq != 0.0) {
+            const double app = AT(E, p, p), aqq = AT(E, q, q);
+            const double theta = (aqq - app) / (2.0 * apq);
+            const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            c = 1.0 / sqrt(tt * tt + 1.0);
+            s = tt * c;
+          }
+          if (lane == p) s = -s;  // J_pp = c, J_qp = -s, J_pq = s, J_qq = c
         }
-        for (int k = 0; k < P; ++k) {
-          const double apk = A[p * P + k], aqk = A[q * P + k];
-          A[p * P + k] = c * apk - s * aqk;
-          A[q * P + k] = s * apk + c * aqk;
+        cc[lane] = c;
+        ss[lane] = s;
+      }
+      __syncthreads();
+      for (int t = lane; t < P * P; t += 64) {
+        const int i = t / P, j = t - (t / P) * P;
+        const int ip = rr_partner(i, r, m), jp = rr_partner(j, r, m);
+        const double cj = cc[j], sj = ss[j];
+        if (i <= j) {  // (J^T E J)_ij, mirrored: E stays exactly symmetric
+          const double ci = cc[i], si = ss[i];
+          const double v = ci * (cj * AT(E, i, j) + sj * AT(E, i, jp)) + si * (cj * AT(E, ip, j) + sj * AT(E, ip, jp));
+          AT(E2, i, j) = v;
+          AT(E2, j, i) = v;
         }
-        for (int k = 0; k < P; ++k) {
-          const double vkp = V[k * P + p], vkq = V[k * P + q];
-          V[k * P + p] = c * vkp - s * vkq;
-          V[k * P + q] = s * vkp + c * vkq;
-        }
+        AT(V2, i, j) = AT(V, i, j) * cj + AT(V, i, jp) * sj;
       }
+      __syncthreads();
+      double* t1 = E;
+      E = E2;
+      E2 = t1;
+      t1 = V;
+      V = V2;
+      V2 = t1;
     }
   }
-  for (int i = 0; i < P; ++i) w[i] = A[i * P + i];
-  // ascending order (mpmath.eigh order)
-  for (int i = 1; i < P; ++i) {
-    for (int j = i; j > 0 && w[j - 1] > w[j]; --j) {
-      const double t = w[j - 1];
-      w[j - 1] = w[j];
-      w[j] = t;
-      for (int k = 0; k < P; ++k) {
-        const double v = V[k * P + j - 1];
-        V[k * P + j - 1] = V[k * P + j];
-        V[k * P + j] = v;
-      }
-    }
-  }
-}
-
-// mpmath's LU_decomp (the library behind fisherTools' 'inv' and 'lu'): row
-// pivot maximising |A[k][j]| / sum_{l>=j} |A[k][l]|; numerically singular
-// (false) when a row sum or a pivot is <= ||A||_1 * 2^-52.  In place: the unit
-// lower factor below the diagonal, the upper factor on and above; swaps in p.
-__device__ bool lu_pivoted(double* A, int P, int* p) {
-  double n1 = 0.0;
-  for (int j = 0; j < P; ++j) {
-    double s = 0.0;
-    for (int i = 0; i < P; ++i) s += fabs(A[i * P + j]);
-    n1 = fmax(n1, s);
-  }
-  const double tol = n1 * 2.220446049250313e-16;
-  for (int j = 0; j < P - 1; ++j) {
-    double biggest = 0.0;
-    int pj = -1;
-    for (int k = j; k < P; ++k) {
-      double s = 0.0;
-      for (int l = j; l < P; ++l) s += fabs(A[k * P + l]);
-      if (!(s > tol)) return false;
-      const double cur = (1.0 / s) * fabs(A[k * P + j]);
-      if (cur > biggest) {
-        biggest = cur;
-        pj = k;
-      }
-    }
-    if (pj < 0) return false;
-    p[j] = pj;
-    for (int l = 0; l < P; ++l) {
-      const double t = A[j * P + l];
-      A[j * P + l] = A[pj * P + l];
-      A[pj * P + l] = t;
-    }
-    if (!(fabs(A[j * P + j]) > tol)) return false;
-    for (int i = j + 1; i < P; ++i) {
-      A[i * P + j] /= A[j * P + j];
-      for (int k = j + 1; k < P; ++k) A[i * P + k] -= A[i * P + j] * A[j * P + k];
-    }
-  }
-  return fabs(A[(P - 1) * P + P - 1]) > tol;
-}
-
-// Inverse of a unit lower (unit = true) or an upper triangular matrix held
-// in T (row-major), to R.
-__device__ void tri_inverse(const double* T, int P, bool lower_unit, double* R) {
-  for (int i = 0; i < P * P; ++i) R[i] = 0.0;
-  if (lower_unit) {
+  if (lane < P) {  // rank of each eigenvalue; the sorted eigenvectors go to V2
+    const double wi = AT(E, lane, lane);
+    const bool ni = isnan(wi);
+    int rk = 0;
     for (int j = 0; j < P; ++j) {
-      R[j * P + j] = 1.0;
-      for (int i = j + 1; i < P; ++i) {
-        double s = 0.0;
-        for (int k = j; k < i; ++k) s -= T[i * P + k] * R[k * P + j];
-        R[i * P + j] = s;
-      }
+      const double wj = AT(E, j, j);
+      const bool nj = isnan(wj);
+      const bool before = ni ? (nj ? j < lane : true) : (!nj && (wj < wi || (wj == wi && j < lane)));
+      rk += before ? 1 : 0;
     }
-  } else {
-    for (int j = P - 1; j >= 0; --j) {
-      R[j * P + j] = 1.0 / T[j * P + j];
-      for (int i = j - 1; i >= 0; --i) {
-        double s = 0.0;
-        for (int k = i + 1; k <= j; ++k) s -= T[i * P + k] * R[k * P + j];
-        R[i * P + j] = s / T[i * P + i];
-      }
-    }
+    w[rk] = wi;
+    for (int k = 0; k < P; ++k) AT(V2, k, rk) = AT(V, k, lane);
   }
+  __syncthreads();
+  double* t1 = V;
+  V = V2;
+  V2 = t1;
 }
 
-// fisherTools.CovMatr per event (fisherTools.py:53-196), one thread each.
-// meth / alt: GWF_INV_* ids.  Steps: Jacobi scaling ws = 1/sqrt(F_ii) (a zero
-// diagonal skips it, a negative one gives NaN); eigenvalues of the scaled
+// fisherTools.CovMatr per event (fisherTools.py:53-196).  meth / alt:
+// GWF_INV_* ids.  Steps: Jacobi scaling ws = 1/sqrt(F_ii) (a zero diagonal
+// skips it, a negative one or any NaN gives NaN); eigenvalues of the scaled
 // matrix A (condition number); a negative eigenvalue or a failed Cholesky
 // switches 'cho' to alt; then the chosen inverse of A, symmetrised and
 // unscaled; eps = max|C F - I| against F, or against the truncated scaled
 // matrix when 'svd' truncates (as the reference stores it, 128-136).
-__global__ void k_cov(const double* __restrict__ F, int P, int64_t n_ev, int meth_in, int alt, int truncate,
-                      double svals_thresh, double cond_max, double* __restrict__ C, double* __restrict__ eps,
-                      double* __restrict__ cond, int* __restrict__ method) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_ev) return;
-  double Fm[PMAX * PMAX], A[PMAX * PMAX], V[PMAX * PMAX], L[PMAX * PMAX], w[PMAX], ws[PMAX];
-  bool allnan = true, anynan = false;
-  for (int i = 0; i < P * P; ++i) {
-    Fm[i] = F[(int64_t)i * n_ev + e];
-    const bool nn = isnan(Fm[i]);
-    allnan = allnan && nn;
-    anynan = anynan || nn;
+__global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P, int64_t n_ev, int meth_in, int alt,
+                                            int truncate, double svals_thresh, double cond_max, double* __restrict__ C,
+                                            double* __restrict__ eps, double* __restrict__ cond,
+                                            int* __restrict__ method) {
+  __shared__ double sm[6 * CMAT];
+  __shared__ double vw[PMAX], vws[PMAX], vcc[PMAX], vss[PMAX];
+  __shared__ int vpiv[PMAX], vperm[PMAX];
+  const int64_t e = xcd_event();
+  if (e >= n_ev) return;  // the block is this one wave
+  const int lane = threadIdx.x;
+  double *Fm = sm, *A = sm + CMAT, *E = sm + 2 * CMAT, *E2 = sm + 3 * CMAT, *V = sm + 4 * CMAT, *V2 = sm + 5 * CMAT;
+  for (int t = lane; t < 6 * CMAT; t += 64) sm[t] = 0.0;
+  __syncthreads();
+  bool nan_l = false;
+  for (int t = lane; t < P * P; t += 64) {
+    const double v = F[(int64_t)t * n_ev + e];
+    AT(Fm, t / P, t - (t / P) * P) = v;
+    nan_l = nan_l || isnan(v);
   }
-  auto put_nan = [&](int m) {
-    for (int i = 0; i < P * P; ++i) C[(int64_t)i * n_ev + e] = NAN;
-    if (eps) eps[e] = NAN;
-    if (cond) cond[e] = NAN;
-    if (method) method[e] = m;
+  bool neg_l = false, zero_l = false;
+  __syncthreads();
+  if (lane < P) {
+    const double d = AT(Fm, lane, lane);
+    zero_l = d == 0.0;
+    neg_l = d < 0.0;
+  }
+  auto put_nan = [&]() {
+    for (int t = lane; t < P * P; t += 64) C[(int64_t)t * n_ev + e] = NAN;
+    if (lane == 0) {
+      if (eps) eps[e] = NAN;
+      if (cond) cond[e] = NAN;
+      if (method) method[e] = -1;
+    }
   };
-  if (allnan || anynan) {
-    put_nan(-1);
+  if (__ballot(nan_l || neg_l)) {
+    put_nan();
     return;
   }
   // Jacobi scaling ws = diag(1/sqrt(F_ii)); a zero diagonal skips it
   // (fisherTools.py:81-94)
-  bool reweight = true;
-  for (int i = 0; i < P; ++i) {
-    if (!(Fm[i * P + i] > 0.0)) {
-      if (Fm[i * P + i] == 0.0) {
-        reweight = false;
-      } else {
-        put_nan(-1);
-        return;
-      }
-    }
+  const bool reweight = __ballot(zero_l) == 0;
+  if (lane < P) vws[lane] = reweight ? 1.0 / sqrt(AT(Fm, lane, lane)) : 1.0;
+  __syncthreads();
+  for (int t = lane; t < P * P; t += 64) {
+    const int i = t / P, j = t - (t / P) * P;
+    const double a = vws[i] * AT(Fm, i, j) * vws[j];
+    AT(A, i, j) = a;
+    AT(E, i, j) = a;
   }
-  for (int i = 0; i < P; ++i) ws[i] = reweight ? 1.0 / sqrt(Fm[i * P + i]) : 1.0;
-  for (int i = 0; i < P; ++i)
-    for (int j = 0; j < P; ++j) A[i * P + j] = ws[i] * Fm[i * P + j] * ws[j];
-  for (int i = 0; i < P * P; ++i) L[i] = A[i];
-  jacobi_eig(L, P, w, V);  // L used as scratch
+  __syncthreads();
+  wave_jacobi(E, E2, V, V2, P, vw, vcc, vss, lane);
   double emax = 0.0, emin = INFINITY;
   bool neg = false;
   for (int i = 0; i < P; ++i) {
-    emax = fmax(emax, fabs(w[i]));
-    emin = fmin(emin, fabs(w[i]));
-    neg = neg || (w[i] < 0.0);
+    emax = fmax(emax, fabs(vw[i]));
+    emin = fmin(emin, fabs(vw[i]));
+    neg = neg || (vw[i] < 0.0);
   }
   const double cnd = emax / emin;
-  if (cond) cond[e] = cnd;
+  if (cond && lane == 0) cond[e] = cnd;
+  // E, E2, V2 are free from here; V holds the eigenvectors
   int m = meth_in;
   if (m == GWF_INV_CHO) {
     if (neg) {
       m = alt;
     } else {  // mpmath.cholesky: a pivot below 2^-52 is not positive-definite
-      for (int i = 0; i < P * P; ++i) L[i] = 0.0;
-      for (int j = 0; j < P && m == GWF_INV_CHO; ++j) {
-        double d = A[j * P + j];
-        for (int k = 0; k < j; ++k) d -= L[j * P + k] * L[j * P + k];
-        if (!(d >= 2.220446049250313e-16)) {
+      double* L = E;
+      for (int t = lane; t < P * P; t += 64) AT(L, t / P, t - (t / P) * P) = 0.0;
+      __syncthreads();
+      for (int j = 0; j < P; ++j) {
+        double d = AT(A, j, j);
+        for (int k = 0; k < j; ++k) d -= AT(L, j, k) * AT(L, j, k);
+        if (!(d >= MP_TOL)) {  // every lane computed the same d
           m = alt;
           break;
         }
         const double ljj = sqrt(d);
-        L[j * P + j] = ljj;
-        for (int i = j + 1; i < P; ++i) {
-          double s = A[i * P + j];
-          for (int k = 0; k < j; ++k) s -= L[i * P + k] * L[j * P + k];
-          L[i * P + j] = s / ljj;
+        if (lane == j) AT(L, j, j) = ljj;
+        if (lane > j && lane < P) {
+          double s = AT(A, lane, j);
+          for (int k = 0; k < j; ++k) s -= AT(L, lane, k) * AT(L, j, k);
+          AT(L, lane, j) = s / ljj;
         }
+        __syncthreads();
       }
     }
   }
-  double Cc[PMAX * PMAX];
-  bool trunc_used = false;
+  double* Cc;
   if (m == GWF_INV_CHO) {
-    // c = L^{-1} (lower triangular), cc = c^T c
-    double Ci[PMAX * PMAX];
-    for (int i = 0; i < P * P; ++i) Ci[i] = 0.0;
-    for (int j = 0; j < P; ++j) {
-      Ci[j * P + j] = 1.0 / L[j * P + j];
+    // Ci = L^{-1} (lower triangular, one column per lane), Cc = Ci^T Ci
+    const double* L = E;
+    double* Ci = E2;
+    if (lane < P) {
+      const int j = lane;
+      for (int i = 0; i < j; ++i) AT(Ci, i, j) = 0.0;
+      AT(Ci, j, j) = 1.0 / AT(L, j, j);
       for (int i = j + 1; i < P; ++i) {
         double s = 0.0;
-        for (int k = j; k < i; ++k) s -= L[i * P + k] * Ci[k * P + j];
-        Ci[i * P + j] = s / L[i * P + i];
+        for (int k = j; k < i; ++k) s -= AT(L, i, k) * AT(Ci, k, j);
+        AT(Ci, i, j) = s / AT(L, i, i);
       }
     }
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < P; ++k) s += Ci[k * P + i] * Ci[k * P + j];
-        Cc[i * P + j] = s;
-      }
+    __syncthreads();
+    Cc = V2;
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      double s = 0.0;
+      for (int k = 0; k < P; ++k) s += AT(Ci, k, i) * AT(Ci, k, j);
+      AT(Cc, i, j) = s;
+    }
   } else if (m == GWF_INV_SVD || m == GWF_INV_SVD_REG) {
     // SVD of the symmetric A from its eigenpairs: singular values |w_k|,
     // A^+ = sum_k sign(w_k) v_k v_k^T / |w_k|
     double smax = 0.0;
-    for (int k = 0; k < P; ++k) smax = fmax(smax, fabs(w[k]));
+    for (int k = 0; k < P; ++k) smax = fmax(smax, fabs(vw[k]));
     const bool trunc = (m == GWF_INV_SVD) && truncate && fabs(cnd) > cond_max;
-    double sinv[PMAX], st[PMAX];
-    for (int k = 0; k < P; ++k) {
-      const double s = fabs(w[k]), sg = w[k] < 0.0 ? -1.0 : 1.0;
+    bool zero_sv = false;
+    double* sinv = vcc;
+    double* st = vss;
+    if (lane < P) {
+      const double s = fabs(vw[lane]), sg = vw[lane] < 0.0 ? -1.0 : 1.0;
       if (m == GWF_INV_SVD_REG) {
-        sinv[k] = s > svals_thresh ? sg / s : 0.0;  // singular values <= thresh left out
+        sinv[lane] = s > svals_thresh ? sg / s : 0.0;  // singular values <= thresh left out
       } else if (trunc) {
         const bool keep = s / smax > svals_thresh;
-        sinv[k] = sg / (keep ? s : smax * svals_thresh);
-        st[k] = sg * (keep ? s : smax * svals_thresh);
+        sinv[lane] = sg / (keep ? s : smax * svals_thresh);
+        st[lane] = sg * (keep ? s : smax * svals_thresh);
       } else {
-        if (s == 0.0) {  // 1/0: no finite inverse
-          put_nan(-1);
-          return;
-        }
-        sinv[k] = sg / s;
+        zero_sv = s == 0.0;  // 1/0: no finite inverse
+        sinv[lane] = zero_sv ? 0.0 : sg / s;
       }
     }
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] * sinv[k];
-        Cc[i * P + j] = s;
-      }
-    if (trunc) {  // the truncated (scaled) Fisher the inversion error is taken against
-      trunc_used = true;
-      for (int i = 0; i < P; ++i)
-        for (int j = i; j < P; ++j) {
-          double s = 0.0;
-          for (int k = 0; k < P; ++k) s += V[i * P + k] * V[j * P + k] * st[k];
-          Fm[i * P + j] = s;
-          Fm[j * P + i] = s;
-        }
-    }
-  } else {  // 'inv' (mpmath inverse) or 'lu' (U^{-1} (P L)^{-1}, fisherTools.py:156-161)
-    int piv[PMAX];
-    for (int i = 0; i < P * P; ++i) L[i] = A[i];
-    if (!lu_pivoted(L, P, piv)) {
-      put_nan(-1);
+    if (__ballot(zero_sv)) {
+      put_nan();
       return;
     }
-    double Ui[PMAX * PMAX], Li[PMAX * PMAX];
-    tri_inverse(L, P, false, Ui);
-    tri_inverse(L, P, true, Li);
-    // permutation of the row swaps: row r of Pm has its 1 in column perm[r]
-    int perm[PMAX];
-    for (int r = 0; r < P; ++r) perm[r] = r;
-    for (int k = 0; k < P - 1; ++k) {
-      const int t = perm[k];
-      perm[k] = perm[piv[k]];
-      perm[piv[k]] = t;
+    __syncthreads();
+    Cc = E;
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      double s = 0.0;
+      for (int k = 0; k < P; ++k) s += AT(V, i, k) * AT(V, j, k) * sinv[k];
+      AT(Cc, i, j) = s;
+      if (trunc && i <= j) {  // the truncated (scaled) Fisher the inversion error is taken against
+        double u = 0.0;
+        for (int k = 0; k < P; ++k) u += AT(V, i, k) * AT(V, j, k) * st[k];
+        AT(Fm, i, j) = u;
+        AT(Fm, j, i) = u;
+      }
     }
+  } else {  // 'inv' (mpmath inverse) or 'lu' (U^{-1} (P L)^{-1}, fisherTools.py:156-161)
+    // mpmath's LU_decomp: row pivot maximising |L[k][j]| / sum_{l>=j} |L[k][l]|;
+    // numerically singular (NaN) when a row sum or a pivot is <= ||A||_1 2^-52
+    double* L = E;
+    for (int t = lane; t < P * P; t += 64) AT(L, t / P, t - (t / P) * P) = AT(A, t / P, t - (t / P) * P);
+    __syncthreads();
+    double n1 = 0.0;
+    if (lane < P) {
+      for (int i = 0; i < P; ++i) n1 += fabs(AT(L, i, lane));
+    }
+    const double tol = wave_max(n1) * MP_TOL;
+    bool ok = true;
+    for (int j = 0; j < P - 1 && ok; ++j) {
+      bool bad = false;
+      if (lane >= j && lane < P) {
+        double s = 0.0;
+        for (int l = j; l < P; ++l) s += fabs(AT(L, lane, l));
+        bad = !(s > tol);
+        vcc[lane] = (1.0 / s) * fabs(AT(L, lane, j));
+      }
+      __syncthreads();
+      if (__ballot(bad)) {
+        ok = false;
+        break;
+      }
+      double biggest = 0.0;
+      int pj = -1;
+      for (int k = j; k < P; ++k) {
+        if (vcc[k] > biggest) {
+          biggest = vcc[k];
+          pj = k;
+        }
+      }
+      if (pj < 0) {
+        ok = false;
+        break;
+      }
+      if (lane == 0) vpiv[j] = pj;
+      if (lane < P) {
+        const double t = AT(L, j, lane);
+        AT(L, j, lane) = AT(L, pj, lane);
+        AT(L, pj, lane) = t;
+      }
+      __syncthreads();
+      if (!(fabs(AT(L, j, j)) > tol)) {
+        ok = false;
+        break;
+      }
+      if (lane > j && lane < P) {
+        const double lij = AT(L, lane, j) / AT(L, j, j);
+        AT(L, lane, j) = lij;
+        for (int k = j + 1; k < P; ++k) AT(L, lane, k) -= lij * AT(L, j, k);
+      }
+      __syncthreads();
+    }
+    if (!ok || !(fabs(AT(L, P - 1, P - 1)) > tol)) {
+      put_nan();
+      return;
+    }
+    // Ui = U^{-1} and Li = (unit L)^{-1}, one column per lane
+    double *Ui = E2, *Li = V2, *X = V;
+    if (lane < P) {
+      const int j = lane;
+      for (int i = j + 1; i < P; ++i) AT(Ui, i, j) = 0.0;
+      AT(Ui, j, j) = 1.0 / AT(L, j, j);
+      for (int i = j - 1; i >= 0; --i) {
+        double s = 0.0;
+        for (int k = i + 1; k <= j; ++k) s -= AT(L, i, k) * AT(Ui, k, j);
+        AT(Ui, i, j) = s / AT(L, i, i);
+      }
+      for (int i = 0; i < j; ++i) AT(Li, i, j) = 0.0;
+      AT(Li, j, j) = 1.0;
+      for (int i = j + 1; i < P; ++i) {
+        double s = 0.0;
+        for (int k = j; k < i; ++k) s -= AT(L, i, k) * AT(Li, k, j);
+        AT(Li, i, j) = s;
+      }
+    }
+    if (lane == 0) {  // permutation of the row swaps: row r of Pm has its 1 in column perm[r]
+      for (int r = 0; r < P; ++r) vperm[r] = r;
+      for (int k = 0; k < P - 1; ++k) {
+        const int t = vperm[k];
+        vperm[k] = vperm[vpiv[k]];
+        vperm[vpiv[k]] = t;
+      }
+    }
+    __syncthreads();
     // X = L^{-1} M: 'inv' takes M = Pm (the inverse of A = Pm^T L U), 'lu'
     // takes M = Pm^T, as the reference's U^{-1} (Pm L)^{-1} does
-    double X[PMAX * PMAX];
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0.0;
-        if (m == GWF_INV_INV) {
-          for (int r = 0; r < P; ++r)
-            if (perm[r] == j) s = Li[i * P + r];
-        } else {
-          s = Li[i * P + perm[j]];
-        }
-        X[i * P + j] = s;
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      double s = 0.0;
+      if (m == GWF_INV_INV) {
+        for (int r = 0; r < P; ++r)
+          if (vperm[r] == j) s = AT(Li, i, r);
+      } else {
+        s = AT(Li, i, vperm[j]);
       }
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < P; ++k) s += Ui[i * P + k] * X[k * P + j];
-        Cc[i * P + j] = s;
-      }
-  }
-  for (int i = 0; i < P; ++i)
-    for (int j = 0; j < P; ++j) {
-      const double s = 0.5 * (Cc[i * P + j] + Cc[j * P + i]);
-      C[(int64_t)(i * P + j) * n_ev + e] = ws[i] * s * ws[j];
-      A[i * P + j] = ws[i] * s * ws[j];
+      AT(X, i, j) = s;
     }
-  if (eps) {
-    double mx = 0.0;
-    for (int i = 0; i < P; ++i)
-      for (int j = 0; j < P; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < P; ++k) s += A[i * P + k] * Fm[k * P + j];
-        mx = fmax(mx, fabs(s - (i == j ? 1.0 : 0.0)));
-      }
-    eps[e] = mx;
+    __syncthreads();
+    Cc = V2;
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      double s = 0.0;
+      for (int k = 0; k < P; ++k) s += AT(Ui, i, k) * AT(X, k, j);
+      AT(Cc, i, j) = s;
+    }
   }
-  (void)trunc_used;
-  if (method) method[e] = m;
+  __syncthreads();
+  for (int t = lane; t < P * P; t += 64) {
+    const int i = t / P, j = t - (t / P) * P;
+    const double s = 0.5 * (AT(Cc, i, j) + AT(Cc, j, i));
+    const double c = vws[i] * s * vws[j];
+    C[(int64_t)t * n_ev + e] = c;
+    AT(A, i, j) = c;
+  }
+  if (eps) {
+    __syncthreads();
+    double mx = 0.0;
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      double s = 0.0;
+      for (int k = 0; k < P; ++k) s += AT(A, i, k) * AT(Fm, k, j);
+      mx = fmax(mx, fabs(s - (i == j ? 1.0 : 0.0)));
+    }
+    mx = wave_max(mx);
+    if (lane == 0) eps[e] = mx;
+  }
+  if (method && lane == 0) method[e] = m;
 }
 
-__global__ void k_check(const double* __restrict__ F, int P, int64_t n_ev, double* __restrict__ evals,
-                        double* __restrict__ evecs, double* __restrict__ cond) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// fisherTools.CheckFisher per event (fisherTools.py:216-279): eigenvalues
+// (ascending), eigenvectors (evecs[e][i][k] = component i of vector k) and
+// the condition number max|w| / min|w|; an all-NaN matrix gives NaN.
+__global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int P, int64_t n_ev,
+                                              double* __restrict__ evals, double* __restrict__ evecs,
+                                              double* __restrict__ cond) {
+  __shared__ double sm[4 * CMAT];
+  __shared__ double vw[PMAX], vcc[PMAX], vss[PMAX];
+  const int64_t e = xcd_event();
   if (e >= n_ev) return;
-  double A[PMAX * PMAX], V[PMAX * PMAX], w[PMAX];
-  bool allnan = true;
-  for (int i = 0; i < P * P; ++i) {
-    A[i] = F[(int64_t)i * n_ev + e];
-    allnan = allnan && isnan(A[i]);
+  const int lane = threadIdx.x;
+  double *E = sm, *E2 = sm + CMAT, *V = sm + 2 * CMAT, *V2 = sm + 3 * CMAT;
+  for (int t = lane; t < 4 * CMAT; t += 64) sm[t] = 0.0;
+  __syncthreads();
+  bool num_l = false;
+  for (int t = lane; t < P * P; t += 64) {
+    const double v = F[(int64_t)t * n_ev + e];
+    AT(E, t / P, t - (t / P) * P) = v;
+    num_l = num_l || !isnan(v);
   }
-  if (allnan) {
-    for (int i = 0; i < P; ++i) evals[e * P + i] = NAN;
+  if (__ballot(num_l) == 0) {
+    for (int t = lane; t < P; t += 64) evals[e * P + t] = NAN;
     if (evecs)
-      for (int i = 0; i < P * P; ++i) evecs[e * P * P + i] = NAN;
-    cond[e] = NAN;
+      for (int t = lane; t < P * P; t += 64) evecs[e * P * P + t] = NAN;
+    if (lane == 0) cond[e] = NAN;
     return;
   }
-  jacobi_eig(A, P, w, V);
+  __syncthreads();
+  wave_jacobi(E, E2, V, V2, P, vw, vcc, vss, lane);
   double emax = 0.0, emin = INFINITY;
   for (int i = 0; i < P; ++i) {
-    evals[e * P + i] = w[i];
-    emax = fmax(emax, fabs(w[i]));
-    emin = fmin(emin, fabs(w[i]));
+    emax = fmax(emax, fabs(vw[i]));
+    emin = fmin(emin, fabs(vw[i]));
   }
+  for (int t = lane; t < P; t += 64) evals[e * P + t] = vw[t];
   if (evecs)
-    for (int i = 0; i < P * P; ++i) evecs[e * P * P + i] = V[i];
-  cond[e] = emax / emin;
+    for (int t = lane; t < P * P; t += 64) evecs[e * P * P + t] = AT(V, t / P, t - (t / P) * P);
+  if (lane == 0) cond[e] = emax / emin;
 }
 
 __global__ void k_inv_err(const double* __restrict__ F, const double* __restrict__ C, int P, int64_t n_ev,
@@ -1853,7 +1962,7 @@ int gwf_cov_invert_ex(const double* F, int32_t P, int64_t n_ev, int32_t inv_meth
     return fail(-EINVAL, "alt_method has to be different from invMethodIn");
   if (n_ev == 0) return 0;
   if (!F || !cov_out) return fail(-EINVAL, "NULL array");
-  hipLaunchKernelGGL(k_cov, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
+  hipLaunchKernelGGL(k_cov, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
                      (int)inv_method, (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out,
                      eps_out, cond_out, (int*)method_out);
   return hip_check(hipGetLastError(), "gwf_cov_invert launch");
@@ -1881,7 +1990,7 @@ int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out
   if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
   if (n_ev == 0) return 0;
   if (!F || !evals_out || !cond_out) return fail(-EINVAL, "NULL array");
-  hipLaunchKernelGGL(k_check, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
+  hipLaunchKernelGGL(k_check, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
                      evals_out, evecs_out, cond_out);
   return hip_check(hipGetLastError(), "gwf_check_fisher launch");
 }

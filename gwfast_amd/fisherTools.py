@@ -4,8 +4,8 @@ Mirrors gwfast/fisherTools.py:32-196 (CovMatr, every inversion method),
 199-210 (compute_inversion_error), 216-279 (CheckFisher) and the helpers
 after them.  The reference runs a
 per-event mpmath loop at 53-bit precision and stores float128; here each
-event is inverted by one GPU thread in FP64 (gwf_cov_invert /
-gwf_check_fisher) and float64 is returned.
+event is inverted by one GPU wavefront in FP64, its matrices in LDS
+(gwf_cov_invert / gwf_check_fisher), and float64 is returned.
 """
 import numpy as np
 

@@ -146,7 +146,7 @@ def compute_errs(events, net, snr_th=12.0, params_fix=(), res=1000, screen=None,
     sig0 = next(iter(net.signals.values()))
     sig0._complete(events)
     wf = sig0.wf_model
-    rows = E.to_dev(E.param_rows(events, wf.is_tidal))
+    rows = sig0._rows(events)
     n = rows.shape[1]
     all_snrs = all_fishers = None
     if return_all:

@@ -28,7 +28,7 @@ import time
 import numpy as np
 
 from . import _native as N
-from ._engine import torch, device, to_dev, call
+from ._engine import torch, device, to_dev, call, check_rows
 
 NFEAT = 11
 KFEAT = 16
@@ -141,6 +141,7 @@ class SNRScreen(object):
     def raw_features(self, rows):
         T = torch()
         rows = rows.contiguous()
+        check_rows(rows, need=11)
         n = rows.shape[1]
         raw = T.empty((n, NFEAT), dtype=T.float32, device=device())
         call('gwf_snr_features', _ptr(rows), rows.stride(0), n, None, None, _ptr(raw), None, _stream())
@@ -150,6 +151,7 @@ class SNRScreen(object):
         """(n, 16) bf16 standardised features of device parameter rows."""
         T = torch()
         rows = rows.contiguous()
+        check_rows(rows, need=11)
         n = rows.shape[1]
         X = T.empty((n, KFEAT), dtype=T.bfloat16, device=device())
         call('gwf_snr_features', _ptr(rows), rows.stride(0), n, _ptr(self.mean), _ptr(self.istd), None, _ptr(X),
@@ -267,6 +269,7 @@ class SNRScreen(object):
     def _screen_call(self, rows, snr_lo, snr_hi, want_log, want_cls):
         T = torch()
         rows = rows.contiguous()
+        check_rows(rows, need=11)
         n = rows.shape[1]
         ls = T.empty(n, dtype=T.float32, device=device()) if want_log else None
         cl = T.empty(n, dtype=T.int8, device=device()) if want_cls else None
@@ -362,7 +365,7 @@ def screened_fisher(net, screen, rows, threshold=12.0, band=(8.0, 16.0), res=200
     counts = {'n_below': int(cls.numel()) - n_above - n_band, 'n_band': n_band, 'n_above': n_above,
               'n_exact_snr': n_band, 'n_fisher': int(cand.numel())}
     if cand.numel() == 0:
-        P = 11
+        P = next(iter(net.signals.values())).wf_model.nParams
         return dict(counts, idx=cand, F=T.empty((P, P, 0), dtype=T.float64, device=device()),
                     snr=T.empty(0, dtype=T.float64, device=device()), n_dropped_after_fisher=0)
     cand_rows = gather(rows, cand)

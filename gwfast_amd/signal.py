@@ -205,7 +205,7 @@ class GWSignal(object):
                  chi1y=0., chi2y=0., LambdaTilde=0., deltaLambda=0., ecc=0., rot=0., is_m1m2=False,
                  is_chi1chi2=False, is_prec_ang=False, is_Lam1Lam2=False, return_single_comp=None):
         """signal.py:485-655: complex strain on the grid f (N_f, N_ev) for the
-        arm at rotation rot (0, 60, 120 degrees); is_m1m2 reads (Mc, eta) as
+        arm at rotation rot (degrees; 0, 60, 120 are a triangle's arms); is_m1m2 reads (Mc, eta) as
         (m1, m2); return_single_comp in 'Ap', 'Ac', 'Psip', 'Psic', 'At',
         'Psit' returns that real component instead (629-652)."""
         if is_prec_ang:
@@ -226,17 +226,17 @@ class GWSignal(object):
         if self.wf_model.is_eccentric:
             ev['ecc'] = ecc
         ev = {k: np.atleast_1d(np.asarray(v, dtype=np.float64)) for k, v in ev.items()}
-        arm = _arm_of(self, rot)
+        det = rot_detector(self, rot)
         rows = self._rows(ev)
         fd = _grid_dev(f, rows.shape[1])
         if return_single_comp is not None:
             t = E.torch()
-            c = strain_components_device(self, rows, fd, arm)
+            c = strain_components_device(self, rows, fd, 0, det=det)
             Ap, Ac, Psi = c[0], c[1], c[2]
             out = {'Ap': lambda: Ap, 'Ac': lambda: Ac, 'Psip': lambda: Psi, 'Psic': lambda: Psi + np.pi * 0.5,
                    'At': lambda: t.hypot(Ap, Ac), 'Psit': lambda: Psi + t.atan2(Ac, Ap)}[return_single_comp]()
             return out.cpu().numpy()
-        h, _ = strain_device(self, rows, fd, arm, want_h=True, want_J=False, null_stream=False)
+        h, _ = strain_device(self, rows, fd, 0, want_h=True, want_J=False, det=det)
         hc = h.cpu().numpy()
         return hc[..., 0] + 1j * hc[..., 1]
 
@@ -244,9 +244,8 @@ class GWSignal(object):
         """signal.py:425-466: plus and cross amplitudes (Ap, Ac) of the arm at
         rotation rot on the grid f (N_f, N_ev)."""
         ev = self._complete(dict(evParams))
-        arm = _arm_of(self, rot)
         rows = self._rows(ev)
-        c = strain_components_device(self, rows, _grid_dev(f, rows.shape[1]), arm)
+        c = strain_components_device(self, rows, _grid_dev(f, rows.shape[1]), 0, det=rot_detector(self, rot))
         return c[0].cpu().numpy(), c[1].cpu().numpy()
 
     def WFOverlap(self, WF1, WF2, evParams1, evParams2, res=1000, return_separate=False, **kwargs):
@@ -309,6 +308,7 @@ def reparam_device(F, rows, use_m1m2, use_chi1chi2):
     (chiS, chiA) rows (signal.py:818-845), by the exact Jacobian."""
     if not use_m1m2 and use_chi1chi2:
         return F
+    E.check_rows(rows, need=11)
     E.call('gwf_fisher_reparam', N.ptr(rows), rows.stride(0), N.ptr(F), F.shape[0], F.shape[2], int(bool(use_m1m2)),
            int(not use_chi1chi2), N.stream_ptr())
     return F
@@ -411,6 +411,7 @@ def snr_sq_device(signals, rows, res):
     per event (ragged)."""
     t = E.torch()
     wf = signals[0].wf_model
+    E.check_rows(rows, wf)
     n = rows.shape[1]
     narms = sum(s.n_arms for s in signals)
     out = t.empty((narms, n), dtype=t.float64, device=rows.device)
@@ -429,6 +430,7 @@ def fisher_device(signals, wf, rows, res, arm=-1, out=None, lin=False):
     """Fused network Fisher (P, P, N_ev) device tensor; arm >= 0 selects one
     arm; lin: linearly spaced grids (FisherMatr spacing='lin')."""
     t = E.torch()
+    E.check_rows(rows, wf)
     n = rows.shape[1]
     P = wf.nParams
     if out is None:
@@ -449,6 +451,7 @@ def fisher_arms_device(signals, wf, rows, res, lin=False):
     order) from one gwf_fisher_arms launch: (n_arms, P, P, N_ev) device
     tensor; res: int, or one grid length per event (ragged)."""
     t = E.torch()
+    E.check_rows(rows, wf)
     n = rows.shape[1]
     P = wf.nParams
     n_arms = sum(s.n_arms for s in signals)
@@ -483,6 +486,7 @@ def fisher_masked_device(signals, wf, rows, res, words, lin=False):
     """Network Fisher (P, P, N_ev) device tensor summing, per event, the arms
     whose bit is set in words (one fused launch)."""
     t = E.torch()
+    E.check_rows(rows, wf)
     n = rows.shape[1]
     if np.ndim(res) != 0:
         raise NotImplementedError('duty-cycle masks need one grid length for all events')
@@ -497,6 +501,7 @@ def fisher_masked_device(signals, wf, rows, res, words, lin=False):
 
 def grid_device(sig, rows, res, lin=False):
     t = E.torch()
+    E.check_rows(rows, sig.wf_model)
     n = rows.shape[1]
     f = t.empty((res, n), dtype=t.float64, device=rows.device)
     S = t.empty((res, n), dtype=t.float64, device=rows.device)
@@ -506,11 +511,15 @@ def grid_device(sig, rows, res, lin=False):
     return f, S
 
 
-def _arm_of(sig, rot):
-    arm = int(round(rot / 60.))
-    if abs(rot - 60. * arm) > 1e-12 or arm not in range(sig.n_arms):
-        raise ValueError('rot must be 0 (L) or 0, 60, 120 degrees (T)')
-    return arm
+def rot_detector(sig, rot):
+    """The detector block of the single arm at rotation rot (degrees) from
+    the x-arm, for any rot: the pattern functions depend on the arm only
+    through xax + rot (signal.py:360-376), so this is arm 0 of the same site
+    with its x-arm turned by rot and no null stream."""
+    det = sig.c_struct()
+    det.compute2arms = 0
+    det.xax_deg = float(sig.det_xax) + float(rot)
+    return det
 
 
 def _grid_dev(f, n):
@@ -534,34 +543,40 @@ def _complete_for(wf, evParams):
     return ev
 
 
-def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True, wf=None):
+def strain_device(sig, rows, f, arm, want_h=True, want_J=True, null_stream=True, wf=None, det=None):
     """h (N_f, N_ev, 2) and J (P, N_ev, N_f, 2) device tensors for one arm
-    (of the model wf, the signal's own by default)."""
+    (of the model wf, the signal's own by default; det: a detector block
+    other than the signal's own, e.g. from rot_detector)."""
     t = E.torch()
     wf = sig.wf_model if wf is None else wf
+    E.check_rows(rows, wf)
     n = rows.shape[1]
     nf = f.shape[0]
     P = wf.nParams
     h = t.empty((nf, n, 2), dtype=t.float64, device=rows.device) if want_h else None
     J = t.empty((P, n, nf, 2), dtype=t.float64, device=rows.device) if want_J else None
     ws, nb = E.workspace(n)
-    det = sig.c_struct()
-    if not null_stream:
-        det.compute2arms = 0
+    if det is None:
+        det = sig.c_struct()
+        if not null_stream:
+            det.compute2arms = 0
     E.call('gwf_strain', wf.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n, N.ptr(f), nf,
            N.ptr(h), N.ptr(J), N.ptr(ws), nb, N.stream_ptr())
     return h, J
 
 
-def strain_components_device(sig, rows, f, arm):
-    """(Ap, Ac, Psi) (3, N_f, N_ev) device tensor for one physical arm."""
+def strain_components_device(sig, rows, f, arm, det=None):
+    """(Ap, Ac, Psi) (3, N_f, N_ev) device tensor for one physical arm
+    (det: a detector block other than the signal's own)."""
     t = E.torch()
+    E.check_rows(rows, sig.wf_model)
     n = rows.shape[1]
     nf = f.shape[0]
     out = t.empty((3, nf, n), dtype=t.float64, device=rows.device)
     ws, nb = E.workspace(n)
-    det = sig.c_struct()
-    det.compute2arms = 0
+    if det is None:
+        det = sig.c_struct()
+        det.compute2arms = 0
     E.call('gwf_strain_components', sig.wf_model.c_struct(), ctypes_byref(det), arm, N.ptr(rows), rows.stride(0), n,
            N.ptr(f), nf, N.ptr(out), N.ptr(ws), nb, N.stream_ptr())
     return out
@@ -569,6 +584,7 @@ def strain_components_device(sig, rows, f, arm):
 
 def strain_device_into(sig, rows, f, arm, J):
     """Derivative rows of one arm into a caller buffer J (P, N_ev, N_f, 2)."""
+    E.check_rows(rows, sig.wf_model)
     n = rows.shape[1]
     nf = f.shape[0]
     ws, nb = E.workspace(n)

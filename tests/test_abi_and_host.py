@@ -191,3 +191,86 @@ def test_ragged_grid_length_validation():
         S._res_arg(np.array([500, 600, 700]), 2, 'cpu')
     r0, r = S._res_arg(np.array([2, 500]), 2, 'cpu')
     assert r0 == 0 and r.tolist() == [2, 500]
+
+
+def test_tidal_conversions_equal_the_pinned_oracle():
+    """The host conversion (gwfastUtils.py:398-417) equals the oracle's,
+    which the per-body tidal-phase test pins."""
+    from oracle import waveforms as OW
+    eta = np.array([0.24, 0.2499, 0.2, 0.1])
+    L1, L2 = np.array([300., 10., 1500., 0.]), np.array([700., 2000., 100., 50.])
+    a = U.Lamt_delLam_from_Lam12(L1, L2, eta)
+    b = OW.lamt_dellam_from_lam12(L1, L2, eta)
+    assert np.allclose(a[0], b[0], rtol=1e-14, atol=1e-12) and np.allclose(a[1], b[1], rtol=1e-14, atol=1e-12)
+
+
+# ------------------------------------------------------- parameter rows
+def _models():
+    from gwfast_amd.waveforms import IMRPhenomD, IMRPhenomD_NRTidalv2, TaylorF2_RestrictedPN
+    return [(IMRPhenomD(), 11), (TaylorF2_RestrictedPN(), 11), (TaylorF2_RestrictedPN(is_tidal=True), 13),
+            (TaylorF2_RestrictedPN(is_eccentric=True), 12),
+            (TaylorF2_RestrictedPN(is_tidal=True, is_eccentric=True), 14), (IMRPhenomD_NRTidalv2(), 13)]
+
+
+def test_parameter_rows_match_each_model(monkeypatch):
+    """GWSignal builds exactly the rows its model's kernels read (11, +2
+    tidal, +1 eccentric), and the catalogue driver takes its rows from
+    GWSignal."""
+    import inspect
+    import torch
+    from gwfast_amd import _engine as E, forecast
+    from gwfast_amd.signal import GWSignal
+    monkeypatch.setattr(E, 'to_dev', lambda a, dtype=None: torch.as_tensor(np.ascontiguousarray(a)))
+    ev = catalog.draw_bns(3, seed=2)
+    ev['ecc'] = np.full(3, 0.05)
+    for wf, want in _models():
+        assert E.n_rows(wf) == want == wf.nParams
+        sig = GWSignal(wf, psd_path=glob.PSD_ET, detector_shape='L', verbose=False)
+        assert sig._rows(dict(ev)).shape == (want, 3)
+    assert 'sig0._rows(events)' in inspect.getsource(forecast.compute_errs)
+
+
+def test_parameter_row_checks_reject_bad_rows():
+    """The ABI takes the row stride but not the row count: the host rejects
+    rows the kernels would read past their end, and other layouts."""
+    import torch
+    from gwfast_amd import _engine as E
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    wf = TaylorF2_RestrictedPN(is_tidal=True)
+    with pytest.raises(ValueError, match='needs 13 parameter rows, got 11'):
+        E.check_rows(torch.zeros((11, 4), dtype=torch.float64), wf)
+    with pytest.raises(ValueError, match='float64'):
+        E.check_rows(torch.zeros((13, 4), dtype=torch.float32), wf)
+    with pytest.raises(ValueError, match='2-D'):
+        E.check_rows(torch.zeros(13, dtype=torch.float64), wf)
+    with pytest.raises(ValueError, match='contiguous along events'):
+        E.check_rows(torch.zeros((4, 13), dtype=torch.float64).t(), wf)
+    with pytest.raises(ValueError, match='on the GPU'):
+        E.check_rows(torch.zeros((13, 4), dtype=torch.float64), wf)
+    with pytest.raises(ValueError, match='needs 11 parameter rows, got 3'):
+        E.check_rows(torch.zeros((3, 4), dtype=torch.float64), need=11)
+
+
+def test_any_arm_rotation_is_a_turned_x_arm():
+    """GWstrain / GWAmplitudes at any rot (signal.py:485) run arm 0 of the
+    same site with its x-arm turned by rot: on the oracle, the strain at
+    rot = 37 deg equals the strain of the turned detector at rot = 0, for L
+    and T shapes."""
+    from gwfast_amd.signal import GWSignal, rot_detector
+    from gwfast_amd.waveforms import IMRPhenomD
+    from oracle import detector as OD, waveforms as OW
+    ev = catalog.draw_bbh(3, seed=4)
+    f = np.geomspace(5., 200., 30)[:, None] * np.ones((1, 3))
+    wf = OW.IMRPhenomD()
+    for shape, site in (('L', 'CE1Id'), ('T', 'ETS')):
+        d = glob.detectors[site]
+        a = OD.Detector(glob.PSD_ET, shape, d['lat'], d['long'], d['xax'], useEarthMotion=True)
+        b = OD.Detector(glob.PSD_ET, shape, d['lat'], d['long'], d['xax'] + 37., useEarthMotion=True)
+        ha, hb = a.gw_strain(wf, ev, f, rot=37.), b.gw_strain(wf, ev, f, rot=0.)
+        assert np.max(np.abs(ha - hb)) <= 1e-12 * np.max(np.abs(ha))
+        sig = GWSignal(IMRPhenomD(), psd_path=glob.PSD_ET, detector_shape=shape, det_lat=d['lat'],
+                       det_long=d['long'], det_xax=d['xax'], useEarthMotion=True, verbose=False)
+        # stand-in device tables: the block only records their addresses
+        sig._psd_dev = tuple(type('T', (), {'data_ptr': lambda self: 0})() for _ in range(2))
+        det = rot_detector(sig, 37.)
+        assert det.xax_deg == pytest.approx(d['xax'] + 37.) and det.compute2arms == 0

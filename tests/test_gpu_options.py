@@ -514,3 +514,52 @@ def test_tidal_eccentric_taylorf2(gpu):
     F0 = net.FisherMatr(dict(ev0), res=1000)
     Ft = hip_net(TaylorF2_RestrictedPN(is_tidal=True), NET_C1, True).FisherMatr(dict(ev0), res=1000)
     assert norm_err(F0[:13, :13], Ft) <= 1e-13
+
+
+def test_gwstrain_any_rotation(gpu):
+    """GWstrain and GWAmplitudes at rot = 37 deg (any angle, signal.py:485)
+    against the oracle's pattern functions at that rotation, on the ET
+    triangle and an L detector."""
+    wf, owf = _models('PD')
+    ev = bbh(4, seed=83)
+    for sites in (NET_C1[:1], NET_C1[1:2]):
+        sig = next(iter(hip_net(wf, sites, True).signals.values()))
+        od = oracle_net(sites, True)[0]
+        f, _ = od.grid(owf, ev, 300)
+        args = (ev['Mc'], ev['eta'], ev['dL'], ev['theta'], ev['phi'], ev['iota'], ev['psi'], ev['tcoal'],
+                ev['Phicoal'], ev['chi1z'], ev['chi2z'])
+        Ap, Ac, Psi = od.strain_components(owf, ev, f, rot=37.)
+        got_ap = sig.GWstrain(f, *args, rot=37., is_chi1chi2=True, return_single_comp='Ap')
+        got_ac = sig.GWstrain(f, *args, rot=37., is_chi1chi2=True, return_single_comp='Ac')
+        amax = np.max(np.hypot(Ap, Ac))
+        print(sites[0][0], 'rot 37: Ap err %.3e Ac err %.3e' % (np.max(np.abs(got_ap - Ap)) / amax,
+                                                               np.max(np.abs(got_ac - Ac)) / amax), flush=True)
+        assert np.max(np.abs(got_ap - Ap)) <= 1e-12 * amax
+        assert np.max(np.abs(got_ac - Ac)) <= 1e-12 * amax
+        a2, c2 = sig.GWAmplitudes(dict(ev), f, rot=37.)
+        assert np.max(np.abs(a2 - Ap)) <= 1e-12 * amax and np.max(np.abs(c2 - Ac)) <= 1e-12 * amax
+        h = sig.GWstrain(f, *args, rot=37., is_chi1chi2=True)
+        ho = od.gw_strain(owf, ev, f, rot=37.)
+        # the strain's phase reaches 1e9 rad: compare to its rounding
+        assert np.max(np.abs(h - ho)) <= 1e-6 * np.max(np.abs(ho))
+
+
+def test_catalogue_forecast_eccentric_model(gpu, tmp_path):
+    """The catalogue driver with eccentric TaylorF2 (the driver's 'tf2_ecc',
+    calculate_forecasts_from_catalog.py:68; 12 parameters, row 11 = ecc):
+    the chunk's Fisher matrices equal DetNet.FisherMatr on the detected
+    events."""
+    from gwfast_amd import forecast as FC
+    from gwfast_amd.waveforms import TaylorF2_RestrictedPN
+    wf = TaylorF2_RestrictedPN(is_eccentric=True, fRef_ecc=5.)
+    ev = bbh(40, seed=85)
+    ev['ecc'] = np.linspace(0.0, 0.1, 40)
+    net = hip_net(wf, NET_C1, True)
+    suffs = FC.run_catalog(dict(ev), net, str(tmp_path), batch_size=40, res=400, verbose=False)
+    snr = net.SNR(dict(ev), res=400)
+    det = np.nonzero(snr > 12.0)[0]
+    assert len(det) > 0
+    F = np.load(tmp_path / ('fishers%s.npy' % suffs[0]))
+    Fr = net.FisherMatr({k: v[det] for k, v in ev.items()}, res=400)
+    print('ecc catalogue: %d detected, Fisher %s' % (len(det), F.shape), flush=True)
+    assert F.shape == (12, 12, len(det)) and np.array_equal(F, Fr)

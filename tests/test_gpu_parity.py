@@ -243,6 +243,71 @@ def test_ragged_grids_match_uniform(gpu):
         assert np.max(np.abs(snr[sel] - su) / su) <= 1e-15
 
 
+C3_LENS = np.array([500, 511, 512, 513, 575, 576, 577, 1000, 2048, 4095, 4096, 4097, 7777, 9999, 10000, 10000],
+                   dtype=np.int32)
+
+
+def test_c3_ragged_tidal_network_vs_oracle(gpu):
+    """Config C3 on its own workload: tidal TaylorF2 (13 parameters, Kerr
+    ISCO cut: 1.3-4.2 kHz for this BNS population) on ETS + CE1Id + CE2NM
+    with Earth rotation, one grid length per event from 500 to 10^4 points
+    (64-point chunk edges included) in one launch; every event against the
+    oracle run at its own length."""
+    from oracle import detector as OD
+    wf, owf = _models('TF2', is_tidal=True, which_ISCO='Kerr')
+    n = len(C3_LENS)
+    ev = bns(n, seed=41)
+    net = hip_net(wf, NET_C1, True)
+    odets = oracle_net(NET_C1, True)
+    F = net.FisherMatr(dict(ev), res=C3_LENS)
+    snr = net.SNR(dict(ev), res=C3_LENS)
+    assert F.shape == (13, 13, n)
+    worst = [0., 0., 0.]
+    for e in range(n):
+        one = {k: v[e:e + 1] for k, v in ev.items()}
+        r = int(C3_LENS[e])
+        so = OD.network_snr(odets, owf, one, r)
+        Fd = OD.network_fisher(odets, owf, one, r, mode='dual')
+        Ff = OD.network_fisher(odets, owf, one, r, mode='faithful')
+        errs = (abs(snr[e] - so[0]) / so[0], norm_err(F[:, :, e:e + 1], Fd), norm_err(F[:, :, e:e + 1], Ff))
+        worst = [max(a, b) for a, b in zip(worst, errs)]
+        print('C3 event %d res %d: SNR %.2f rel err %.2e, Fisher vs dual %.2e vs faithful %.2e' % ((e, r, so[0]) + errs),
+              flush=True)
+    assert worst[0] <= 1e-10
+    assert worst[1] <= 1e-10
+    assert worst[2] <= 1e-6
+
+
+def test_c3_full_size_properties(gpu):
+    """Config C3 at full size: 10^5 tidal BNS events, grid lengths uniform in
+    [500, 10^4], one ragged launch: finite, exactly symmetric,
+    F[dL,dL] dL^2 / SNR^2 = 1 and F[Phic,Phic] / SNR^2 = 1 against the ragged
+    SNR, F[dL,Phic] = 0, and bitwise determinism of two launches."""
+    import torch
+    from gwfast_amd import _engine as E
+    wf, _ = _models('TF2', is_tidal=True, which_ISCO='Kerr')
+    n = 100_000
+    ev = bns(n, seed=11)
+    res = np.random.default_rng(11).integers(500, 10_001, n).astype(np.int32)
+    net = hip_net(wf, NET_C1, True)
+    rows = E.to_dev(E.param_rows(ev, True))
+    F1 = net.fisher_device(rows, res)
+    F2 = net.fisher_device(rows, res)
+    snr = net.snr_device(rows, res)
+    torch.cuda.synchronize()
+    assert torch.equal(F1, F2)
+    F = F1.cpu().numpy()
+    snr = snr.cpu().numpy()
+    assert np.all(np.isfinite(F)) and np.all(snr > 0)
+    assert np.max(np.abs(F - F.transpose(1, 0, 2))) == 0.0
+    dL = ev['dL']
+    e1 = np.max(np.abs(F[2, 2] * dL ** 2 / snr ** 2 - 1.0))
+    e2 = np.max(np.abs(F[8, 8] / snr ** 2 - 1.0))
+    e3 = np.max(np.abs(F[2, 8]) / np.sqrt(F[2, 2] * F[8, 8]))
+    print('C3 10^5 events: dL identity %.2e, Phic identity %.2e, dL-Phic %.2e' % (e1, e2, e3), flush=True)
+    assert e1 <= 1e-10 and e2 <= 1e-10 and e3 <= 1e-12
+
+
 def test_ragged_chunk_edges(gpu):
     """Ragged grid lengths at the 64-point chunk edges of the producer /
     consumer pairs (2, 3, 63, 64, 65, 127, 128, 129 and 1000 points), more
