@@ -56,6 +56,8 @@ SIGNATURES = {
                                c_void_p, c_void_p, c_size_t, c_void_p]),
     'gwf_fisher': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64, c_int32,
                                   c_int32, c_void_p, c_void_p, c_size_t, c_void_p]),
+    'gwf_fisher_noalloc': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,
+                                          c_int32, c_int32, c_void_p, c_void_p]),
     'gwf_snr_ragged': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,
                                       c_void_p, c_void_p, c_void_p, c_size_t, c_void_p]),
     'gwf_fisher_ragged': (ctypes.c_int, [_P(GwfWaveform), _P(GwfDetector), c_int32, c_void_p, c_int64, c_int64,

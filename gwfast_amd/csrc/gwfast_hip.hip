@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gwf_paths.h"
@@ -1531,6 +1532,37 @@ int check_common(const gwf_waveform* wf, const double* par, int64_t ld, int64_t 
   return 0;
 }
 
+// Per-device scratch of gwf_fisher_noalloc: the event records and the queue
+// counters of up to GWF_NOALLOC_MAX_EVENTS events in a file-scope device
+// array (each GPU the library runs on holds its own copy until the process
+// ends).  Host code never passes `scratch` itself to a kernel: its device
+// address comes from hipGetSymbolAddress, once per device.
+constexpr size_t NOALLOC_BYTES = (size_t)GWF_NOALLOC_MAX_EVENTS * REC_DOUBLES * sizeof(double) + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device with the device current.
+int scratch_address(void** out) {
+  int dev = 0;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= SCRATCH_DEVICES) return fail(-EINVAL, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    if (!p) return fail(-EIO, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 // the screen kernels (gwf_mlp.hip) report through the same gwf_last_error
@@ -1803,6 +1835,17 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
   int rc = gwf_prepare(wf, dets, n_det, params, ld, n_ev, res, workspace, ws_bytes, stream);
   if (rc) return rc;
   return gwf_fisher_prepared(wf, dets, n_det, n_ev, res, arm, fisher_out, workspace, ws_bytes, stream);
+}
+
+int gwf_fisher_noalloc(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                       int64_t ld, int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* stream) {
+  if (n_ev > GWF_NOALLOC_MAX_EVENTS)
+    return fail(-EINVAL, "gwf_fisher_noalloc takes at most " + std::to_string(GWF_NOALLOC_MAX_EVENTS) +
+                             " events per call (gwf_fisher with a workspace takes more)");
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return gwf_fisher(wf, dets, n_det, params, ld, n_ev, res, arm, fisher_out, ws, NOALLOC_BYTES, stream);
 }
 
 int gwf_snr_ragged(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params, int64_t ld,

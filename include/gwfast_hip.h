@@ -131,6 +131,20 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
                int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* workspace, size_t ws_bytes,
                void* stream);
 
+/* gwf_fisher without a workspace argument, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * event records and queue counters live in a file-scope device array of the
+ * library, sized for GWF_NOALLOC_MAX_EVENTS events (about 45.5 MB, held by
+ * each GPU the library runs on until the process ends); larger n_ev returns
+ * -EINVAL.  The call allocates nothing; the caller passes the inputs and
+ * fisher_out as for gwf_fisher, and gets the same results.  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race, so serialise them. */
+#define GWF_NOALLOC_MAX_EVENTS 8192
+int gwf_fisher_noalloc(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, const double* params,
+                       int64_t ld, int64_t n_ev, int32_t res, int32_t arm, double* fisher_out, void* stream);
+
 /* DetNet.FisherMatr with the duty-cycle masks of GWSignal(DutyFactor=...)
  * (signal.py:932-936, 968-969, 1004-1005, 1038-1039, 1071-1072; network.py:
  * 84-152) and of the catalogue driver (calculate_forecasts_from_catalog.py:
