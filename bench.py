@@ -80,7 +80,7 @@ def _cpu_model():
     return 'unknown'
 
 
-def cpu_baseline(per_core=128, res=RES, seed=0, max_cores=16):
+def cpu_baseline(per_core=512, res=RES, seed=0, max_cores=16):
     """The oracle (reference-faithful construction: AD-equivalent rows through
     exp(i Psi), closed-form rows, trapz Gram) on per_core events per core of
     the same workload, one single-threaded process per core (the reference's
@@ -383,7 +383,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--n-events', type=int, default=N_EV)
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-per-core', type=int, default=128, help='CPU baseline events per core')
+    ap.add_argument('--cpu-per-core', type=int, default=512, help='CPU baseline events per core')
     ap.add_argument('--no-c2', action='store_true', help='skip the 10^5-event (C2) leg')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')

@@ -307,4 +307,18 @@ GWF_HD void std_(double* p, const D<N>& x) {
 }
 GWF_HD void std_(double* p, double x) { p[0] = x; }
 
+// Store of one lane of a tangent-split evaluation: with tk >= 0 the D<1> x
+// carries tangent tk only, and the lane writes that tangent (plus the value
+// when tk == 0) into the full (value, tangents) slot at p; tk < 0 stores
+// the whole dual.
+template <int N>
+GWF_HD void std_tk(double* p, const D<N>& x, int tk) {
+  if (tk < 0) {
+    std_(p, x);
+  } else {
+    if (tk == 0) p[0] = x.v;
+    p[1 + tk] = x.d[0];
+  }
+}
+
 }  // namespace gwf

@@ -227,13 +227,42 @@ GWF_HD constexpr bool is_pd(int model) {
 }
 
 // -------------------------------------------------------------- setup body
+// The IMRPhenomD models' waveform coefficients, split over NT lanes: lane tk
+// evaluates every coefficient as a D<1> carrying tangent tk only (its seed
+// the only nonzero one) and stores the value (tk == 0) and that tangent.
+// The same arithmetic as the NT-tangent evaluation, in a fifth of the
+// registers, so the dependent chain of the collocation solve no longer
+// spills.
+template <int MODEL>
+GWF_HD void setup_waveform_split(const gwf_waveform& w, const double* par, int64_t ld, int64_t e, double f0,
+                                 double* rec, int tk) {
+  static_assert(is_pd(MODEL), "split setup: IMRPhenomD models");
+  const double Mc = par[0 * ld + e], eta = par[1 * ld + e];
+  const double chi1 = par[9 * ld + e], chi2 = par[10 * ld + e];
+  auto seed = [&](double v, int slot) { return mkseed<1>(v, slot == tk ? 0 : -1); };
+  const double f0u = (w.fref > 0.0) ? w.fref : f0;
+  const D<1> Mcs = seed(Mc, 0), etas = seed(eta, 1), c1s = seed(chi1, 2), c2s = seed(chi2, 3);
+  if (MODEL == GWF_MODEL_IMRPHENOMD) {
+    pd_setup<1>(Mcs, etas, c1s, c2s, f0u, w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT,
+                mkc<1>(1.0), mkc<1>(1.0), tk);
+  } else {
+    D<1> L1, L2;
+    lam12_from_lamt(seed(par[11 * ld + e], 4), seed(par[12 * ld + e], 5), etas, L1, L2);
+    const D<1> QM1 = quadmon(L1), QM2 = quadmon(L2);
+    pd_setup<1>(Mcs, etas, c1s, c2s, f0u, w.apply_fcut != 0, w.qnm_a, w.qnm_fring, w.qnm_fdamp, w.n_qnm, rec, SLOT,
+                QM1, QM2, tk);
+    nrt_setup<1>(Mcs, etas, c1s, c2s, L1, L2, QM1, QM2, rec, SLOT, tk);
+  }
+}
+
+template <int MODEL>
+GWF_HD void setup_event_consts(const NetK& net, const double* par, int64_t ld, int64_t e, double* rec);
+
 // Fills the event record: waveform coefficients (duals) + projection constants.
 template <int MODEL, int NT>
 GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* par, int64_t ld, int64_t e, double f0,
                         double* rec) {
-  const double Mc = par[0 * ld + e], eta = par[1 * ld + e], dL = par[2 * ld + e];
-  const double theta = par[3 * ld + e], phi = par[4 * ld + e], iota = par[5 * ld + e];
-  const double psi = par[6 * ld + e], tc = par[7 * ld + e];
+  const double Mc = par[0 * ld + e], eta = par[1 * ld + e];
   const double chi1 = par[9 * ld + e], chi2 = par[10 * ld + e];
   if (MODEL == GWF_MODEL_IMRPHENOMD) {
     const double f0u = (w.fref > 0.0) ? w.fref : f0;
@@ -263,6 +292,17 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
     tf2_setup<NT>(w, mkseed<NT>(Mc, 0), mkseed<NT>(eta, 1), mkseed<NT>(chi1, 2), mkseed<NT>(chi2, 3),
                   mkseed<NT>(Lt, 4), mkseed<NT>(dLm, 5), rec, SLOT);
   }
+  setup_event_consts<MODEL>(net, par, ld, e, rec);
+}
+
+// The event constants of the record (after its waveform coefficients, whose
+// fcut it reads): inclination factors, 1/dL, tcoal, sky angles, ln(M G) and
+// its (Mc, eta) tangents, and the projection constants of every site.
+template <int MODEL>
+GWF_HD void setup_event_consts(const NetK& net, const double* par, int64_t ld, int64_t e, double* rec) {
+  const double Mc = par[0 * ld + e], eta = par[1 * ld + e], dL = par[2 * ld + e];
+  const double theta = par[3 * ld + e], phi = par[4 * ld + e], iota = par[5 * ld + e];
+  const double psi = par[6 * ld + e], tc = par[7 * ld + e];
   double* ev = rec + REC_EV;
   const double ci = cos(iota), si = sin(iota);
   ev[EV_DL] = dL;
@@ -277,11 +317,12 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   ev[EV_CRA] = cos(phi);
   ev[EV_SRA] = sin(phi);
   ev[EV_FCUT] = rec[(is_pd(MODEL) ? (int)PD_FCUT : (int)TF_FCUT) * SLOT];
-  {  // ln(M G f) = ln(M G) + ln f on the grids, whose ln f comes with f
+  {  // ln(M G f) = ln(M G) + ln f on the grids, whose ln f comes with f;
+     // ln(M G) = ln Mc - (3/5) ln eta + ln G: tangents 1/Mc and -(3/5)/eta
     const double* mg = rec + (is_pd(MODEL) ? (int)PD_MG : (int)TF_MG) * SLOT;
     ev[EV_LMG] = log(mg[0]);
-    ev[EV_LMG_D0] = mg[1] / mg[0];
-    ev[EV_LMG_D1] = mg[2] / mg[0];
+    ev[EV_LMG_D0] = 1.0 / Mc;
+    ev[EV_LMG_D1] = -0.6 / eta;
   }
   // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
   const double ra = phi, dec = 0.5 * PI - theta;

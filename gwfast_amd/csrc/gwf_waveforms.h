@@ -261,7 +261,7 @@ GWF_HD X pd_phi_piecewise(const PdPhase<T>& c, const X& x, bool apply_fcut) {
 template <int N>
 GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2, double f0,
                      bool apply_fcut, const double* qa, const double* qfr, const double* qfd, int nq,
-                     double* rec, int stride, const D<N>& QM1, const D<N>& QM2) {
+                     double* rec, int stride, const D<N>& QM1, const D<N>& QM2, int tk = -1) {
   typedef D<N> T;
   const T M = Mc / dpow(eta, 3. / 5.);
   const T MG = M * GMSUN_C3;
@@ -384,7 +384,7 @@ GWF_HD void pd_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<
   // starts: none of them is live past this point but fring and fdamp, which
   // shortens the live ranges through the collocation solve below.
   double* r = rec;
-  auto put = [&](int slot, const T& x) { std_(r + slot * stride, x); };
+  auto put = [&](int slot, const T& x) { std_tk(r + slot * stride, x, tk); };
   put(PD_MG, MG);
   put(PD_ETA, eta);
   put(PD_PIP, c.pip); put(PD_P23, c.p23); put(PD_P13, c.p13); put(PD_P13L, c.p13l); put(PD_PLOG, c.plog);
@@ -1028,7 +1028,7 @@ GWF_HD double nrt_fcut(double Mc, double eta, double Lt, double dLm) {
 // tidal-amplitude factor, merger frequency, and the model's fcut.
 template <int N>
 GWF_HD void nrt_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D<N>& chi2, const D<N>& L1,
-                      const D<N>& L2, const D<N>& QM1, const D<N>& QM2, double* rec, int stride) {
+                      const D<N>& L2, const D<N>& QM1, const D<N>& QM2, double* rec, int stride, int tk = -1) {
   typedef D<N> T;
   const T Seta = seta_of(eta);
   const T m1ByM = 0.5 * (1.0 + Seta);
@@ -1052,7 +1052,7 @@ GWF_HD void nrt_setup(const D<N>& Mc, const D<N>& eta, const D<N>& chi1, const D
   // Overallamp without 1/dL (waveforms.py:1248) times 2 sqrt(pi/5) and -9 kappa2T
   const T OA = 2. * sqrt(5. / (64. * PI)) * M * GMSUN_C2_GPC * M * GMSUN_C3;
   const T fm = nrt_fmerger(eta, Seta, kappa2T);
-  auto put = [&](int slot, const T& x) { std_(rec + slot * stride, x); };
+  auto put = [&](int slot, const T& x) { std_tk(rec + slot * stride, x, tk); };
   put(PD_TPH, -kappa2T * 2.4375 / (m1ByM * m2ByM));
   put(PD_SSS, (SS + SSS) * TF2OA);
   put(PD_TAMP, OA * (2. * sqrt(PI / 5.)) * (-9.0 * kappa2T));

@@ -61,7 +61,12 @@ template <int MODEL, int NT>
 __global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const double* __restrict__ par, int64_t ld,
                                                int64_t n_ev, const double* __restrict__ fgrid, int64_t nf,
                                                double* __restrict__ rec) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // IMRPhenomD models: NT lanes per event, lane tk evaluating tangent tk
+  // (setup_waveform_split); TaylorF2: one lane per event
+  constexpr int LPE = is_pd(MODEL) ? NT : 1;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = tid / LPE;
+  const int tk = (int)(tid - e * LPE);
   if (e >= n_ev) return;
   double f0;
   if (fgrid) {  // minimum of the caller's grid column (waveforms.py:1139)
@@ -70,7 +75,13 @@ __global__ __launch_bounds__(256) void k_setup(gwf_waveform w, NetK net, const d
   } else {  // first grid point: 10**log10(fmin) for geomspace, fmin for linspace
     f0 = net.lin ? net.fmin : grid_f(log10(net.fmin), log10(net.fmin), 0, 1);
   }
-  setup_event<MODEL, NT>(w, net, par, ld, e, f0, rec + e * (int64_t)REC_DOUBLES);
+  double* r = rec + e * (int64_t)REC_DOUBLES;
+  if constexpr (is_pd(MODEL)) {
+    setup_waveform_split<MODEL>(w, par, ld, e, f0, r, tk);
+    if (tk == 0) setup_event_consts<MODEL>(net, par, ld, e, r);  // reads only lane 0's stores
+  } else {
+    setup_event<MODEL, NT>(w, net, par, ld, e, f0, r);
+  }
 }
 
 #ifndef GWF_FISHER_MIN_BLOCKS
@@ -768,17 +779,23 @@ __global__ __launch_bounds__(256) void k_vjp(const double* __restrict__ J, const
 
 // ------------------------------------------- covariance (one wave per event)
 // fisherTools' per-event loops (CovMatr, CheckFisher) run one 64-lane wave
-// per event.  The event's P x P matrices live in LDS (row pitch CLD, rows and
-// columns P..PMAX-1 kept zero); lanes work on matrix entries, rows or
-// columns, and the workgroup is that one wave, so __syncthreads() orders
-// the LDS steps without tying events together.  No private arrays: nothing
-// goes to scratch memory.
+// per event.  The event's P x P matrices live in LDS (rows and columns
+// beyond P kept zero); lanes work on matrix entries, rows or columns, and
+// the workgroup is that one wave, so __syncthreads() orders the LDS steps
+// without tying events together.  No private arrays: nothing goes to
+// scratch memory.  PT > 0 fixes P at compile time (index arithmetic by
+// constants, LDS sized for P); PT = 0 takes any P <= PMAX at run time.
 constexpr int PMAX = 16;
-constexpr int CLD = PMAX + 1;
-constexpr int CMAT = PMAX * CLD;
 constexpr double MP_TOL = 2.220446049250313e-16;  // 2^-52: mpmath's tolerance at 53 bits
 
-#define AT(M, i, j) (M)[(i) * CLD + (j)]
+template <int PT>
+struct CovDim {
+  static constexpr int N = PT > 0 ? PT + (PT & 1) : PMAX;  // kept rows / columns (an odd P adds a zero index)
+  static constexpr int LD = N + 1;                         // odd row pitch (LDS banks)
+  static constexpr int MAT = N * LD;
+};
+
+#define AT(M, i, j) (M)[(i) * LD + (j)]
 
 // Blocks are dealt to the 8 XCDs round-robin (block b on XCD b % 8): this
 // mapping gives each XCD one contiguous range of events, so the (P, P, N)
@@ -803,27 +820,55 @@ __device__ __forceinline__ double wave_max(double v) {
 __device__ __forceinline__ int rr_partner(int i, int r, int m) {
   if (i == m) return r;
   if (i == r) return m;
-  return (2 * r - i + 2 * m) % m;
+  const int v = 2 * r - i;  // in (-m, 2m)
+  return v < 0 ? v + m : (v >= m ? v - m : v);
 }
 
 // Jacobi eigen-decomposition of the symmetric P x P matrix in E (destroyed):
 // cyclic Jacobi in round-robin order, the P/2 disjoint rotations of a round
-// applied together as E <- J^T E J and V <- V J (each rotation the classical
-// one that zeroes E_pq: theta = (E_qq - E_pp) / (2 E_pq),
-// t = sgn(theta) / (|theta| + sqrt(theta^2 + 1))), sweeps until the
-// off-diagonal mass is below 1e-36 of the diagonal's (at most 60).  On
-// return w[0..P) holds the eigenvalues in ascending order (mpmath.eigh's
-// order; ties and NaNs keep index order, NaNs last) and the columns of V the
-// eigenvectors.  E, E2, V, V2: LDS matrices with zero rows / columns beyond
-// P (an odd P pairs with the zero index P); the four pointers are swapped as
-// the rounds alternate between them.  cc, ss: PMAX doubles of LDS.
-__device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, int P, double* w, double* cc,
+// applied together as E <- J^T E J (and V <- V J when WANT_V).  Each rotation
+// is the classical one that zeroes E_pq (Numerical Recipes 11.1:
+// t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)), theta = (E_qq - E_pp) /
+// (2 E_pq), here t = sgn(d) h / (|d| + sqrt(d^2 + h^2)) with d = E_qq - E_pp,
+// h = 2 E_pq; E_pp -= t E_pq, E_qq += t E_pq, E_pq = 0), a pair rotated only
+// while E_pq^2 > 2^-104 |E_pp E_qq| (the threshold of Demmel & Veselic's
+// relatively accurate Jacobi).  A round: the lanes of the indices compute
+// their pair's rotation and new diagonal, then one lane per upper
+// off-diagonal entry applies the two-sided update (entries of a round's
+// pairs never mix diagonal values in).  Sweeps until one rotates nothing
+// or the off-diagonal mass is below 1e-36 of the diagonal's (at most 60).
+// On return w[0..P) holds the eigenvalues in ascending order (mpmath.eigh's
+// order; ties and NaNs keep index order, NaNs last) and, when WANT_V, the
+// columns of V the eigenvectors.  E, E2, V, V2: LDS matrices with zero rows
+// / columns beyond P (an odd P pairs with the zero index P); the pointers
+// are swapped as the rounds alternate between them.  cc, ss: PMAX doubles of
+// LDS.
+template <int PT, bool WANT_V>
+__device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, int Prt, double* w, double* cc,
                             double* ss, int lane) {
-  for (int t = lane; t < P * P; t += 64) {
-    const int i = t / P, j = t - (t / P) * P;
-    AT(V, i, j) = (i == j) ? 1.0 : 0.0;
+  constexpr int LD = CovDim<PT>::LD;
+  const int P = PT > 0 ? PT : Prt;
+  if (WANT_V) {
+    for (int t = lane; t < P * P; t += 64) {
+      const int i = t / P, j = t - (t / P) * P;
+      AT(V, i, j) = (i == j) ? 1.0 : 0.0;
+    }
   }
   const int m = P + (P & 1) - 1;
+  const int n_off = P * (P - 1) / 2;
+  // this lane's upper off-diagonal entries k = lane, lane + 64, ... (row-major)
+  constexpr int KMAX = (PMAX * (PMAX - 1) / 2 + 63) / 64;
+  int oi[KMAX], oj[KMAX];
+#pragma unroll
+  for (int u = 0; u < KMAX; ++u) {
+    int kk = lane + 64 * u, i = 0;
+    while (i < P - 1 && kk >= P - 1 - i) {
+      kk -= P - 1 - i;
+      ++i;
+    }
+    oi[u] = i;
+    oj[u] = i + 1 + kk;
+  }
   __syncthreads();
   for (int sweep = 0; sweep < 60; ++sweep) {
     double off = 0.0, diag = 0.0;
@@ -836,46 +881,65 @@ __device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, in
     off = wave_sum(off);
     diag = wave_sum(diag);
     if (!(off > 1e-36 * diag)) break;  // the same sums on every lane: uniform
+    bool rotated = false;
     for (int r = 0; r < m; ++r) {
+      bool rot = false;
       if (lane < P) {
         const int j = rr_partner(lane, r, m);
-        double c = 1.0, s = 0.0;
+        double c = 1.0, s = 0.0, dnew = AT(E, lane, lane);
         if (j < P) {
           const int p = min(lane, j), q = max(lane, j);
           const double apq = AT(E, p, q);
-          if (apq != 0.0) {
-            const double app = AT(E, p, p), aqq = AT(E, q, q);
-            const double theta = (aqq - app) / (2.0 * apq);
-            const double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-            c = 1.0 / sqrt(tt * tt + 1.0);
+          const double app = AT(E, p, p), aqq = AT(E, q, q);
+          if (apq * apq > (MP_TOL * MP_TOL) * fabs(app * aqq)) {
+            const double d = aqq - app, h = 2.0 * apq;
+            const double num = d > 0.0 ? h : (d < 0.0 ? -h : fabs(h));
+            const double tt = num / (fabs(d) + sqrt(d * d + h * h));
+            c = rsqrt(tt * tt + 1.0);
             s = tt * c;
+            dnew = lane == p ? app - tt * apq : aqq + tt * apq;
+            rot = true;
           }
           if (lane == p) s = -s;  // J_pp = c, J_qp = -s, J_pq = s, J_qq = c
         }
         cc[lane] = c;
         ss[lane] = s;
+        AT(E2, lane, lane) = dnew;
       }
+      if (__ballot(rot) == 0) continue;  // nothing to rotate this round (uniform)
+      rotated = true;
       __syncthreads();
-      for (int t = lane; t < P * P; t += 64) {
-        const int i = t / P, j = t - (t / P) * P;
-        const int ip = rr_partner(i, r, m), jp = rr_partner(j, r, m);
-        const double cj = cc[j], sj = ss[j];
-        if (i <= j) {  // (J^T E J)_ij, mirrored: E stays exactly symmetric
-          const double ci = cc[i], si = ss[i];
-          const double v = ci * (cj * AT(E, i, j) + sj * AT(E, i, jp)) + si * (cj * AT(E, ip, j) + sj * AT(E, ip, jp));
+#pragma unroll
+      for (int u = 0; u < KMAX; ++u) {
+        if (lane + 64 * u < n_off) {
+          const int i = oi[u], j = oj[u];
+          const int ip = rr_partner(i, r, m), jp = rr_partner(j, r, m);
+          double v = 0.0;
+          if (!(jp == i && ss[j] != 0.0)) {  // a rotated pair's own entry becomes 0
+            const double ci = cc[i], si = ss[i], cj = cc[j], sj = ss[j];
+            v = ci * (cj * AT(E, i, j) + sj * AT(E, i, jp)) + si * (cj * AT(E, ip, j) + sj * AT(E, ip, jp));
+          }
           AT(E2, i, j) = v;
           AT(E2, j, i) = v;
         }
-        AT(V2, i, j) = AT(V, i, j) * cj + AT(V, i, jp) * sj;
+      }
+      if (WANT_V) {
+        for (int t = lane; t < P * P; t += 64) {
+          const int i = t / P, j = t - (t / P) * P;
+          AT(V2, i, j) = AT(V, i, j) * cc[j] + AT(V, i, rr_partner(j, r, m)) * ss[j];
+        }
       }
       __syncthreads();
       double* t1 = E;
       E = E2;
       E2 = t1;
-      t1 = V;
-      V = V2;
-      V2 = t1;
+      if (WANT_V) {
+        t1 = V;
+        V = V2;
+        V2 = t1;
+      }
     }
+    if (!rotated) break;
   }
   if (lane < P) {  // rank of each eigenvalue; the sorted eigenvectors go to V2
     const double wi = AT(E, lane, lane);
@@ -888,12 +952,15 @@ __device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, in
       rk += before ? 1 : 0;
     }
     w[rk] = wi;
-    for (int k = 0; k < P; ++k) AT(V2, k, rk) = AT(V, k, lane);
+    if (WANT_V)
+      for (int k = 0; k < P; ++k) AT(V2, k, rk) = AT(V, k, lane);
   }
   __syncthreads();
-  double* t1 = V;
-  V = V2;
-  V2 = t1;
+  if (WANT_V) {
+    double* t1 = V;
+    V = V2;
+    V2 = t1;
+  }
 }
 
 // fisherTools.CovMatr per event (fisherTools.py:53-196).  meth / alt:
@@ -903,10 +970,13 @@ __device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, in
 // switches 'cho' to alt; then the chosen inverse of A, symmetrised and
 // unscaled; eps = max|C F - I| against F, or against the truncated scaled
 // matrix when 'svd' truncates (as the reference stores it, 128-136).
-__global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P, int64_t n_ev, int meth_in, int alt,
+template <int PT>
+__global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Prt, int64_t n_ev, int meth_in, int alt,
                                             int truncate, double svals_thresh, double cond_max, double* __restrict__ C,
                                             double* __restrict__ eps, double* __restrict__ cond,
                                             int* __restrict__ method) {
+  constexpr int LD = CovDim<PT>::LD, CMAT = CovDim<PT>::MAT;
+  const int P = PT > 0 ? PT : Prt;
   __shared__ double sm[6 * CMAT];
   __shared__ double vw[PMAX], vws[PMAX], vcc[PMAX], vss[PMAX];
   __shared__ int vpiv[PMAX], vperm[PMAX];
@@ -953,7 +1023,7 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P,
     AT(E, i, j) = a;
   }
   __syncthreads();
-  wave_jacobi(E, E2, V, V2, P, vw, vcc, vss, lane);
+  wave_jacobi<PT, false>(E, E2, V, V2, P, vw, vcc, vss, lane);
   double emax = 0.0, emin = INFINITY;
   bool neg = false;
   for (int i = 0; i < P; ++i) {
@@ -963,7 +1033,7 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P,
   }
   const double cnd = emax / emin;
   if (cond && lane == 0) cond[e] = cnd;
-  // E, E2, V2 are free from here; V holds the eigenvectors
+  // E, E2, V, V2 are free from here (the svd methods compute the eigenvectors)
   int m = meth_in;
   if (m == GWF_INV_CHO) {
     if (neg) {
@@ -1015,7 +1085,11 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P,
     }
   } else if (m == GWF_INV_SVD || m == GWF_INV_SVD_REG) {
     // SVD of the symmetric A from its eigenpairs: singular values |w_k|,
-    // A^+ = sum_k sign(w_k) v_k v_k^T / |w_k|
+    // A^+ = sum_k sign(w_k) v_k v_k^T / |w_k|.  The eigenvectors: the same
+    // Jacobi run again on A, with V (the values come out the same)
+    for (int t = lane; t < P * P; t += 64) AT(E, t / P, t - (t / P) * P) = AT(A, t / P, t - (t / P) * P);
+    __syncthreads();
+    wave_jacobi<PT, true>(E, E2, V, V2, P, vw, vcc, vss, lane);
     double smax = 0.0;
     for (int k = 0; k < P; ++k) smax = fmax(smax, fabs(vw[k]));
     const bool trunc = (m == GWF_INV_SVD) && truncate && fabs(cnd) > cond_max;
@@ -1188,9 +1262,12 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int P,
 // fisherTools.CheckFisher per event (fisherTools.py:216-279): eigenvalues
 // (ascending), eigenvectors (evecs[e][i][k] = component i of vector k) and
 // the condition number max|w| / min|w|; an all-NaN matrix gives NaN.
-__global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int P, int64_t n_ev,
+template <int PT>
+__global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int Prt, int64_t n_ev,
                                               double* __restrict__ evals, double* __restrict__ evecs,
                                               double* __restrict__ cond) {
+  constexpr int LD = CovDim<PT>::LD, CMAT = CovDim<PT>::MAT;
+  const int P = PT > 0 ? PT : Prt;
   __shared__ double sm[4 * CMAT];
   __shared__ double vw[PMAX], vcc[PMAX], vss[PMAX];
   const int64_t e = xcd_event();
@@ -1213,7 +1290,7 @@ __global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int 
     return;
   }
   __syncthreads();
-  wave_jacobi(E, E2, V, V2, P, vw, vcc, vss, lane);
+  wave_jacobi<PT, true>(E, E2, V, V2, P, vw, vcc, vss, lane);
   double emax = 0.0, emin = INFINITY;
   for (int i = 0; i < P; ++i) {
     emax = fmax(emax, fabs(vw[i]));
@@ -1502,8 +1579,11 @@ int launch_setup(const gwf_waveform* wf, const NetK& net, const double* par, int
   // event), so small batches spread one wave per CU (measured at 10^4 events:
   // 64-thread blocks 0.095 ms, 256-thread blocks 0.132 ms; at 10^6 events
   // 4.46 against 4.37 ms)
-  const int bs = n_ev <= 65536 ? 64 : 256;
-  const dim3 grid((unsigned)((n_ev + bs - 1) / bs)), block(bs);
+  // (IMRPhenomD models: NT lanes per event, the tangent-split record)
+  const int lpe = wf->model == GWF_MODEL_IMRPHENOMD ? 4 : (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2 ? 6 : 1);
+  const int64_t nt = n_ev * lpe;
+  const int bs = nt <= 65536 * 4 ? 64 : 256;
+  const dim3 grid((unsigned)((nt + bs - 1) / bs)), block(bs);
   if (wf->model == GWF_MODEL_IMRPHENOMD)
     hipLaunchKernelGGL((k_setup<GWF_MODEL_IMRPHENOMD, 4>), grid, block, 0, st, *wf, net, par, ld, n_ev, fgrid, nf, rec);
   else if (wf->model == GWF_MODEL_IMRPHENOMD_NRTIDALV2)
@@ -2005,9 +2085,18 @@ int gwf_cov_invert_ex(const double* F, int32_t P, int64_t n_ev, int32_t inv_meth
     return fail(-EINVAL, "alt_method has to be different from invMethodIn");
   if (n_ev == 0) return 0;
   if (!F || !cov_out) return fail(-EINVAL, "NULL array");
-  hipLaunchKernelGGL(k_cov, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
-                     (int)inv_method, (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out,
-                     eps_out, cond_out, (int*)method_out);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
+                       (int)inv_method, (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out,
+                       eps_out, cond_out, (int*)method_out);
+  };
+  switch (P) {  // the path's sizes at compile time
+    case 11: go(k_cov<11>); break;
+    case 12: go(k_cov<12>); break;
+    case 13: go(k_cov<13>); break;
+    case 14: go(k_cov<14>); break;
+    default: go(k_cov<0>);
+  }
   return hip_check(hipGetLastError(), "gwf_cov_invert launch");
 }
 
@@ -2033,8 +2122,17 @@ int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out
   if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
   if (n_ev == 0) return 0;
   if (!F || !evals_out || !cond_out) return fail(-EINVAL, "NULL array");
-  hipLaunchKernelGGL(k_check, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
-                     evals_out, evecs_out, cond_out);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev, evals_out,
+                       evecs_out, cond_out);
+  };
+  switch (P) {
+    case 11: go(k_check<11>); break;
+    case 12: go(k_check<12>); break;
+    case 13: go(k_check<13>); break;
+    case 14: go(k_check<14>); break;
+    default: go(k_check<0>);
+  }
   return hip_check(hipGetLastError(), "gwf_check_fisher launch");
 }
 
