@@ -533,7 +533,11 @@ GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f,
 // ------------------------------------------------------ strain derivatives
 // Reduced derivative vectors of one arm at one point:
 //   re[X] + i im[X] = dX h * exp(-i Psi), tcoal row already divided by 86400
-// (signal.py:919-920); X in ParNums order.
+// (signal.py:919-920); X in ParNums order.  With h = A (F+ c+ + i Fx cx)
+// exp(i Psi) / dL (c+ = (1 + cos^2 iota)/2, cx = cos iota), Ap = A F+ c+ / dL,
+// Ac = A Fx cx / dL, every row is (dX Ap - Ac dX Psi) + i (dX Ac + Ap dX Psi);
+// the products A c+ / dL, A cx / dL, F+ c+, Fx cx and the pattern-angle
+// terms A F'_alpha c / dL are formed once and shared by the rows.
 template <int MODEL, int NT, int P>
 GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const double* sc, const Geo& g, const ArmK& arm,
                        int motion, double f, double re[P], double im[P], double* Ap_out, double* Ac_out) {
@@ -567,10 +571,14 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   const double al_th = -tw * g.dt_th * INV_DAY, al_ph = 1.0 - tw * g.dt_ph * INV_DAY;
 
   const double Av = wp.A.v * idL;
-  const double Ap = Av * Fp * cp, Ac = Av * Fc * cc;
+  const double Avp = Av * cp, Avc = Av * cc;      // A c+ / dL, A cx / dL
+  const double Fpp = Fp * cp, Fcc = Fc * cc;      // F+ c+, Fx cx
+  const double Ap = Fp * Avp, Ac = Fc * Avc;
+  const double Xp = Fp_al * Avp, Xc = Fc_al * Avc;  // d(Ap, Ac) / d alpha
   *Ap_out = Ap;
   *Ac_out = Ac;
   const double twopif = 2. * PI * f;
+  const double tdt = twopif * g.dt_t;             // 2 pi f d(Delta t)/dt
 
 #define GWF_SET(X, dAp, dAc, dPsi)        \
   {                                       \
@@ -581,51 +589,55 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   // Mc, eta: waveform amplitude/phase + Earth-rotation time dependence
   {
     const double dA = wp.A.d[0] * idL;
-    GWF_SET(0, (dA * Fp + Av * Fp_al * al_mc) * cp, (dA * Fc + Av * Fc_al * al_mc) * cc,
-            -wp.Phi.d[0] + twopif * g.dt_t * tn_mc);
+    GWF_SET(0, dA * Fpp + Xp * al_mc, dA * Fcc + Xc * al_mc, tdt * tn_mc - wp.Phi.d[0]);
   }
   {
     const double dA = wp.A.d[1] * idL;
-    GWF_SET(1, (dA * Fp + Av * Fp_al * al_eta) * cp, (dA * Fc + Av * Fc_al * al_eta) * cc,
-            -wp.Phi.d[1] + twopif * g.dt_t * tn_eta);
+    GWF_SET(1, dA * Fpp + Xp * al_eta, dA * Fcc + Xc * al_eta, tdt * tn_eta - wp.Phi.d[1]);
   }
   // dL
-  GWF_SET(2, -Ap * idL, -Ac * idL, 0.0);
+  re[2] = -Ap * idL;
+  im[2] = -Ac * idL;
   // theta, phi: pattern functions (direct + via the location delay) and phiL
-  GWF_SET(3, Av * (Fp_th + Fp_al * al_th) * cp, Av * (Fc_th + Fc_al * al_th) * cc, twopif * g.dt_th);
-  GWF_SET(4, Av * Fp_al * al_ph * cp, Av * Fc_al * al_ph * cc, twopif * g.dt_ph);
+  GWF_SET(3, Avp * (Fp_th + Fp_al * al_th), Avc * (Fc_th + Fc_al * al_th), twopif * g.dt_th);
+  GWF_SET(4, Xp * al_ph, Xc * al_ph, twopif * g.dt_ph);
   // iota
-  GWF_SET(5, Av * Fp * ev[EV_DCP], Av * Fc * ev[EV_DCC], 0.0);
+  re[5] = Av * (Fp * ev[EV_DCP]);
+  im[5] = Av * (Fc * ev[EV_DCC]);
   // psi: dFp/dpsi = 2 Fc, dFc/dpsi = -2 Fp
-  GWF_SET(6, Av * (2. * Fc) * cp, Av * (-2. * Fp) * cc, 0.0);
+  re[6] = (Fc + Fc) * Avp;
+  im[6] = -(Fp + Fp) * Avc;
   // tcoal, per second
-  GWF_SET(7, Av * Fp_al * al_tc * cp * INV_DAY, Av * Fc_al * al_tc * cc * INV_DAY,
-          twopif + twopif * g.dt_t * tn_tc * INV_DAY);
-  // Phicoal
-  GWF_SET(8, 0.0, 0.0, -1.0);
-  // spins (and tidal parameters)
+  {
+    const double al = al_tc * INV_DAY;
+    GWF_SET(7, Xp * al, Xc * al, twopif + tdt * (tn_tc * INV_DAY));
+  }
+  // Phicoal: d Psi = -1
+  re[8] = Ac;
+  im[8] = -Ap;
+  // spins (and tidal parameters): waveform amplitude and phase only
   {
     const double dA = wp.A.d[2] * idL;
-    GWF_SET(9, dA * Fp * cp, dA * Fc * cc, -wp.Phi.d[2]);
+    GWF_SET(9, dA * Fpp, dA * Fcc, -wp.Phi.d[2]);
   }
   {
     const double dA = wp.A.d[3] * idL;
-    GWF_SET(10, dA * Fp * cp, dA * Fc * cc, -wp.Phi.d[3]);
+    GWF_SET(10, dA * Fpp, dA * Fcc, -wp.Phi.d[3]);
   }
   if (P > 11) {
     // tidal parameters (phase: TaylorF2, NRTidalv2; amplitude: NRTidalv2
     // tidal amplitude and Planck taper, zero tangent for TaylorF2), or the
     // eccentricity of eccentric TaylorF2 (phase only)
     const double dA4 = wp.A.d[NT > 4 ? 4 : 0] * idL;
-    GWF_SET(11, dA4 * Fp * cp, dA4 * Fc * cc, -wp.Phi.d[NT > 4 ? 4 : 0]);
+    GWF_SET(11, dA4 * Fpp, dA4 * Fcc, -wp.Phi.d[NT > 4 ? 4 : 0]);
   }
   if (P > 12) {
     const double dA5 = wp.A.d[NT > 5 ? 5 : 0] * idL;
-    GWF_SET(12, dA5 * Fp * cp, dA5 * Fc * cc, -wp.Phi.d[NT > 5 ? 5 : 0]);
+    GWF_SET(12, dA5 * Fpp, dA5 * Fcc, -wp.Phi.d[NT > 5 ? 5 : 0]);
   }
   if (P > 13) {  // eccentricity after the tidal rows
     const double dA6 = wp.A.d[NT > 6 ? 6 : 0] * idL;
-    GWF_SET(13, dA6 * Fp * cp, dA6 * Fc * cc, -wp.Phi.d[NT > 6 ? 6 : 0]);
+    GWF_SET(13, dA6 * Fpp, dA6 * Fcc, -wp.Phi.d[NT > 6 ? 6 : 0]);
   }
 #undef GWF_SET
 }
