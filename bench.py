@@ -264,7 +264,9 @@ def cov_leg(F, steps=10):
     out.update({'n_events': n, 'n_params': P, 'events_per_s': n / (out['cov_invert_ms'] * 1e-3),
                 'methods': {'cho': int((m == 0).sum()), 'svd': int((m == 1).sum()), 'nan': int((m == -1).sum())},
                 'median_inversion_error': float(np.nanmedian(e)),
-                'path': 'gwf_cov_invert (k_cov) and gwf_check_fisher (k_check): one wavefront per event, LDS matrices'})
+                'path': 'gwf_cov_invert and gwf_check_fisher: one event per lane with the matrices in registers '
+                         '(k_cov_lane, k_check_lane), the events they leave by one wavefront per event '
+                         '(k_cov, k_check)'})
     return out
 
 

@@ -963,6 +963,294 @@ __device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, in
   }
 }
 
+// ------------------------------------------------ covariance, one per lane
+// CovMatr with invMethodIn = 'cho' (the reference's default) for P = 11-13,
+// one event per lane with its matrices in registers: the upper triangle
+// (P(P+1)/2 doubles) of the Jacobi-scaled matrix A, a cyclic Jacobi in
+// serial order, unrolled (the same rotations and threshold as wave_jacobi),
+// for the eigenvalues and the condition number; then mpmath's Cholesky,
+// the inverse of the factor and C = Ci^T Ci, each in place, the scaled-back
+// C written out, and eps = max|C F - I| with F read a column at a time.
+// The events this path does not finish (a NaN or negative diagonal, a
+// negative eigenvalue or a failed Cholesky, which take the alternative
+// method) get the marker COV_PENDING as their last covariance element and
+// are completed by k_cov<PT, true>, which skips every other event.
+constexpr unsigned long long COV_PENDING = 0x7ff4c0de0000cafeull;  // a NaN payload arithmetic does not produce
+
+template <int P>
+__device__ __forceinline__ constexpr int upk(int i, int j) {  // packed upper triangle, any order of i, j
+  return i <= j ? i * P - i * (i - 1) / 2 + (j - i) : j * P - j * (j - 1) / 2 + (i - j);
+}
+template <int P>
+__device__ __forceinline__ constexpr int lok(int i, int j) {  // packed lower triangle, i >= j
+  return i * (i + 1) / 2 + j;
+}
+
+// Cyclic Jacobi on one lane's packed upper triangle E (pairs in serial order
+// p < q; the rotation formula and threshold of wave_jacobi; sweeps until one
+// rotates nothing).  WANT_V: the rotations are also accumulated into V
+// (row-major P x P, set to I first), so column k of V is the eigenvector of
+// the eigenvalue left in E's diagonal slot k.  Fully unrolled: E and V stay
+// in registers.
+template <int P, bool WANT_V>
+__device__ __forceinline__ void lane_jacobi(double* E, double* V) {
+  if (WANT_V) {
+#pragma unroll
+    for (int i = 0; i < P * P; ++i) V[i] = (i / P == i % P) ? 1.0 : 0.0;
+  }
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0.0, dg = 0.0;
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+      dg += E[upk<P>(i, i)] * E[upk<P>(i, i)];
+#pragma unroll
+      for (int j = i + 1; j < P; ++j) off += E[upk<P>(i, j)] * E[upk<P>(i, j)];
+    }
+    if (!(off > 1e-36 * dg)) break;
+    bool rotated = false;
+#pragma unroll
+    for (int p = 0; p < P - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < P; ++q) {
+        const double apq = E[upk<P>(p, q)], app = E[upk<P>(p, p)], aqq = E[upk<P>(q, q)];
+        if (apq * apq > (MP_TOL * MP_TOL) * fabs(app * aqq)) {
+          const double d = aqq - app, h = 2.0 * apq;
+          const double num = d > 0.0 ? h : (d < 0.0 ? -h : fabs(h));
+          const double tt = num / (fabs(d) + sqrt(d * d + h * h));
+          const double c = rsqrt(tt * tt + 1.0), sn = tt * c;
+#pragma unroll
+          for (int k = 0; k < P; ++k) {
+            if (k == p || k == q) continue;
+            const double akp = E[upk<P>(k, p)], akq = E[upk<P>(k, q)];
+            E[upk<P>(k, p)] = c * akp - sn * akq;
+            E[upk<P>(k, q)] = sn * akp + c * akq;
+          }
+          E[upk<P>(p, p)] = app - tt * apq;
+          E[upk<P>(q, q)] = aqq + tt * apq;
+          E[upk<P>(p, q)] = 0.0;
+          if (WANT_V) {
+#pragma unroll
+            for (int k = 0; k < P; ++k) {
+              const double vkp = V[k * P + p], vkq = V[k * P + q];
+              V[k * P + p] = c * vkp - sn * vkq;
+              V[k * P + q] = sn * vkp + c * vkq;
+            }
+          }
+          rotated = true;
+        }
+      }
+    }
+    if (!rotated) break;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(64, P <= 12 ? 2 : 1) void k_cov_lane(const double* __restrict__ F, int64_t n_ev,
+                                                 double* __restrict__ C, double* __restrict__ eps,
+                                                 double* __restrict__ cond, int* __restrict__ method) {
+  constexpr int NU = P * (P + 1) / 2;
+  const int64_t e0 = (int64_t)blockIdx.x * 64;  // the wave's first event (uniform)
+  const int lane = threadIdx.x;
+  const int64_t e = e0 + lane;
+  if (e >= n_ev) return;
+  // element (i, j) of event e: a wave-uniform base plus the lane, so each
+  // load or store needs a scalar address and one shared lane offset
+  const double* Fb = F + e0;
+  double* Cb = C + e0;
+  // (the offset passes through an empty asm statement so that it is formed
+  // in scalar registers at each use, not kept per element in vector ones)
+  auto uoff = [&](int i, int j) {
+    int64_t off = (int64_t)(i * P + j) * n_ev;
+    __asm__ volatile("" : "+s"(off));
+    return off;
+  };
+  auto f_at = [&](int i, int j) { return Fb[uoff(i, j) + lane]; };
+  auto pending = [&]() { Cb[uoff(P - 1, P - 1) + lane] = __longlong_as_double((long long)COV_PENDING); };
+  // NaN anywhere, or a negative diagonal: the wavefront kernel writes the NaN outputs
+  bool bad = false, zero = false;
+#pragma unroll
+  for (int i = 0; i < P * P; ++i) bad |= isnan(f_at(i / P, i % P));
+  double ws[P];
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const double d = f_at(i, i);
+    bad |= d < 0.0;
+    zero |= d == 0.0;
+  }
+  if (bad) {
+    pending();
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < P; ++i) ws[i] = zero ? 1.0 : 1.0 / sqrt(f_at(i, i));
+  __asm__ volatile("" ::: "memory");
+  double E[NU];
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+#pragma unroll
+    for (int j = i; j < P; ++j) E[upk<P>(i, j)] = ws[i] * f_at(i, j) * ws[j];
+  lane_jacobi<P, false>(E, nullptr);
+  double emax = 0.0, emin = INFINITY;
+  bool neg = false;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const double w = E[upk<P>(i, i)];
+    emax = fmax(emax, fabs(w));
+    emin = fmin(emin, fabs(w));
+    neg = neg || (w < 0.0);
+  }
+  if (cond) cond[e] = emax / emin;
+  if (neg) {  // 'cho' switches to the alternative method
+    pending();
+    return;
+  }
+  // mpmath.cholesky of A (its lower triangle from F's lower triangle), in place.
+  // (The barrier makes F be read again here rather than kept in registers
+  // through the Jacobi sweeps.)
+  __asm__ volatile("" ::: "memory");
+  double L[NU];
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+#pragma unroll
+    for (int j = 0; j <= i; ++j) L[lok<P>(i, j)] = ws[i] * f_at(i, j) * ws[j];
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    double d = L[lok<P>(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= L[lok<P>(j, k)] * L[lok<P>(j, k)];
+    ok = ok && (d >= MP_TOL);
+    const double ljj = sqrt(d);
+    L[lok<P>(j, j)] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < P; ++i) {
+      double s = L[lok<P>(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) s -= L[lok<P>(i, k)] * L[lok<P>(j, k)];
+      L[lok<P>(i, j)] = s / ljj;
+    }
+  }
+  if (!ok) {  // a pivot below 2^-52: not positive-definite for mpmath
+    pending();
+    return;
+  }
+  // Ci = L^{-1}, column by column upward, each column replacing L's
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    double col[P];
+    col[j] = 1.0 / L[lok<P>(j, j)];
+#pragma unroll
+    for (int i = j + 1; i < P; ++i) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = j; k < i; ++k) s -= L[lok<P>(i, k)] * (k == j ? col[j] : col[k]);
+      col[i] = s / L[lok<P>(i, i)];
+    }
+#pragma unroll
+    for (int i = j; i < P; ++i) L[lok<P>(i, j)] = col[i];
+  }
+  // C = Ci^T Ci row by row, row i replacing Ci's column i; scaled back and written
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    double row[P];
+#pragma unroll
+    for (int j = i; j < P; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = j; k < P; ++k) s += L[lok<P>(k, i)] * L[lok<P>(k, j)];
+      row[j] = s;
+    }
+#pragma unroll
+    for (int j = i; j < P; ++j) {
+      const double c = ws[i] * row[j] * ws[j];
+      L[lok<P>(j, i)] = c;  // now C[i][j] (= C[j][i]) at the lower slot (j, i)
+      Cb[uoff(i, j) + lane] = c;
+      if (j != i) Cb[uoff(j, i) + lane] = c;
+    }
+  }
+  if (eps) {  // max |C F - I|, F a column at a time
+    __asm__ volatile("" ::: "memory");
+    double mx = 0.0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      double fc[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) fc[k] = f_at(k, j);
+#pragma unroll
+      for (int i = 0; i < P; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < P; ++k) s += L[i >= k ? lok<P>(i, k) : lok<P>(k, i)] * fc[k];
+        mx = fmax(mx, fabs(s - (i == j ? 1.0 : 0.0)));
+      }
+    }
+    eps[e] = mx;
+  }
+  if (method) method[e] = GWF_INV_CHO;
+}
+
+// fisherTools.CheckFisher (fisherTools.py:216-279) for P = 11, 12: one event
+// per lane, the upper triangle of F and the eigenvectors in registers
+// (lane_jacobi), eigenvalues ranked ascending as k_check ranks them.  Events
+// with a NaN entry get the marker COV_PENDING as their condition number and
+// are completed by k_check<PT, true>, which skips every other event.
+template <int P, bool WANT_V>
+__global__ __launch_bounds__(64, 1) void k_check_lane(const double* __restrict__ F, int64_t n_ev,
+                                                       double* __restrict__ evals, double* __restrict__ evecs,
+                                                       double* __restrict__ cond) {
+  constexpr int NU = P * (P + 1) / 2;
+  const int64_t e0 = (int64_t)blockIdx.x * 64;  // the wave's first event (uniform)
+  const int lane = threadIdx.x;
+  const int64_t e = e0 + lane;
+  if (e >= n_ev) return;
+  const double* Fb = F + e0;
+  auto f_at = [&](int i, int j) {
+    int64_t off = (int64_t)(i * P + j) * n_ev;  // scalar, as in k_cov_lane
+    __asm__ volatile("" : "+s"(off));
+    return Fb[off + lane];
+  };
+  bool bad = false;
+#pragma unroll
+  for (int i = 0; i < P * P; ++i) bad |= isnan(f_at(i / P, i % P));
+  if (bad) {
+    cond[e] = __longlong_as_double((long long)COV_PENDING);
+    return;
+  }
+  double E[NU], V[WANT_V ? P * P : 1];
+#pragma unroll
+  for (int i = 0; i < P; ++i)
+#pragma unroll
+    for (int j = i; j < P; ++j) E[upk<P>(i, j)] = f_at(i, j);
+  lane_jacobi<P, WANT_V>(E, V);
+  double emax = 0.0, emin = INFINITY;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    emax = fmax(emax, fabs(E[upk<P>(i, i)]));
+    emin = fmin(emin, fabs(E[upk<P>(i, i)]));
+  }
+  double* ev = evals + e * P;
+  double* vv = WANT_V ? evecs + e * P * P : nullptr;
+#pragma unroll
+  for (int i = 0; i < P; ++i) {
+    const double wi = E[upk<P>(i, i)];
+    const bool ni = isnan(wi);
+    int rk = 0;
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const double wj = E[upk<P>(j, j)];
+      const bool nj = isnan(wj);
+      const bool before = ni ? (nj ? j < i : true) : (!nj && (wj < wi || (wj == wi && j < i)));
+      rk += before ? 1 : 0;
+    }
+    ev[rk] = wi;
+    if (WANT_V) {
+#pragma unroll
+      for (int k = 0; k < P; ++k) vv[k * P + rk] = V[k * P + i];
+    }
+  }
+  cond[e] = emax / emin;
+}
+
 // fisherTools.CovMatr per event (fisherTools.py:53-196).  meth / alt:
 // GWF_INV_* ids.  Steps: Jacobi scaling ws = 1/sqrt(F_ii) (a zero diagonal
 // skips it, a negative one or any NaN gives NaN); eigenvalues of the scaled
@@ -970,7 +1258,7 @@ __device__ void wave_jacobi(double*& E, double*& E2, double*& V, double*& V2, in
 // switches 'cho' to alt; then the chosen inverse of A, symmetrised and
 // unscaled; eps = max|C F - I| against F, or against the truncated scaled
 // matrix when 'svd' truncates (as the reference stores it, 128-136).
-template <int PT>
+template <int PT, bool PENDING_ONLY = false>
 __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Prt, int64_t n_ev, int meth_in, int alt,
                                             int truncate, double svals_thresh, double cond_max, double* __restrict__ C,
                                             double* __restrict__ eps, double* __restrict__ cond,
@@ -982,6 +1270,8 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Pr
   __shared__ int vpiv[PMAX], vperm[PMAX];
   const int64_t e = xcd_event();
   if (e >= n_ev) return;  // the block is this one wave
+  // PENDING_ONLY: only the events k_cov_lane left (marker in the last element)
+  if (PENDING_ONLY && __double_as_longlong(C[(int64_t)(P * P - 1) * n_ev + e]) != (long long)COV_PENDING) return;
   const int lane = threadIdx.x;
   double *Fm = sm, *A = sm + CMAT, *E = sm + 2 * CMAT, *E2 = sm + 3 * CMAT, *V = sm + 4 * CMAT, *V2 = sm + 5 * CMAT;
   for (int t = lane; t < 6 * CMAT; t += 64) sm[t] = 0.0;
@@ -1023,7 +1313,15 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Pr
     AT(E, i, j) = a;
   }
   __syncthreads();
-  wave_jacobi<PT, false>(E, E2, V, V2, P, vw, vcc, vss, lane);
+  // PENDING_ONLY with 'cho': k_cov_lane has already found that Cholesky
+  // does not apply, so the alternative method is taken; when that is an
+  // SVD, one Jacobi run with eigenvectors serves both it and the condition
+  // number
+  const bool vecs_first = PENDING_ONLY && meth_in == GWF_INV_CHO && (alt == GWF_INV_SVD || alt == GWF_INV_SVD_REG);
+  if (vecs_first)
+    wave_jacobi<PT, true>(E, E2, V, V2, P, vw, vcc, vss, lane);
+  else
+    wave_jacobi<PT, false>(E, E2, V, V2, P, vw, vcc, vss, lane);
   double emax = 0.0, emin = INFINITY;
   bool neg = false;
   for (int i = 0; i < P; ++i) {
@@ -1033,10 +1331,11 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Pr
   }
   const double cnd = emax / emin;
   if (cond && lane == 0) cond[e] = cnd;
-  // E, E2, V, V2 are free from here (the svd methods compute the eigenvectors)
+  // E, E2, V, V2 are free from here (the svd methods compute the eigenvectors,
+  // or with vecs_first hold them in V)
   int m = meth_in;
   if (m == GWF_INV_CHO) {
-    if (neg) {
+    if (neg || PENDING_ONLY) {
       m = alt;
     } else {  // mpmath.cholesky: a pivot below 2^-52 is not positive-definite
       double* L = E;
@@ -1087,9 +1386,11 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Pr
     // SVD of the symmetric A from its eigenpairs: singular values |w_k|,
     // A^+ = sum_k sign(w_k) v_k v_k^T / |w_k|.  The eigenvectors: the same
     // Jacobi run again on A, with V (the values come out the same)
-    for (int t = lane; t < P * P; t += 64) AT(E, t / P, t - (t / P) * P) = AT(A, t / P, t - (t / P) * P);
-    __syncthreads();
-    wave_jacobi<PT, true>(E, E2, V, V2, P, vw, vcc, vss, lane);
+    if (!vecs_first) {
+      for (int t = lane; t < P * P; t += 64) AT(E, t / P, t - (t / P) * P) = AT(A, t / P, t - (t / P) * P);
+      __syncthreads();
+      wave_jacobi<PT, true>(E, E2, V, V2, P, vw, vcc, vss, lane);
+    }
     double smax = 0.0;
     for (int k = 0; k < P; ++k) smax = fmax(smax, fabs(vw[k]));
     const bool trunc = (m == GWF_INV_SVD) && truncate && fabs(cnd) > cond_max;
@@ -1262,7 +1563,7 @@ __global__ __launch_bounds__(64) void k_cov(const double* __restrict__ F, int Pr
 // fisherTools.CheckFisher per event (fisherTools.py:216-279): eigenvalues
 // (ascending), eigenvectors (evecs[e][i][k] = component i of vector k) and
 // the condition number max|w| / min|w|; an all-NaN matrix gives NaN.
-template <int PT>
+template <int PT, bool PENDING_ONLY = false>
 __global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int Prt, int64_t n_ev,
                                               double* __restrict__ evals, double* __restrict__ evecs,
                                               double* __restrict__ cond) {
@@ -1272,6 +1573,8 @@ __global__ __launch_bounds__(64) void k_check(const double* __restrict__ F, int 
   __shared__ double vw[PMAX], vcc[PMAX], vss[PMAX];
   const int64_t e = xcd_event();
   if (e >= n_ev) return;
+  // PENDING_ONLY: only the events k_check_lane left (marker as condition number)
+  if (PENDING_ONLY && __double_as_longlong(cond[e]) != (long long)COV_PENDING) return;
   const int lane = threadIdx.x;
   double *E = sm, *E2 = sm + CMAT, *V = sm + 2 * CMAT, *V2 = sm + 3 * CMAT;
   for (int t = lane; t < 4 * CMAT; t += 64) sm[t] = 0.0;
@@ -2085,17 +2388,36 @@ int gwf_cov_invert_ex(const double* F, int32_t P, int64_t n_ev, int32_t inv_meth
     return fail(-EINVAL, "alt_method has to be different from invMethodIn");
   if (n_ev == 0) return 0;
   if (!F || !cov_out) return fail(-EINVAL, "NULL array");
+  hipStream_t st = (hipStream_t)stream;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev,
-                       (int)inv_method, (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out,
-                       eps_out, cond_out, (int*)method_out);
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, st, F, (int)P, n_ev, (int)inv_method,
+                       (int)alt_method, (int)(truncate != 0), svals_thresh, cond_numb_max, cov_out, eps_out, cond_out,
+                       (int*)method_out);
   };
-  switch (P) {  // the path's sizes at compile time
-    case 11: go(k_cov<11>); break;
-    case 12: go(k_cov<12>); break;
-    case 13: go(k_cov<13>); break;
-    case 14: go(k_cov<14>); break;
-    default: go(k_cov<0>);
+  // 'cho' at the path's sizes: one event per lane (k_cov_lane), the events
+  // it leaves (alternative method, NaN outputs) by the wavefront kernel.
+  // The lane kernel reads F after writing C, so it needs them apart.
+  const char *f0 = (const char*)F, *f1 = f0 + (size_t)P * P * n_ev * sizeof(double);
+  const char *c0 = (const char*)cov_out, *c1 = c0 + (size_t)P * P * n_ev * sizeof(double);
+  const bool lane_path = inv_method == GWF_INV_CHO && P >= 11 && P <= 13 && (f1 <= c0 || c1 <= f0);
+  auto lane = [&](auto klane) {
+    hipLaunchKernelGGL(klane, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, st, F, n_ev, cov_out, eps_out,
+                       cond_out, (int*)method_out);
+  };
+  if (lane_path) {
+    switch (P) {
+      case 11: lane(k_cov_lane<11>); go(k_cov<11, true>); break;
+      case 12: lane(k_cov_lane<12>); go(k_cov<12, true>); break;
+      default: lane(k_cov_lane<13>); go(k_cov<13, true>);
+    }
+  } else {
+    switch (P) {  // the path's sizes at compile time
+      case 11: go(k_cov<11>); break;
+      case 12: go(k_cov<12>); break;
+      case 13: go(k_cov<13>); break;
+      case 14: go(k_cov<14>); break;
+      default: go(k_cov<0>);
+    }
   }
   return hip_check(hipGetLastError(), "gwf_cov_invert launch");
 }
@@ -2122,16 +2444,44 @@ int gwf_check_fisher(const double* F, int32_t P, int64_t n_ev, double* evals_out
   if (n_ev < 0) return fail(-EINVAL, "n_ev must be >= 0");
   if (n_ev == 0) return 0;
   if (!F || !evals_out || !cond_out) return fail(-EINVAL, "NULL array");
+  hipStream_t st = (hipStream_t)stream;
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, (hipStream_t)stream, F, (int)P, n_ev, evals_out,
-                       evecs_out, cond_out);
+    hipLaunchKernelGGL(kern, dim3(xcd_grid(n_ev)), dim3(64), 0, st, F, (int)P, n_ev, evals_out, evecs_out,
+                       cond_out);
   };
-  switch (P) {
-    case 11: go(k_check<11>); break;
-    case 12: go(k_check<12>); break;
-    case 13: go(k_check<13>); break;
-    case 14: go(k_check<14>); break;
-    default: go(k_check<0>);
+  // P = 11, 12: one event per lane (k_check_lane), the events with a NaN
+  // entry by the wavefront kernel; the lane kernel needs F apart from the
+  // outputs
+  auto apart = [&](const void* p, size_t bytes) {
+    const char *a0 = (const char*)F, *a1 = a0 + (size_t)P * P * n_ev * sizeof(double);
+    const char *b0 = (const char*)p, *b1 = b0 + bytes;
+    return !p || a1 <= b0 || b1 <= a0;
+  };
+  const size_t nb = (size_t)n_ev * sizeof(double);
+  const bool lane_path = (P == 11 || P == 12) && apart(evals_out, nb * P) && apart(evecs_out, nb * P * P) &&
+                         apart(cond_out, nb);
+  auto lane = [&](auto klane) {
+    hipLaunchKernelGGL(klane, dim3((unsigned)((n_ev + 63) / 64)), dim3(64), 0, st, F, n_ev, evals_out, evecs_out,
+                       cond_out);
+  };
+  if (lane_path) {
+    if (P == 11) {
+      if (evecs_out) lane(k_check_lane<11, true>);
+      else lane(k_check_lane<11, false>);
+      go(k_check<11, true>);
+    } else {
+      if (evecs_out) lane(k_check_lane<12, true>);
+      else lane(k_check_lane<12, false>);
+      go(k_check<12, true>);
+    }
+  } else {
+    switch (P) {
+      case 11: go(k_check<11>); break;
+      case 12: go(k_check<12>); break;
+      case 13: go(k_check<13>); break;
+      case 14: go(k_check<14>); break;
+      default: go(k_check<0>);
+    }
   }
   return hip_check(hipGetLastError(), "gwf_check_fisher launch");
 }

@@ -256,6 +256,75 @@ def test_covmatr_methods_vs_oracle(gpu, case):
     assert np.allclose(eps, epso, rtol=1e-3, atol=1e-10)
 
 
+def _graded_spd(rng, P, n, spread):
+    """n symmetric positive-definite P x P matrices (P, P, n) with diagonal
+    scales spread over `spread` decades, like Fisher matrices."""
+    X = rng.standard_normal((n, P, P))
+    A = X @ np.transpose(X, (0, 2, 1)) + P * np.eye(P)
+    s = np.geomspace(1.0, 10.0 ** spread, P)
+    return np.transpose(A, (1, 2, 0)) * s[:, None, None] * s[None, :, None]
+
+
+@pytest.mark.parametrize('P,alt', [(11, 'svd'), (12, 'svd'), (13, 'svd'), (12, 'inv'), (12, 'lu'), (12, 'svd_reg')])
+def test_covmatr_lane_path_vs_oracle(gpu, P, alt):
+    """'cho' at 11-13 parameters runs one event per lane; the events it
+    leaves (an indefinite matrix takes alt_method, an all-NaN one gives NaN)
+    are finished by the wavefront kernel.  A mixed batch against the
+    oracle's mpmath restatement (fisherTools.py:53-196)."""
+    from gwfast_amd import fisherTools as FT
+    from oracle import covariance as OC
+    rng = np.random.default_rng(50 + P)
+    F = _graded_spd(rng, P, 6, 5)
+    B = np.diag(np.linspace(1.0, 2.0, P))
+    B[0, 1] = B[1, 0] = 5.0                     # indefinite 2 x 2 block
+    F[:, :, 2] = B
+    F[:, :, 4] = np.nan
+    C, eps, cond, meth = FT.CovMatr(F, return_details=True, alt_method=alt)
+    Co, epso, metho = OC.cov_matr(F, alt_method=alt)
+    assert np.array_equal(meth, metho), (meth, metho)
+    assert meth[2] == OC.METHOD_IDS[alt] and meth[4] == -1 and np.all(meth[[0, 1, 3, 5]] == 0)
+    assert np.all(np.isnan(C[:, :, 4])) and np.isnan(eps[4]) and np.isnan(cond[4])
+    ok = meth >= 0
+    scale = np.max(np.abs(Co[:, :, ok]), axis=(0, 1))
+    err = np.max(np.abs(C[:, :, ok] - Co[:, :, ok]) / scale)
+    print(P, alt, 'cov err %.3e' % err, flush=True)
+    assert err <= 1e-9
+    assert np.allclose(eps[ok], epso[ok], rtol=1e-3, atol=1e-10)
+
+
+@pytest.mark.parametrize('P', [11, 12, 13])
+def test_check_fisher_lane_and_wavefront_paths(gpu, P):
+    """CheckFisher at 11 and 12 parameters runs one event per lane, at 13 one
+    wavefront per event; an all-NaN event goes to the wavefront kernel.
+    Eigenvalues against LAPACK (numpy.linalg.eigh) and the oracle's mpmath
+    restatement (fisherTools.py:216-279), eigenvector residuals and
+    orthonormality, ascending order, condition numbers."""
+    from gwfast_amd import fisherTools as FT
+    from oracle import covariance as OC
+    rng = np.random.default_rng(60 + P)
+    n = 70                                      # more than one wave of lanes
+    F = _graded_spd(rng, P, n, 6)
+    F[:, :, 65] = np.nan
+    ev, vec, cond = FT.CheckFisher(F)
+    assert np.all(np.isnan(ev[65])) and np.all(np.isnan(vec[65])) and np.isnan(cond[65])
+    keep = np.arange(n) != 65
+    Ft = np.transpose(F[:, :, keep], (2, 0, 1))
+    wn = np.linalg.eigvalsh(Ft)
+    lam_max = np.max(np.abs(wn), axis=1, keepdims=True)
+    assert np.max(np.abs(ev[keep] - wn) / lam_max) <= 1e-12
+    assert np.all(np.diff(ev[keep], axis=1) >= 0)
+    V = vec[keep]
+    r = Ft @ V - V * ev[keep][:, None, :]
+    assert np.max(np.abs(r) / lam_max[:, :, None]) <= 1e-10
+    assert np.max(np.abs(np.transpose(V, (0, 2, 1)) @ V - np.eye(P))) <= 1e-12
+    assert np.allclose(cond[keep], np.max(np.abs(ev[keep]), axis=1) / np.min(np.abs(ev[keep]), axis=1), rtol=0,
+                       atol=0)
+    evo, _, condo = OC.check_fisher(F[:, :, :3])
+    assert np.max(np.abs(ev[:3] - evo) / lam_max[:3]) <= 1e-12
+    # the smallest eigenvalue carries an absolute error of order eps * lam_max
+    assert np.all(np.abs(cond[:3] - condo) <= 1e-14 * cond[:3] * condo)
+
+
 @pytest.mark.parametrize('model', ['TF2', 'PD'])
 def test_fisher_linear_spacing_and_df(gpu, model):
     """FisherMatr(spacing='lin') (signal.py:893-894) against the oracle on
