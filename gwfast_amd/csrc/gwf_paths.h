@@ -24,6 +24,9 @@ constexpr double INV_DAY = 1.0 / (3600. * 24.);
 enum EvConst {
   EV_IDL = 0, EV_DL, EV_CP, EV_CC, EV_DCP, EV_DCC, EV_TC, EV_RA, EV_FCUT, EV_PHIC, EV_CRA, EV_SRA,
   EV_LMG, EV_LMG_D0, EV_LMG_D1,  // ln(M G) and its (Mc, eta) tangents
+  // tau_star = K v^-8 (1 + c2 v^2 + c3 v^3 + ... + (c6 + c6L ln v) v^6 + c7 v^7):
+  // K = (5/256) M G / eta over (Mc, eta), the eta-dependent c_k and d c_k / d eta
+  EV_TK, EV_TK_D0, EV_TK_D1, EV_TC2, EV_TC4, EV_TC4_D1, EV_TC5, EV_TC6, EV_TC6_D1, EV_TC7, EV_TC7_D1,
   EV_COUNT
 };
 enum SiteConst {
@@ -295,6 +298,70 @@ GWF_HD void setup_event(const gwf_waveform& w, const NetK& net, const double* pa
   setup_event_consts<MODEL>(net, par, ld, e, rec);
 }
 
+// tau_star (waveforms.py:878-901, 1299-1322) as K v^-8 P(v), P the polynomial of the
+// 0-3.5PN terms in v = (pi M G f)^(1/3): the eta-dependent coefficients and
+// K = (5/256) M G / eta once per event (mg: the record's M G dual, whose
+// first two tangents are d/dMc and d/deta), so a grid point evaluates P by
+// Horner's rule instead of re-forming the eta polynomials.
+constexpr double TAU_C3 = -32. / 5. * PI;
+constexpr double TAU_C2_D1 = 11. / 3.;
+constexpr double TAU_C5_D1 = 13. / 3. * PI;
+constexpr double TAU_C6L = 3424. / 105. * 2.0;  // of ln v
+
+GWF_HD void tau_event_coeffs(const double* mg, double eta, double* ev) {
+  const double eta2 = eta * eta;
+  const double k = 5. / 256.;
+  ev[EV_TK] = k * mg[0] / eta;
+  ev[EV_TK_D0] = k * mg[1] / eta;
+  ev[EV_TK_D1] = k * (mg[2] / eta - mg[0] / eta2);
+  ev[EV_TC2] = 743. / 252. + 11. / 3. * eta;
+  ev[EV_TC4] = 3058673. / 508032. + 5429. / 504. * eta + 617. / 72. * eta2;
+  ev[EV_TC4_D1] = 5429. / 504. + 2. * 617. / 72. * eta;
+  ev[EV_TC5] = -(7729. / 252. - 13. / 3. * eta) * PI;
+  const double c6e = 3147553127. / 3048192. - 451. / 12. * PI * PI;
+  ev[EV_TC6] = -10052469856691. / 23471078400. + 128. / 3. * PI * PI + 6848. / 105. * EULER_GAMMA + c6e * eta -
+               15211. / 1728. * eta2 + 25565. / 1296. * eta2 * eta + 3424. / 105. * log(16.);
+  ev[EV_TC6_D1] = c6e - 2. * 15211. / 1728. * eta + 3. * 25565. / 1296. * eta2;
+  ev[EV_TC7] = (-15419335. / 127008. - 75703. / 756. * eta + 14809. / 378. * eta2) * PI;
+  ev[EV_TC7_D1] = (-75703. / 756. + 2. * 14809. / 378. * eta) * PI;
+}
+
+// q <- c + u q over (Mc, eta), two FMAs per tangent
+GWF_HD void tau_horner(D<2>& q, const D<2>& u, double c, double c_d1) {
+  const double qv = q.v;
+  q.v = fma(u.v, qv, c);
+  q.d[0] = fma(u.d[0], qv, u.v * q.d[0]);
+  q.d[1] = fma(u.d[1], qv, fma(u.v, q.d[1], c_d1));
+}
+
+GWF_HD D<2> tau_star_ev(const double* ev, const D<2>& v, const D<2>& lv) {
+  D<2> q;
+  q.v = ev[EV_TC7];
+  q.d[0] = 0.0;
+  q.d[1] = ev[EV_TC7_D1];
+  // c6 + c6L ln v carries tangents through ln v
+  const double c6 = fma(TAU_C6L, lv.v, ev[EV_TC6]);
+  {
+    const double qv = q.v;
+    q.v = fma(v.v, qv, c6);
+    q.d[0] = fma(v.d[0], qv, fma(v.v, q.d[0], TAU_C6L * lv.d[0]));
+    q.d[1] = fma(v.d[1], qv, fma(v.v, q.d[1], fma(TAU_C6L, lv.d[1], ev[EV_TC6_D1])));
+  }
+  tau_horner(q, v, ev[EV_TC5], TAU_C5_D1);
+  tau_horner(q, v, ev[EV_TC4], ev[EV_TC4_D1]);
+  tau_horner(q, v, TAU_C3, 0.0);
+  tau_horner(q, v, ev[EV_TC2], TAU_C2_D1);
+  const D<2> v2 = v * v;
+  D<2> p = v2 * q;
+  p.v += 1.0;
+  const D<2> v4 = v2 * v2;
+  D<2> K;
+  K.v = ev[EV_TK];
+  K.d[0] = ev[EV_TK_D0];
+  K.d[1] = ev[EV_TK_D1];
+  return K * p / (v4 * v4);
+}
+
 // The event constants of the record (after its waveform coefficients, whose
 // fcut it reads): inclination factors, 1/dL, tcoal, sky angles, ln(M G) and
 // its (Mc, eta) tangents, and the projection constants of every site.
@@ -323,6 +390,7 @@ GWF_HD void setup_event_consts(const NetK& net, const double* par, int64_t ld, i
     ev[EV_LMG] = log(mg[0]);
     ev[EV_LMG_D0] = 1.0 / Mc;
     ev[EV_LMG_D1] = -0.6 / eta;
+    tau_event_coeffs(mg, eta, ev);
   }
   // sky position and polarisation (ra = phi, dec = pi/2 - theta; gwfastUtils.py:151-164)
   const double ra = phi, dec = 0.5 * PI - theta;
@@ -470,7 +538,6 @@ template <int MODEL, int NT, bool HAVE_LNF = false>
 GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool need_tau, WfPoint<MODEL, NT>& o,
                      double lnf = 0.0) {
   const int smg = is_pd(MODEL) ? (int)PD_MG : (int)TF_MG;
-  const int seta = is_pd(MODEL) ? (int)PD_ETA : (int)TF_ETA;
   const D<2> MG = ldd<2>(rec + smg * SLOT);
   // ln(M G f) from the event's ln(M G) and ln f (its tangents are those of ln M G)
   constexpr bool have_lnf = HAVE_LNF;
@@ -501,7 +568,7 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
     o.A = ldd<NT>(rec + TF_AMP * SLOT) * (1.0 / (f * sqrt(f13)));
   }
   if (need_tau) {
-    o.tau = tau_star_v(MG, ldd<2>(rec + seta * SLOT), v, lv);
+    o.tau = tau_star_ev(rec + REC_EV, v, lv);
   } else {
     o.tau = mkc<2>(0.0);
   }
@@ -537,11 +604,15 @@ GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f,
 // exp(i Psi) / dL (c+ = (1 + cos^2 iota)/2, cx = cos iota), Ap = A F+ c+ / dL,
 // Ac = A Fx cx / dL, every row is (dX Ap - Ac dX Psi) + i (dX Ac + Ap dX Psi);
 // the products A c+ / dL, A cx / dL, F+ c+, Fx cx and the pattern-angle
-// terms A F'_alpha c / dL are formed once and shared by the rows.
+// terms A F'_alpha c / dL are formed once and shared by the rows.  Every
+// row is linear in the amplitude and its tangents, so a scale q of all rows
+// (and of Ap, Ac) costs one multiply, folded into 1/dL: the Fisher sums use
+// q = sqrt(Gram weight) and then need no per-row weighting.
 template <int MODEL, int NT, int P>
 GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const double* sc, const Geo& g, const ArmK& arm,
-                       int motion, double f, double re[P], double im[P], double* Ap_out, double* Ac_out) {
-  const double idL = ev[EV_IDL];
+                       int motion, double f, double re[P], double im[P], double* Ap_out, double* Ac_out,
+                       double q = 1.0) {
+  const double idL0 = ev[EV_IDL], idL = idL0 * q;
   const double cp = ev[EV_CP], cc = ev[EV_CC];
   const double P1 = sc[SC_P1], P2 = sc[SC_P2];
   const double s2x = arm.s2x, c2x = arm.c2x;
@@ -596,8 +667,8 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
     GWF_SET(1, dA * Fpp + Xp * al_eta, dA * Fcc + Xc * al_eta, tdt * tn_eta - wp.Phi.d[1]);
   }
   // dL
-  re[2] = -Ap * idL;
-  im[2] = -Ac * idL;
+  re[2] = -Ap * idL0;
+  im[2] = -Ac * idL0;
   // theta, phi: pattern functions (direct + via the location delay) and phiL
   GWF_SET(3, Avp * (Fp_th + Fp_al * al_th), Avc * (Fc_th + Fc_al * al_th), twopif * g.dt_th);
   GWF_SET(4, Xp * al_ph, Xc * al_ph, twopif * g.dt_ph);
@@ -643,6 +714,8 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
 }
 
 // Symmetric Gram update G += w (re re^T + im im^T), packed upper triangle.
+// Two explicit FMAs per entry: written as G += (wr re + wi im) the compiler
+// may only contract the inner sum (a multiply, an FMA and an add).
 template <int P>
 GWF_HD void gram_update(double* G, const double* re, const double* im, double w) {
   int k = 0;
@@ -651,7 +724,21 @@ GWF_HD void gram_update(double* G, const double* re, const double* im, double w)
     const double wr = w * re[a], wi = w * im[a];
 #pragma unroll
     for (int b = a; b < P; ++b) {
-      G[k] += wr * re[b] + wi * im[b];
+      G[k] = fma(wi, im[b], fma(wr, re[b], G[k]));
+      ++k;
+    }
+  }
+}
+
+// G += re re^T + im im^T for rows already scaled by sqrt(weight).
+template <int P>
+GWF_HD void gram_update_unit(double* G, const double* re, const double* im) {
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < P; ++a) {
+#pragma unroll
+    for (int b = a; b < P; ++b) {
+      G[k] = fma(im[a], im[b], fma(re[a], re[b], G[k]));
       ++k;
     }
   }
@@ -765,9 +852,13 @@ GWF_HD void fisher_event(const gwf_waveform& w, const NetK& net, const double* r
 // per-site geometry, the arm derivative rows and the Gram updates.  Neither
 // wave holds the other's registers (the consumer's Gram accumulators are not
 // live in the producer), so two waves fit on a SIMD.
+// Per point: f, 4 x trapezoid weight, ln f, sqrt(4 wt / S) over the first
+// two PSD tables (SW0, SW1: the consumer scales each arm's rows by it and by
+// the arm's sqrt weight, instead of weighting the Gram), amplitude,
+// tangents, tau and the Earth-rotation increment.
 template <int NT>
 struct PcLayout {
-  static constexpr int F = 0, WT = 1, LNF = 2, S0 = 3, S1 = 4, AV = 5, AD = 6, PD = 6 + NT, TV = 6 + 2 * NT,
+  static constexpr int F = 0, WT = 1, LNF = 2, SW0 = 3, SW1 = 4, AV = 5, AD = 6, PD = 6 + NT, TV = 6 + 2 * NT,
                        TD = 7 + 2 * NT, CD = 9 + 2 * NT, SD = 10 + 2 * NT, NV = 11 + 2 * NT;
 };
 
@@ -796,8 +887,8 @@ GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec
   buf[L::F * 64 + lane] = f;
   buf[L::WT * 64 + lane] = 4.0 * wt;
   buf[L::LNF * 64 + lane] = lnf;
-  buf[L::S0 * 64 + lane] = s0;
-  buf[L::S1 * 64 + lane] = s1;
+  buf[L::SW0 * 64 + lane] = sqrt((4.0 * wt) / s0);
+  buf[L::SW1 * 64 + lane] = sqrt((4.0 * wt) / s1);
   buf[L::AV * 64 + lane] = wp.A.v;
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
@@ -838,8 +929,14 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
   const double cdel = buf[L::CD * 64 + lane], sdel = buf[L::SD * 64 + lane];
   const double tc = ev[EV_TC];
   int cur_site = -1, cur_psd = -1;
-  double Sf = 1.0;
+  double sw = 0.0;  // sqrt(4 wt / S) of the current PSD
   Geo g;
+  // tables past the first two are interpolated here, once per site
+  auto psd_sw = [&](int psd) {
+    return psd == 0 ? buf[L::SW0 * 64 + lane]
+                    : (psd == 1 ? buf[L::SW1 * 64 + lane]
+                                : sqrt(wt / psd_at(net.psd[psd], f, buf[L::LNF * 64 + lane])));
+  };
   if constexpr (MASKED) {
     for (int s = 0; s < net.n_sites; ++s) {
       const SiteK& S = net.site[s];
@@ -848,8 +945,7 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
       if (!bits) continue;
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (S.psd != cur_psd) {
-        Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
-                        : (S.psd == 1 ? buf[L::S1 * 64 + lane] : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
+        sw = psd_sw(S.psd);
         cur_psd = S.psd;
       }
       site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
@@ -859,8 +955,8 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
         if (!full && !((bits >> j) & 1u)) continue;
         const ArmK& arm = full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j];
         double re[P], im[P], Ap, Ac;
-        arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
-        gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+        arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
+        gram_update_unit<P>(G, re, im);
       }
     }
   } else {
@@ -872,16 +968,15 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (s != cur_site) {
         if (S.psd != cur_psd) {
-          Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
-                          : (S.psd == 1 ? buf[L::S1 * 64 + lane] : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
+          sw = psd_sw(S.psd);
           cur_psd = S.psd;
         }
         site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
         cur_site = s;
       }
       double re[P], im[P], Ap, Ac;
-      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
-      gram_update<P>(G, re, im, wt * arm.wgt / Sf);
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
+      gram_update_unit<P>(G, re, im);
     }
   }
 }

@@ -240,7 +240,7 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
   }
   const double tc = ev[EV_TC];
   int cur_site = -1, cur_psd = -1;
-  double sw = 0.0;  // sqrt(wt / S) of the current PSD
+  double sw = 0.0;  // sqrt(4 wt / S) of the current PSD
   Geo g;
   // the scaled rows of one arm of site s (zero rows on dead lanes)
   auto arm_rows = [&](const ArmK& arm, int s, double* re, double* im) {
@@ -249,23 +249,16 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (s != cur_site) {
         if (S.psd != cur_psd) {
-          const double Sf = S.psd == 0 ? buf[L::S0 * 64 + lane]
-                                       : (S.psd == 1 ? buf[L::S1 * 64 + lane]
-                                                     : psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane]));
-          sw = sqrt(wt / Sf);
+          sw = S.psd == 0 ? buf[L::SW0 * 64 + lane]
+                          : (S.psd == 1 ? buf[L::SW1 * 64 + lane]
+                                        : sqrt(wt / psd_at(net.psd[S.psd], f, buf[L::LNF * 64 + lane])));
           cur_psd = S.psd;
         }
         site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
         cur_site = s;
       }
       double Ap, Ac;
-      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac);
-      const double q = sw * arm.swgt;
-#pragma unroll
-      for (int i = 0; i < P; ++i) {
-        re[i] *= q;
-        im[i] *= q;
-      }
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
     } else {
 #pragma unroll
       for (int i = 0; i < P; ++i) re[i] = im[i] = 0.0;

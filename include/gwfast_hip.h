@@ -135,7 +135,7 @@ int gwf_fisher(const gwf_waveform* wf, const gwf_detector* dets, int32_t n_det, 
  * device memory (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
  * event records and queue counters live in a file-scope device array of the
- * library, sized for GWF_NOALLOC_MAX_EVENTS events (about 45.5 MB, held by
+ * library, sized for GWF_NOALLOC_MAX_EVENTS events (about 46.3 MB, held by
  * each GPU the library runs on until the process ends); larger n_ev returns
  * -EINVAL.  The call allocates nothing; the caller passes the inputs and
  * fisher_out as for gwf_fisher, and gets the same results.  Every call on
