@@ -230,6 +230,21 @@ GWF_HD D<N> operator/(const D<M>& a, const D<N>& b) {
   return r;
 }
 
+// Horner step q <- c + u q with a D<N> accumulator and coefficient and a
+// D<M> variable (M <= N; u's last N - M tangents are zero), in FMAs only:
+// the sum-of-products form costs a multiply, an FMA and an add per full
+// tangent and term.
+template <int N, int M>
+GWF_HD void dhorner(D<N>& q, const D<M>& u, const D<N>& c) {
+  static_assert(M <= N, "variable with more tangents than the accumulator");
+  const double qv = q.v;
+  q.v = fma(u.v, qv, c.v);
+#pragma unroll
+  for (int k = 0; k < M; ++k) q.d[k] = fma(u.d[k], qv, fma(u.v, q.d[k], c.d[k]));
+#pragma unroll
+  for (int k = M; k < N; ++k) q.d[k] = fma(u.v, q.d[k], c.d[k]);
+}
+
 // ------------------------------------------------------------- elementaries
 // Each takes the value-level result and its derivative factor.
 template <int N>

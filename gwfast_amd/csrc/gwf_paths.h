@@ -914,7 +914,7 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
 #endif
   if (base + lane >= res) return;
   const double* ev = rec + REC_EV;
-  const double f = buf[L::F * 64 + lane], wt = buf[L::WT * 64 + lane];
+  const double f = buf[L::F * 64 + lane];
   WfPoint<MODEL, NT> wp;
   wp.A.v = buf[L::AV * 64 + lane];
   wp.Phi.v = 0.0;
@@ -923,10 +923,10 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
     wp.A.d[k] = buf[(L::AD + k) * 64 + lane];
     wp.Phi.d[k] = buf[(L::PD + k) * 64 + lane];
   }
-  wp.tau.v = buf[L::TV * 64 + lane];
-  wp.tau.d[0] = buf[L::TD * 64 + lane];
-  wp.tau.d[1] = buf[(L::TD + 1) * 64 + lane];
-  const double cdel = buf[L::CD * 64 + lane], sdel = buf[L::SD * 64 + lane];
+  // values needed once per site or per arm are read from the chunk where
+  // they are used, which leaves the compiler free to reload them instead of
+  // spilling (the Gram accumulators and an arm's rows fill the registers)
+  auto ld = [&](int slot) { return buf[slot * 64 + lane]; };
   const double tc = ev[EV_TC];
   int cur_site = -1, cur_psd = -1;
   double sw = 0.0;  // sqrt(4 wt / S) of the current PSD
@@ -935,7 +935,7 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
   auto psd_sw = [&](int psd) {
     return psd == 0 ? buf[L::SW0 * 64 + lane]
                     : (psd == 1 ? buf[L::SW1 * 64 + lane]
-                                : sqrt(wt / psd_at(net.psd[psd], f, buf[L::LNF * 64 + lane])));
+                                : sqrt(ld(L::WT) / psd_at(net.psd[psd], f, ld(L::LNF))));
   };
   if constexpr (MASKED) {
     for (int s = 0; s < net.n_sites; ++s) {
@@ -948,13 +948,15 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
         sw = psd_sw(S.psd);
         cur_psd = S.psd;
       }
-      site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
+      site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
       const bool full = bits == all;
       const int cnt = full ? S.nsum : S.narm;
       for (int j = 0; j < cnt; ++j) {
         if (!full && !((bits >> j) & 1u)) continue;
         const ArmK& arm = full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j];
         double re[P], im[P], Ap, Ac;
+        wp.tau.d[0] = ld(L::TD);
+        wp.tau.d[1] = ld(L::TD + 1);
         arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
         gram_update_unit<P>(G, re, im);
       }
@@ -971,10 +973,12 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
           sw = psd_sw(S.psd);
           cur_psd = S.psd;
         }
-        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, wp.tau.v), cdel, sdel, g);
+        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
         cur_site = s;
       }
       double re[P], im[P], Ap, Ac;
+      wp.tau.d[0] = ld(L::TD);
+      wp.tau.d[1] = ld(L::TD + 1);
       arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
       gram_update_unit<P>(G, re, im);
     }

@@ -651,15 +651,26 @@ GWF_HD D<N> pd_phi_at(const double* rec, int stride, const X& x, const X& x13, c
   const T ieta = GWF_L(PD_IETA);
   T phis;
   if (xv < 0.018) {
+    // the inspiral phase as polynomials in x^(1/3) and x^(-1/3) (Horner):
+    // PIP + P13 x^1/3 + P23 x^2/3 + (S1 x + S43 x^4/3 + S53 x^5/3 + S2 x^2) / eta
+    // + PM13 x^-1/3 + PM23 x^-2/3 + PM1 x^-1 + PM53 x^-5/3 + (P13L x^1/3 + PLOG) ln(pi x) / 3
     const auto lpx3 = (lx + log(PI)) * (1.0 / 3.0);
-    const auto x23 = x13 * x13;
     const auto ix13 = 1.0 / x13;
-    const auto ix23 = ix13 * ix13;
-    const auto ix = ix13 * ix23;
-    phis = GWF_L(PD_PIP) + GWF_L(PD_P23) * x23 + GWF_L(PD_P13) * x13 + GWF_L(PD_P13L) * x13 * lpx3 +
-           GWF_L(PD_PLOG) * lpx3 + GWF_L(PD_PM13) * ix13 + GWF_L(PD_PM23) * ix23 + GWF_L(PD_PM1) * ix +
-           GWF_L(PD_PM53) * (ix * ix23) +
-           (GWF_L(PD_S1) * x + GWF_L(PD_S43) * (x * x13) + GWF_L(PD_S53) * (x * x23) + GWF_L(PD_S2) * x * x) * ieta;
+    T pos = GWF_L(PD_S2);
+    dhorner(pos, x13, GWF_L(PD_S53));
+    dhorner(pos, x13, GWF_L(PD_S43));
+    dhorner(pos, x13, GWF_L(PD_S1));
+    pos = pos * ieta;
+    dhorner(pos, x13, GWF_L(PD_P23));
+    dhorner(pos, x13, GWF_L(PD_P13));
+    dhorner(pos, x13, GWF_L(PD_PIP));
+    T neg = GWF_L(PD_PM53) * ix13;  // no x^-4/3 term
+    dhorner(neg, ix13, GWF_L(PD_PM1));
+    dhorner(neg, ix13, GWF_L(PD_PM23));
+    dhorner(neg, ix13, GWF_L(PD_PM13));
+    T lg = GWF_L(PD_P13L);
+    dhorner(lg, x13, GWF_L(PD_PLOG));
+    phis = pos + neg * ix13 + lg * lpx3;
   } else if (xv < rec[PD_FMRDJ * stride]) {
     const auto ix = 1.0 / x;
     phis = (GWF_L(PD_B1) * x - GWF_L(PD_B3) * (ix * ix * ix) * (1.0 / 3.0) + GWF_L(PD_B2) * lx) * ieta +
@@ -692,15 +703,17 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const X& x, const X& x13, b
   const double xv = x.v;
   T amp;
   if (xv < 0.014) {
-    const auto x23 = x13 * x13;
-    const auto x43 = x23 * x23;
-    const auto x53 = x43 * x13;
-    const auto x73 = x53 * x23;
-    const auto x83 = x73 * x13;
-    amp = 1. + x23 * ldd<N>(rec + PD_AM23 * stride) + x43 * ldd<N>(rec + PD_AM43 * stride) +
-          x53 * ldd<N>(rec + PD_AM53 * stride) + x73 * ldd<N>(rec + PD_AM73 * stride) +
-          x83 * ldd<N>(rec + PD_AM83 * stride) +
-          x * (ldd<N>(rec + PD_AM1 * stride) + x * ldd<N>(rec + PD_AM2 * stride) + x * x * ldd<N>(rec + PD_AM3 * stride));
+    // 1 + x^2/3 (AM23 + AM1 x^1/3 + AM43 x^2/3 + AM53 x + AM2 x^4/3 + AM73 x^5/3
+    // + AM83 x^2 + AM3 x^7/3), the polynomial in x^(1/3) by Horner's rule
+    T q = ldd<N>(rec + PD_AM3 * stride);
+    dhorner(q, x13, ldd<N>(rec + PD_AM83 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM73 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM2 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM53 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM43 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM1 * stride));
+    dhorner(q, x13, ldd<N>(rec + PD_AM23 * stride));
+    amp = 1. + (x13 * x13) * q;
   } else if (xv < ldd<N>(rec + PD_FPEAKA * stride).v) {
     amp = ldd<N>(rec + PD_DL0 * stride) + x * ldd<N>(rec + PD_DL1 * stride) +
           x * x * (ldd<N>(rec + PD_DL2 * stride) + x * ldd<N>(rec + PD_DL3 * stride) + x * x * ldd<N>(rec + PD_DL4 * stride));

@@ -225,8 +225,9 @@ def test_full_size_properties(gpu):
 
 def test_ragged_grids_match_uniform(gpu):
     """Per-event grid lengths in one persistent launch give, for every event,
-    the Fisher and SNR of a uniform launch at that event's length (same code
-    path per event: bitwise up to the reduction, <= 1e-15 normalised)."""
+    the Fisher and SNR of a uniform launch at that event's length (the same
+    arithmetic per event, in separately compiled kernels whose multiply-adds
+    may contract differently: equal to rounding, <= 1e-14 normalised)."""
     wf, _ = _models('TF2', is_tidal=True, which_ISCO='Kerr')
     ev = bns(24, seed=31)
     rng = np.random.default_rng(0)
@@ -239,8 +240,8 @@ def test_ragged_grids_match_uniform(gpu):
         sub = {k: v[sel] for k, v in ev.items()}
         Fu = net.FisherMatr(dict(sub), res=int(r))
         su = net.SNR(dict(sub), res=int(r))
-        assert norm_err(F[:, :, sel], Fu) <= 1e-15
-        assert np.max(np.abs(snr[sel] - su) / su) <= 1e-15
+        assert norm_err(F[:, :, sel], Fu) <= 1e-14
+        assert np.max(np.abs(snr[sel] - su) / su) <= 1e-14
 
 
 C3_LENS = np.array([500, 511, 512, 513, 575, 576, 577, 1000, 2048, 4095, 4096, 4097, 7777, 9999, 10000, 10000],
@@ -325,7 +326,7 @@ def test_ragged_chunk_edges(gpu):
         sel = np.where(res == r)[0][:40]
         sub = {k: v[sel] for k, v in ev.items()}
         Fu = net.FisherMatr(dict(sub), res=int(r))
-        assert norm_err(F[:, :, sel], Fu) <= 1e-15, r
+        assert norm_err(F[:, :, sel], Fu) <= 1e-14, r  # separately compiled kernels: equal to rounding
 
 
 # ------------------------------------------------------------ post-Fisher
