@@ -362,6 +362,18 @@ GWF_HD D<2> tau_star_ev(const double* ev, const D<2>& v, const D<2>& lv) {
   return K * p / (v4 * v4);
 }
 
+// The same, value only (the SNR path)
+GWF_HD double tau_star_ev_val(const double* ev, double v, double lv) {
+  double q = ev[EV_TC7];
+  q = fma(v, q, fma(TAU_C6L, lv, ev[EV_TC6]));
+  q = fma(v, q, ev[EV_TC5]);
+  q = fma(v, q, ev[EV_TC4]);
+  q = fma(v, q, TAU_C3);
+  q = fma(v, q, ev[EV_TC2]);
+  const double v2 = v * v, v4 = v2 * v2;
+  return ev[EV_TK] * fma(v2, q, 1.0) / (v4 * v4);
+}
+
 // The event constants of the record (after its waveform coefficients, whose
 // fcut it reads): inclination factors, 1/dL, tcoal, sky angles, ln(M G) and
 // its (Mc, eta) tangents, and the projection constants of every site.
@@ -574,11 +586,14 @@ GWF_HD void wf_point(const gwf_waveform& w, const double* rec, double f, bool ne
   }
 }
 
-// Value-only amplitude without 1/dL and, when need_tau, tau_star (SNR path).
+// Value-only amplitude without 1/dL and, when need_tau, tau_star (SNR path;
+// lnf = ln f from the grid walk, so ln v = (ln(M G) + ln f + ln pi) / 3).
 // IMRPhenomD shares x^(1/3) = (M G f)^(1/3) between them: tau's
 // v = (pi M G f)^(1/3) = pi^(1/3) x^(1/3), as in wf_point.
 template <int MODEL>
-GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f, bool need_tau, double* tau) {
+GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f, double lnf, bool need_tau,
+                             double* tau) {
+  const double* ev = rec + REC_EV;
   if (is_pd(MODEL)) {
     constexpr bool NRT = MODEL == GWF_MODEL_IMRPHENOMD_NRTIDALV2;
     const double MG = rec[PD_MG * SLOT];
@@ -586,13 +601,18 @@ GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f,
     const double x13 = cbrt(x);
     if (need_tau) {
       const double v = x13 * cbrt(PI);
-      *tau = tau_star_v(MG, rec[PD_ETA * SLOT], v, log(v));
+      *tau = tau_star_ev_val(ev, v, (ev[EV_LMG] + lnf + log(PI)) * (1.0 / 3.0));
     } else {
       *tau = 0.0;
     }
     return pd_amp_val(rec, SLOT, x, x13, w.apply_fcut != 0 || NRT, NRT);
   }
-  *tau = need_tau ? tau_star(rec[TF_MG * SLOT], rec[TF_ETA * SLOT], f) : 0.0;
+  if (need_tau) {
+    const double v = cbrt(PI * rec[TF_MG * SLOT] * f);
+    *tau = tau_star_ev_val(ev, v, (ev[EV_LMG] + lnf + log(PI)) * (1.0 / 3.0));
+  } else {
+    *tau = 0.0;
+  }
   const double f13 = cbrt(f);
   return rec[TF_AMP * SLOT] * (1.0 / (f * sqrt(f13)));
 }
@@ -1009,7 +1029,7 @@ GWF_HD void snr_event(const gwf_waveform& w, const NetK& net, const double* rec,
     psd_prefetch(net.psd[0], f, lnf, pre0);
     if (net.n_psd > 1) psd_prefetch(net.psd[1], f, lnf, pre1);
     double tau;
-    const double Av = wf_amp_tau_val<MODEL>(w, rec, f, earth, &tau) * idL;
+    const double Av = wf_amp_tau_val<MODEL>(w, rec, f, lnf, earth, &tau) * idL;
     double cdel, sdel;
     earth_delta(earth, tau, &cdel, &sdel);
     int cur_site = -1, cur_psd = -1;

@@ -3,14 +3,15 @@
 //     (value + tangents w.r.t. Mc, eta, chi1z, chi2z[, LambdaTilde,
 //     deltaLambda]) once and stores it in an event record, and
 //   * a per-frequency evaluation that reads the record and returns the
-//     amplitude A(f) (without the 1/dL factor), the GW phase Phi(f) and the
-//     3.5PN time to coalescence tau(f), each as a dual number.
+//     amplitude A(f) (without the 1/dL factor) and the GW phase Phi(f), each
+//     as a dual number (the 3.5PN time to coalescence tau(f), the same for
+//     both models, waveforms.py:878-901 and 1299-1322, is evaluated from
+//     per-event coefficients of the event record: gwf_paths.h).
 // Formulas follow gwfast/waveforms.py:
-//   TaylorF2_RestrictedPN  Phi 743-862, Ampl 864-876, tau_star 878-901,
-//                          fcut 903-953;
+//   TaylorF2_RestrictedPN  Phi 743-862, Ampl 864-876, fcut 903-953;
 //   IMRPhenomD             Phi 992-1153, Ampl 1155-1254, _finalspin
-//                          1256-1276, _radiatednrg 1278-1297, tau_star
-//                          1299-1322, fcut 1324-1333;
+//                          1256-1276, _radiatednrg 1278-1297, fcut
+//                          1324-1333;
 //   tidal conversions      gwfast/gwfastUtils.py:398-444.
 #pragma once
 #include "gwf_dual.h"
@@ -747,15 +748,16 @@ GWF_HD D<N> pd_amp_at(const double* rec, int stride, const X& x, const X& x13, b
 // x13 = cbrt(x), computed once by the caller (the SNR path reuses it for tau)
 GWF_HD double pd_amp_val(const double* rec, int stride, double x, double x13, bool apply_fcut, bool nrt = false) {
   double amp;
-  if (x < 0.014) {
-    const double x23 = x13 * x13;
-    const double x43 = x23 * x23;
-    const double x53 = x43 * x13;
-    const double x73 = x53 * x23;
-    const double x83 = x73 * x13;
-    amp = 1. + x23 * rec[PD_AM23 * stride] + x43 * rec[PD_AM43 * stride] + x53 * rec[PD_AM53 * stride] +
-          x73 * rec[PD_AM73 * stride] + x83 * rec[PD_AM83 * stride] +
-          x * (rec[PD_AM1 * stride] + x * rec[PD_AM2 * stride] + x * x * rec[PD_AM3 * stride]);
+  if (x < 0.014) {  // Horner in x^(1/3), as pd_amp_at
+    double q = rec[PD_AM3 * stride];
+    q = fma(x13, q, rec[PD_AM83 * stride]);
+    q = fma(x13, q, rec[PD_AM73 * stride]);
+    q = fma(x13, q, rec[PD_AM2 * stride]);
+    q = fma(x13, q, rec[PD_AM53 * stride]);
+    q = fma(x13, q, rec[PD_AM43 * stride]);
+    q = fma(x13, q, rec[PD_AM1 * stride]);
+    q = fma(x13, q, rec[PD_AM23 * stride]);
+    amp = fma(x13 * x13, q, 1.0);
   } else if (x < rec[PD_FPEAKA * stride]) {
     amp = rec[PD_DL0 * stride] + x * rec[PD_DL1 * stride] +
           x * x * (rec[PD_DL2 * stride] + x * rec[PD_DL3 * stride] + x * x * rec[PD_DL4 * stride]);
@@ -974,33 +976,6 @@ GWF_HD D<N> tf2_phi_at(const double* rec, int stride, const V& v, const V& lv) {
                    (ldd<N>(rec + TF_E6 * stride) + ldd<N>(rec + TF_E6L * stride) * lv) * v6);
   }
   return ph;
-}
-
-// ------------------------------------------------------- tau_star (both)
-// 3.5PN time to coalescence (waveforms.py:878-901 = 1299-1322); MG = Mtot*G
-// and eta as duals over (Mc, eta).
-template <class T>
-GWF_HD T tau_star_v(const T& MG, const T& eta, const T& v, const T& lv) {
-  const T eta2 = eta * eta;
-  const T v2 = v * v;
-  const T v4 = v2 * v2;
-  const T v8 = v4 * v4;
-  const T OverallFac = (5. / 256) * MG / (eta * v8);
-  const T t05 = 1. + (743. / 252. + 11. / 3. * eta) * v2 - 32. / 5. * PI * (v2 * v) +
-                (3058673. / 508032. + 5429. / 504. * eta + 617. / 72. * eta2) * v4 -
-                (7729. / 252. - 13. / 3. * eta) * PI * (v4 * v);
-  const T t6 = (-10052469856691. / 23471078400. + 128. / 3. * PI * PI + 6848. / 105. * EULER_GAMMA +
-                (3147553127. / 3048192. - 451. / 12. * PI * PI) * eta - 15211. / 1728. * eta2 + 25565. / 1296. * eta2 * eta +
-                3424. / 105. * (log(16.) + 2.0 * lv)) *
-               (v4 * v2);
-  const T t7 = (-15419335. / 127008. - 75703. / 756. * eta + 14809. / 378. * eta2) * PI * (v4 * v2 * v);
-  return OverallFac * (t05 + t6 + t7);
-}
-
-template <class T>
-GWF_HD T tau_star(const T& MG, const T& eta, double f) {
-  const T v = dcbrt(PI * MG * f);
-  return tau_star_v(MG, eta, v, dlog(v));
 }
 
 // --------------------------------------------------- IMRPhenomD_NRTidalv2
