@@ -1005,6 +1005,111 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
   }
 }
 
+// Consumer for the 11-parameter models' network sum with the "pure" rows
+// factored out.  Four of the 11 rows are the arm's response u = Ap + i Ac
+// times a complex factor c_p that does not depend on the arm: dL (c = -1/dL),
+// Phicoal (c = -i) and the two spins (c = A'/A - i Phi', the amplitude and
+// phase tangents).  With T_j = sum_arms u conj(d_j) over the 7 other rows j
+// and W = sum_arms |u|^2 at a point, their Fisher entries are Re(c_p T_j)
+// and Re(c_p conj(c_q)) W: per arm that leaves the 28-entry Gram of the 7
+// rows, T and W (86 FMAs instead of 132), and per point the 38 entries
+// involving a pure row.  Rows scaled by sqrt(weight) make T and W weighted.
+template <int P>
+GWF_HD constexpr int pk(int a, int b) {  // packed upper-triangle index, a <= b
+  return a * P - a * (a - 1) / 2 + (b - a);
+}
+
+template <int MODEL, int NT>
+GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int res, int lane, const double* buf,
+                            double* G) {
+  constexpr int P = 11;
+  typedef PcLayout<NT> L;
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
+  __asm__ volatile("" ::: "memory");
+#endif
+  if (base + lane >= res) return;
+  const double* ev = rec + REC_EV;
+  const double f = buf[L::F * 64 + lane];
+  WfPoint<MODEL, NT> wp;
+  auto ld = [&](int slot) { return buf[slot * 64 + lane]; };
+  // the arm rows need the (Mc, eta) tangents only; the spins' are read after
+  // the arms, for their pure-row factors
+  wp.A.v = ld(L::AV);
+  wp.Phi.v = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) {
+    wp.A.d[k] = k < 2 ? ld(L::AD + k) : 0.0;
+    wp.Phi.d[k] = k < 2 ? ld(L::PD + k) : 0.0;
+  }
+  const double tc = ev[EV_TC];
+  int cur_site = -1, cur_psd = -1;
+  double sw = 0.0;
+  Geo g;
+  auto psd_sw = [&](int psd) {
+    return psd == 0 ? ld(L::SW0)
+                    : (psd == 1 ? ld(L::SW1) : sqrt(ld(L::WT) / psd_at(net.psd[psd], f, ld(L::LNF))));
+  };
+  constexpr int NP[7] = {0, 1, 3, 4, 5, 6, 7};  // the rows that are not pure
+  double Tr[7], Ti[7], W = 0.0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) Tr[j] = Ti[j] = 0.0;
+  for (int k = 0; k < net.n_sum_arms; ++k) {
+    const ArmK& arm = net.sum_arm[k];
+    const int s = arm.site;
+    const SiteK& S = net.site[s];
+    const double* sc = rec + REC_SITE + s * SC_COUNT;
+    if (s != cur_site) {
+      if (S.psd != cur_psd) {
+        sw = psd_sw(S.psd);
+        cur_psd = S.psd;
+      }
+      site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
+      cur_site = s;
+    }
+    double re[P], im[P], Ap, Ac;
+    wp.tau.d[0] = ld(L::TD);
+    wp.tau.d[1] = ld(L::TD + 1);
+    arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+#pragma unroll
+      for (int b = a; b < 7; ++b) {
+        double& x = G[pk<P>(NP[a], NP[b])];
+        x = fma(im[NP[a]], im[NP[b]], fma(re[NP[a]], re[NP[b]], x));
+      }
+      Tr[a] = fma(Ac, im[NP[a]], fma(Ap, re[NP[a]], Tr[a]));
+      Ti[a] = fma(-Ap, im[NP[a]], fma(Ac, re[NP[a]], Ti[a]));
+    }
+    W = fma(Ac, Ac, fma(Ap, Ap, W));
+  }
+  // c_p = cr + i ci: dL (-1/dL, 0), Phicoal (0, -1), chi1z, chi2z
+  // (A'/A, -Phi'); a zero amplitude (past a cut) has u = 0 and c = 0 is used
+  const double idL0 = ev[EV_IDL];
+  const double ia = wp.A.v != 0.0 ? 1.0 / wp.A.v : 0.0;
+  const double a9 = ld(L::AD + 2) * ia, b9 = -ld(L::PD + 2);
+  const double a10 = ld(L::AD + 3) * ia, b10 = -ld(L::PD + 3);
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const int r = NP[j];
+    double& g2 = G[r < 2 ? pk<P>(r, 2) : pk<P>(2, r)];
+    g2 = fma(-idL0, Tr[j], g2);
+    G[pk<P>(r, 8)] += Ti[j];
+    double& g9 = G[pk<P>(r, 9)];
+    g9 = fma(a9, Tr[j], fma(-b9, Ti[j], g9));
+    double& g10 = G[pk<P>(r, 10)];
+    g10 = fma(a10, Tr[j], fma(-b10, Ti[j], g10));
+  }
+  G[pk<P>(2, 2)] = fma(idL0 * idL0, W, G[pk<P>(2, 2)]);
+  G[pk<P>(2, 9)] = fma(-idL0 * a9, W, G[pk<P>(2, 9)]);
+  G[pk<P>(2, 10)] = fma(-idL0 * a10, W, G[pk<P>(2, 10)]);
+  G[pk<P>(8, 8)] += W;
+  G[pk<P>(8, 9)] = fma(-b9, W, G[pk<P>(8, 9)]);
+  G[pk<P>(8, 10)] = fma(-b10, W, G[pk<P>(8, 10)]);
+  G[pk<P>(9, 9)] = fma(fma(a9, a9, b9 * b9), W, G[pk<P>(9, 9)]);
+  G[pk<P>(9, 10)] = fma(fma(a9, a10, b9 * b10), W, G[pk<P>(9, 10)]);
+  G[pk<P>(10, 10)] = fma(fma(a10, a10, b10 * b10), W, G[pk<P>(10, 10)]);
+}
+
 // --------------------------------------------------------------- SNR body
 // Per-arm trapz((Ap^2 + Ac^2)/S, f) of every arm of the network in one grid
 // walk: the amplitude, tau(f) and the PSD segments are evaluated once per

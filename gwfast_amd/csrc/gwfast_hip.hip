@@ -452,8 +452,13 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
                         for (int q = 0; q < NG; ++q) G[q] = 0.0;
                         if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
                       }
-                      pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
-                                                       G, mask);
+#ifndef GWF_NO_PURE_ROWS
+                      if constexpr (P == 11 && !MASKED)
+                        pc_consume_pure<MODEL, NT>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk, G);
+                      else
+#endif
+                        pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane,
+                                                         chunk, G, mask);
                     },
                     [&](int e) { store_packed<P>(G, F, n_ev, e, lane); });
   }
