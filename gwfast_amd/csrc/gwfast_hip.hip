@@ -299,6 +299,150 @@ __device__ __forceinline__ void pc_consume_mx(const NetK& net, const double* rec
   }
 }
 
+// Consumer for 12-14 parameters with the pure rows factored out (as
+// pc_consume_pure: dL, Phicoal and the waveform-only rows 9..P-1 are the
+// arm response u times c_p = A'/A - i Phi', T_j = sum_arms u conj(d_j) over
+// the 7 other rows and W = sum_arms |u|^2).  On the VALU: per arm the
+// 28-entry Gram of the 7 rows, T and W; per point the pure x pure entries
+// Re(c_p conj(c_q)) W.  On the matrix cores: the pure x other entries
+// sum_points (c_pr Tr_j - c_pi Ti_j) -- per chunk each lane writes its
+// point's [c_r | Tr] and then [-c_i | Ti] into the wave's LDS tile (pure
+// factors in columns 0..NPU-1, T in 8..14) and 16 v_mfma_f64_16x16x4f64 per
+// part contract over the 64 points (A[p][k] = tile[4t + k][p],
+// B[k][j] = tile[4t + k][8 + j]; D[p][j] in lane j + 16 (p % 4), item p / 4).
+// 32 MFMAs per chunk instead of 32 per arm.  G holds the 28 entries of the
+// 7 rows (packed, pk<7>) and then the pure x pure ones (pk<NPU>).
+template <int MODEL, int NT, int P>
+__device__ __forceinline__ void pc_consume_pure_mx(const NetK& net, const double* rec, int base, int res, int lane,
+                                                    const double* buf, double* X, double* G, v4d& acc) {
+  constexpr int NPU = P - 7;
+  static_assert(NPU >= 2 && NPU <= 8, "pure rows: dL, Phicoal and up to six waveform-only rows");
+  typedef PcLayout<NT> L;
+  __asm__ volatile("" ::: "memory");
+  const bool live = base + lane < res;
+  const double* ev = rec + REC_EV;
+  constexpr int NP[7] = {0, 1, 3, 4, 5, 6, 7};
+  double Tr[7], Ti[7], cr[NPU], ci[NPU];
+#pragma unroll
+  for (int j = 0; j < 7; ++j) Tr[j] = Ti[j] = 0.0;
+#pragma unroll
+  for (int p = 0; p < NPU; ++p) cr[p] = ci[p] = 0.0;
+  if (live) {
+    auto ld = [&](int slot) { return buf[slot * 64 + lane]; };
+    const double f = ld(L::F);
+    WfPoint<MODEL, NT> wp;
+    wp.A.v = ld(L::AV);
+    wp.Phi.v = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) {  // the 7 rows need the (Mc, eta) tangents only
+      wp.A.d[k] = k < 2 ? ld(L::AD + k) : 0.0;
+      wp.Phi.d[k] = k < 2 ? ld(L::PD + k) : 0.0;
+    }
+    const double tc = ev[EV_TC];
+    int cur_site = -1, cur_psd = -1;
+    double sw = 0.0, W = 0.0;
+    Geo g;
+    for (int k = 0; k < net.n_sum_arms; ++k) {
+      const ArmK& arm = net.sum_arm[k];
+      const int s = arm.site;
+      const SiteK& S = net.site[s];
+      const double* sc = rec + REC_SITE + s * SC_COUNT;
+      if (s != cur_site) {
+        if (S.psd != cur_psd) {
+          sw = S.psd == 0 ? ld(L::SW0)
+                          : (S.psd == 1 ? ld(L::SW1) : sqrt(ld(L::WT) / psd_at(net.psd[S.psd], f, ld(L::LNF))));
+          cur_psd = S.psd;
+        }
+        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
+        cur_site = s;
+      }
+      double re[P], im[P], Ap, Ac;
+      wp.tau.d[0] = ld(L::TD);
+      wp.tau.d[1] = ld(L::TD + 1);
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt);
+#pragma unroll
+      for (int a = 0; a < 7; ++a) {
+#pragma unroll
+        for (int b = a; b < 7; ++b) {
+          double& x = G[pk<7>(a, b)];
+          x = fma(im[NP[a]], im[NP[b]], fma(re[NP[a]], re[NP[b]], x));
+        }
+        Tr[a] = fma(Ac, im[NP[a]], fma(Ap, re[NP[a]], Tr[a]));
+        Ti[a] = fma(-Ap, im[NP[a]], fma(Ac, re[NP[a]], Ti[a]));
+      }
+      W = fma(Ac, Ac, fma(Ap, Ap, W));
+    }
+    // c_p: dL (-1/dL, 0), Phicoal (0, -1), rows 9.. (A'/A, -Phi') with the
+    // tangent of row r in slot r - 7; a zero amplitude (past a cut) has u = 0
+    const double ia = wp.A.v != 0.0 ? 1.0 / wp.A.v : 0.0;
+    cr[0] = -ev[EV_IDL];
+    ci[1] = -1.0;
+#pragma unroll
+    for (int p = 2; p < NPU; ++p) {
+      cr[p] = ld(L::AD + p) * ia;
+      ci[p] = -ld(L::PD + p);
+    }
+#pragma unroll
+    for (int p = 0; p < NPU; ++p)
+#pragma unroll
+      for (int q = p; q < NPU; ++q) {
+        double& x = G[28 + pk<NPU>(p, q)];
+        x = fma(fma(cr[p], cr[q], ci[p] * ci[q]), W, x);
+      }
+  }
+  const int rd = (lane >> 4) * PQ_XS;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+#pragma unroll
+    for (int p = 0; p < NPU; ++p) X[lane * PQ_XS + p] = part ? -ci[p] : cr[p];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) X[lane * PQ_XS + 8 + j] = part ? Ti[j] : Tr[j];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const double a = X[4 * t * PQ_XS + rd + (lane & 15)];
+      const double b = X[4 * t * PQ_XS + rd + 8 + (lane & 7)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Stores the Fisher of pc_consume_pure_mx: the VALU entries wave-reduced,
+// the pure x other entries from the matrix-core accumulator.
+template <int P>
+__device__ __forceinline__ void store_pure_mx(const double* G, const v4d& acc, double* __restrict__ F, int64_t n_ev,
+                                              int64_t e, int lane) {
+  constexpr int NPU = P - 7;
+  constexpr int NP[7] = {0, 1, 3, 4, 5, 6, 7};
+  auto PU = [](int p) { return p == 0 ? 2 : 7 + p; };  // 2, 8, 9, ..., P-1
+  auto put = [&](int a, int b, double v) {
+    F[(int64_t)(a * P + b) * n_ev + e] = v;
+    if (a != b) F[(int64_t)(b * P + a) * n_ev + e] = v;
+  };
+  int k = 0;
+#pragma unroll
+  for (int a = 0; a < 7; ++a)
+#pragma unroll
+    for (int b = a; b < 7; ++b, ++k) {
+      const double s = wave_sum(G[pk<7>(a, b)]);
+      if (lane == (k & 63)) put(NP[a], NP[b], s);
+    }
+#pragma unroll
+  for (int p = 0; p < NPU; ++p)
+#pragma unroll
+    for (int q = p; q < NPU; ++q, ++k) {
+      const double s = wave_sum(G[28 + pk<NPU>(p, q)]);
+      if (lane == (k & 63)) put(PU(p), PU(q), s);
+    }
+  const int j = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int p = (lane >> 4) + 4 * r;
+    if (p < NPU && j < 7) put(PU(p), NP[j], acc[r]);
+  }
+}
+
 // Stores the event's Fisher from the NACC matrix-core accumulators (element
 // (i, j) in lane j + 16 (i % 4), item i / 4): the upper-triangle element
 // (i <= j) goes to both F[i][j] and F[j][i].
@@ -361,6 +505,11 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   typedef PcLayout<NT> L;
   constexpr bool MASKED = MODE == PQ_MASKED, ARMS = MODE == PQ_ARMS;
   constexpr bool MX = P >= GWF_PQ_MFMA_MIN_P || ARMS;  // Gram on the matrix cores (pc_consume_mx)
+#ifdef GWF_NO_PURE_MX
+  constexpr bool PURE_MX = false;
+#else
+  constexpr bool PURE_MX = true;  // 12-14 parameters: pure rows factored out (pc_consume_pure_mx)
+#endif
   // ring depth: PQ_NBUF buffers, fewer where they would not fit the CU's LDS
   // beside the MFMA tiles
   constexpr int NB = pq_ring_depth(L::NV, MX);
@@ -416,6 +565,23 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       if (!RAGGED) e += stride;
     }
+  } else if constexpr (MX && MODE == PQ_SUM && PURE_MX) {  // pure rows factored out, see pc_consume_pure_mx
+    double* X = xt[slot];
+    for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;
+    constexpr int NPU = P - 7, NG = 28 + NPU * (NPU + 1) / 2;
+    double G[NG];
+    v4d acc;
+    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
+                    [&](int e, int t, int res, const double* chunk) {
+                      if (t == 0) {
+#pragma unroll
+                        for (int q = 0; q < NG; ++q) G[q] = 0.0;
+                        acc = v4d{0.0, 0.0, 0.0, 0.0};
+                      }
+                      pc_consume_pure_mx<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
+                                                       X, G, acc);
+                    },
+                    [&](int e) { store_pure_mx<P>(G, acc, F, n_ev, e, lane); });
   } else if constexpr (MX) {  // Gram on the matrix cores
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
