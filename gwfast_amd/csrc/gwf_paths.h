@@ -1019,9 +1019,9 @@ GWF_HD constexpr int pk(int a, int b) {  // packed upper-triangle index, a <= b
   return a * P - a * (a - 1) / 2 + (b - a);
 }
 
-template <int MODEL, int NT>
+template <int MODEL, int NT, bool MASKED = false>
 GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int res, int lane, const double* buf,
-                            double* G) {
+                            double* G, uint32_t mask = 0u) {
   constexpr int P = 11;
   typedef PcLayout<NT> L;
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
@@ -1053,9 +1053,8 @@ GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int re
   double Tr[7], Ti[7], W = 0.0;
 #pragma unroll
   for (int j = 0; j < 7; ++j) Tr[j] = Ti[j] = 0.0;
-  for (int k = 0; k < net.n_sum_arms; ++k) {
-    const ArmK& arm = net.sum_arm[k];
-    const int s = arm.site;
+  // one arm of site s: the 7 rows' Gram, T and W
+  auto do_arm = [&](const ArmK& arm, int s) {
     const SiteK& S = net.site[s];
     const double* sc = rec + REC_SITE + s * SC_COUNT;
     if (s != cur_site) {
@@ -1081,6 +1080,23 @@ GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int re
       Ti[a] = fma(-Ap, im[NP[a]], fma(Ac, re[NP[a]], Ti[a]));
     }
     W = fma(Ac, Ac, fma(Ap, Ap, W));
+  };
+  if constexpr (MASKED) {  // the arms whose bit is set; a site with all set through its sum arms
+    for (int s = 0; s < net.n_sites; ++s) {
+      const SiteK& S = net.site[s];
+      const uint32_t all = (1u << S.narm) - 1u;
+      const uint32_t bits = (mask >> S.arm0) & all;
+      if (!bits) continue;
+      const bool full = bits == all;
+      const int cnt = full ? S.nsum : S.narm;
+      for (int j = 0; j < cnt; ++j) {
+        if (!full && !((bits >> j) & 1u)) continue;
+        do_arm(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s);
+      }
+    }
+  } else {
+    (void)mask;
+    for (int k = 0; k < net.n_sum_arms; ++k) do_arm(net.sum_arm[k], net.sum_arm[k].site);
   }
   // c_p = cr + i ci: dL (-1/dL, 0), Phicoal (0, -1), chi1z, chi2z
   // (A'/A, -Phi'); a zero amplitude (past a cut) has u = 0 and c = 0 is used

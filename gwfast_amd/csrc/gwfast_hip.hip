@@ -619,8 +619,9 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
                         if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
                       }
 #ifndef GWF_NO_PURE_ROWS
-                      if constexpr (P == 11 && !MASKED)
-                        pc_consume_pure<MODEL, NT>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk, G);
+                      if constexpr (P == 11)
+                        pc_consume_pure<MODEL, NT, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
+                                                           G, mask);
                       else
 #endif
                         pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane,
