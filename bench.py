@@ -327,6 +327,33 @@ def screen_leg(net, n_cat=1_000_000, n_train=1_000_000, epochs=8, res=RES, thres
                               'frac_of_bf16_dense_peak': mlp_tflops / 2500.0}}
 
 
+def catalogue_leg(net, n=100_000, res=1000, seed=21):
+    """The catalogue driver (forecast.compute_errs + to_file, the chain of
+    run/calculate_forecasts_from_catalog.py:421-630 at its res=1000): network
+    SNR, threshold 12, Fisher of the detected events, CheckFisher, CovMatr,
+    90 % sky areas and the driver's files, on 10^5 BBH events in one chunk;
+    the device chain (parameters from host arrays, results back to host
+    arrays) and the file writing (text and .npy, as the driver) timed apart."""
+    import tempfile
+    import torch
+    from gwfast_amd import catalog, forecast as FC
+    ev = catalog.draw_bbh(n, seed=seed)
+    FC.compute_errs({k: np.asarray(v)[:1000] for k, v in ev.items()}, net, res=res)  # warm-up
+    torch.cuda.synchronize()
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        r = FC.compute_errs(ev, net, res=res)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        FC.to_file(r, d, '_0_to_%d' % n)
+        t2 = time.perf_counter()
+    nd = int(len(r['idxs_det']))
+    return {'events_per_s': n / (t2 - t0), 'n_events': n, 'n_detected': nd, 'res': res,
+            'device_chain_s': t1 - t0, 'files_s': t2 - t1, 'events_per_s_device_chain': n / (t1 - t0),
+            'path': 'forecast.compute_errs (gwf_snr, gwf_fisher on the detected events, gwf_check_fisher, '
+                    'gwf_cov_invert, gwf_sigma_sky_area) + forecast.to_file'}
+
+
 def fisher_steps(wf, net, n, steps, warmup, seed=0):
     """The timed step on n events: gwf_prepare (per-event records) then
     gwf_fisher_prepared (the fused Fisher kernel), parameters resident in
@@ -389,7 +416,8 @@ def main():
     ap.add_argument('--no-c2', action='store_true', help='skip the 10^5-event (C2) leg')
     ap.add_argument('--no-plain', action='store_true')
     ap.add_argument('--no-screen', action='store_true', help='skip the learned-screen (10^6 catalogue) leg')
-    ap.add_argument('--no-extra', action='store_true', help='skip the PCIe-inclusive, per-arm and ragged (C3) legs')
+    ap.add_argument('--no-extra', action='store_true',
+                    help='skip the PCIe-inclusive, per-arm, ragged (C3) and catalogue-driver legs')
     ap.add_argument('--screen-only', action='store_true', help='run only the learned-screen leg')
     ap.add_argument('--screen-events', type=int, default=1_000_000)
     ap.add_argument('--screen-train', type=int, default=1_000_000)
@@ -468,6 +496,7 @@ def main():
         out['pcie_inclusive'] = pcie_leg(net, ev, n)
         out['per_arm_return_all'] = per_arm_leg(net, rows, n)
         out['ragged_c3'] = ragged_leg()
+        out['catalogue_1e5'] = catalogue_leg(net)
     if not args.no_screen:
         del F
         out['screen_1e6'] = screen_leg(net, n_cat=args.screen_events, n_train=args.screen_train,
