@@ -1014,6 +1014,13 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
 // and Re(c_p conj(c_q)) W: per arm that leaves the 28-entry Gram of the 7
 // rows, T and W (86 FMAs instead of 132), and per point the 38 entries
 // involving a pure row.  Rows scaled by sqrt(weight) make T and W weighted.
+// Amplitudes at or below this (a Planck taper's last points, a cut) count as
+// zero in the pure-row factors c_p = A'/A - i Phi': 1/A overflows below
+// 5.6e-309, while such a point's rows, all of the order of A and of its
+// tangents (below ~1e-270 there, against ~1e-20 in band), add nothing to
+// the Fisher matrix at double precision.
+constexpr double AMP_TINY = 1e-280;
+
 template <int P>
 GWF_HD constexpr int pk(int a, int b) {  // packed upper-triangle index, a <= b
   return a * P - a * (a - 1) / 2 + (b - a);
@@ -1101,7 +1108,7 @@ GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int re
   // c_p = cr + i ci: dL (-1/dL, 0), Phicoal (0, -1), chi1z, chi2z
   // (A'/A, -Phi'); a zero amplitude (past a cut) has u = 0 and c = 0 is used
   const double idL0 = ev[EV_IDL];
-  const double ia = wp.A.v != 0.0 ? 1.0 / wp.A.v : 0.0;
+  const double ia = fabs(wp.A.v) > AMP_TINY ? 1.0 / wp.A.v : 0.0;
   const double a9 = ld(L::AD + 2) * ia, b9 = -ld(L::PD + 2);
   const double a10 = ld(L::AD + 3) * ia, b10 = -ld(L::PD + 3);
 #pragma unroll

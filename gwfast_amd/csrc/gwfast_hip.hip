@@ -374,7 +374,7 @@ __device__ __forceinline__ void pc_consume_pure_mx(const NetK& net, const double
     }
     // c_p: dL (-1/dL, 0), Phicoal (0, -1), rows 9.. (A'/A, -Phi') with the
     // tangent of row r in slot r - 7; a zero amplitude (past a cut) has u = 0
-    const double ia = wp.A.v != 0.0 ? 1.0 / wp.A.v : 0.0;
+    const double ia = fabs(wp.A.v) > AMP_TINY ? 1.0 / wp.A.v : 0.0;
     cr[0] = -ev[EV_IDL];
     ci[1] = -1.0;
 #pragma unroll
