@@ -177,10 +177,13 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PQ_XS = 17;
 
 // Deepest ring (<= PQ_NBUF buffers) whose chunks (nv doubles per point), the
-// headers and, with the matrix-core Gram, the transposition tiles fit the LDS.
+// headers, the producers' record copies and, with the matrix-core Gram, the
+// transposition tiles fit the LDS.
 constexpr int pq_ring_depth(int nv, bool mx) {
   int nb = PQ_NBUF;
-  while (nb > 2 && PC_EPB * (nb * (nv * 64 * 8 + 16) + 8 + (mx ? 64 * PQ_XS * 8 : 0)) > PQ_LDS_BYTES) --nb;
+  while (nb > 2 &&
+         PC_EPB * (nb * (nv * 64 * 8 + 16) + 8 + (mx ? 64 * PQ_XS * 8 : 0) + REC_SITE * 8) > PQ_LDS_BYTES)
+    --nb;
   return nb;
 }
 #ifndef GWF_PQ_NACC
@@ -517,6 +520,10 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   __shared__ int head[PC_EPB][NB][4];  // event, chunk, grid length of the item in each buffer
   __shared__ int cnt[PC_EPB][2];            // items produced, items consumed
   __shared__ double xt[PC_EPB][MX ? 64 * PQ_XS : 1];  // consumers' transposition tiles
+  // each producer's copy of its event's waveform slots and event constants
+  // (3.9 KB): read per chunk from LDS instead of through the scalar cache,
+  // which the four pairs' records (5.6 KB each, both waves reading) thrash
+  __shared__ double recl[PC_EPB][REC_SITE];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int slot = wave % PC_EPB;
   if (threadIdx.x < 2 * PC_EPB) cnt[threadIdx.x >> 1][threadIdx.x & 1] = 0;
@@ -544,7 +551,10 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       int res = net.res;
       if (RAGGED) res = min(max(__builtin_amdgcn_readfirstlane(res_ev[e]), 2), 1 << 30);  // chunk counts stay in int
       const int nch = (res + 63) / 64;
-      const double* r = rec + e * (int64_t)REC_DOUBLES;
+      const double* rg = rec + e * (int64_t)REC_DOUBLES;
+      double* rl = recl[slot];  // the previous event's chunks are all in the ring: free to overwrite
+      for (int q = lane; q < REC_SITE; q += 64) rl[q] = rg[q];
+      const double* r = rl;
       double a, b;
       grid_ends(LIN, net.fmin, fcut_eff(r[REC_EV + EV_FCUT], net.fmax), &a, &b);
       GridWalk<LIN> gw;
