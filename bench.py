@@ -31,10 +31,10 @@ SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
 GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_fisher_pq launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
-# (43,160.6 KiB, x2.0: a 256 MiB device copy reads 131,083.5 KiB by that counter)
-# and WRITE_SIZE (21,719.8 KiB, x1.0) runs of this script; summary in
+# (26,205.1 KiB, x2.0: a 256 MiB device copy reads 131,083.5 KiB by that counter)
+# and WRITE_SIZE (9,457.9 KiB, x1.0) runs of this script; summary in
 # profiles/k_fisher_imrphenomd_net_10k_2000f.md
-K_FISHER_TRAFFIC_10K = (43160.6 * 2.0 + 21719.8) * 1024
+K_FISHER_TRAFFIC_10K = (26205.1 * 2.0 + 9457.9) * 1024
 
 
 def build_net():

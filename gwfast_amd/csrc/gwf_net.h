@@ -118,6 +118,9 @@ inline int build_net(const gwf_detector* dets, int n_det, int res, NetK& net, st
   }
   net.n_arms = na;
   net.n_psd = np;
+  bool any_fixed = false;
+  for (int d = 0; d < n_det; ++d) any_fixed = any_fixed || net.site[d].motion == MOTION_FIXED;
+  net.mixed_motion = net.any_earth && any_fixed;
   return 0;
 }
 

@@ -78,6 +78,7 @@ struct NetK {
   int n_sites, n_arms, res, any_earth;
   int n_sum_arms, n_psd;  // n_psd: distinct PSD tables
   int lin;                // Fisher grids: linspace instead of geomspace
+  int mixed_motion;       // rotating-Earth and fixed-Earth sites together
   unsigned spin_limit;    // polls per wait of the producer / consumer Fisher kernel
   double fmin, fmax;
   SiteK site[GWF_MAX_DETECTORS];
@@ -631,7 +632,7 @@ GWF_HD double wf_amp_tau_val(const gwf_waveform& w, const double* rec, double f,
 template <int MODEL, int NT, int P>
 GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const double* sc, const Geo& g, const ArmK& arm,
                        int motion, double f, double re[P], double im[P], double* Ap_out, double* Ac_out,
-                       double q = 1.0) {
+                       double q = 1.0, double* v_out = nullptr) {
   const double idL0 = ev[EV_IDL], idL = idL0 * q;
   const double cp = ev[EV_CP], cc = ev[EV_CC];
   const double P1 = sc[SC_P1], P2 = sc[SC_P2];
@@ -670,6 +671,14 @@ GWF_HD void arm_derivs(const WfPoint<MODEL, NT>& wp, const double* ev, const dou
   *Ac_out = Ac;
   const double twopif = 2. * PI * f;
   const double tdt = twopif * g.dt_t;             // 2 pi f d(Delta t)/dt
+  if (v_out) {
+    // v = -2 pi tfac dU/dalpha + i tdt U (U = Ap + i Ac): the Mc, eta and
+    // tcoal rows are U c_j + v tn_j (tn_j: their d t_noloc / dX, zero for
+    // every row when the site does not move)
+    const double k = -tw * tfac;
+    v_out[0] = motion == MOTION_NONE ? 0.0 : fma(k, Xp, -tdt * Ac);
+    v_out[1] = motion == MOTION_NONE ? 0.0 : fma(k, Xc, tdt * Ap);
+  }
 
 #define GWF_SET(X, dAp, dAc, dPsi)        \
   {                                       \
@@ -1131,6 +1140,160 @@ GWF_HD void pc_consume_pure(const NetK& net, const double* rec, int base, int re
   G[pk<P>(9, 9)] = fma(fma(a9, a9, b9 * b9), W, G[pk<P>(9, 9)]);
   G[pk<P>(9, 10)] = fma(fma(a9, a10, b9 * b10), W, G[pk<P>(9, 10)]);
   G[pk<P>(10, 10)] = fma(fma(a10, a10, b10 * b10), W, G[pk<P>(10, 10)]);
+}
+
+// pc_consume_pure with the Mc, eta and tcoal rows factored as well.  Their
+// Earth-rotation terms are one arm vector v times an arm-independent t_noloc
+// tangent s_j, so every row outside theta, phi, iota, psi is
+// d_j = u c_j + v s_j:  Mc, eta: c = A'/A - i Phi', s = -tau'/day (rotating
+// Earth) or 0; tcoal: c = 2 pi i f, s = 1/day; dL, Phicoal, chi1z, chi2z:
+// s = 0.  Per arm: the Gram of the 4 sky rows, Tu_k = sum d_k conj(u),
+// Tv_k = Re sum d_k conj(v), sum |u|^2, sum |v|^2, sum u conj(v) (52 FMAs;
+// 86 with only the pure rows factored); per point the other entries:
+// F_kj = Re(Tu_k conj c_j) + s_j Tv_k,
+// F_jl = Re(c_j conj c_l) W_uu + s_j s_l W_vv + s_l Re(c_j W_uv) + s_j Re(c_l W_uv).
+// s_j must not vary between sites: not for a network that mixes rotating-
+// and fixed-Earth sites (sites without motion have v = 0).
+template <int MODEL, int NT, bool MASKED = false>
+GWF_HD void pc_consume_semi(const NetK& net, const double* rec, int base, int res, int lane, const double* buf,
+                            double* G, uint32_t mask = 0u) {
+  constexpr int P = 11;
+  typedef PcLayout<NT> L;
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
+  __asm__ volatile("" ::: "memory");
+#endif
+  if (base + lane >= res) return;
+  const double* ev = rec + REC_EV;
+  const double f = buf[L::F * 64 + lane];
+  WfPoint<MODEL, NT> wp;
+  auto ld = [&](int slot) { return buf[slot * 64 + lane]; };
+  // the sky rows need no waveform or tau tangent
+  wp.A.v = ld(L::AV);
+  wp.Phi.v = wp.tau.v = wp.tau.d[0] = wp.tau.d[1] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NT; ++k) wp.A.d[k] = wp.Phi.d[k] = 0.0;
+  const double tc = ev[EV_TC];
+  int cur_site = -1, cur_psd = -1;
+  double sw = 0.0;
+  Geo g;
+  auto psd_sw = [&](int psd) {
+    return psd == 0 ? ld(L::SW0)
+                    : (psd == 1 ? ld(L::SW1) : sqrt(ld(L::WT) / psd_at(net.psd[psd], f, ld(L::LNF))));
+  };
+  constexpr int SK[4] = {3, 4, 5, 6};  // theta, phi, iota, psi
+  double Tr[4], Ti[4], Tv[4], Wuu = 0.0, Wvv = 0.0, Wr = 0.0, Wi = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) Tr[j] = Ti[j] = Tv[j] = 0.0;
+  auto do_arm = [&](const ArmK& arm, int s) {
+    const SiteK& S = net.site[s];
+    const double* sc = rec + REC_SITE + s * SC_COUNT;
+    if (s != cur_site) {
+      if (S.psd != cur_psd) {
+        sw = psd_sw(S.psd);
+        cur_psd = S.psd;
+      }
+      site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
+      cur_site = s;
+    }
+    double re[P], im[P], Ap, Ac, v[2];
+    arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt, v);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int k = SK[a];
+#pragma unroll
+      for (int b = a; b < 4; ++b) {
+        double& x = G[pk<P>(k, SK[b])];
+        x = fma(im[k], im[SK[b]], fma(re[k], re[SK[b]], x));
+      }
+      Tr[a] = fma(Ac, im[k], fma(Ap, re[k], Tr[a]));
+      Ti[a] = fma(-Ap, im[k], fma(Ac, re[k], Ti[a]));
+      Tv[a] = fma(v[1], im[k], fma(v[0], re[k], Tv[a]));
+    }
+    Wuu = fma(Ac, Ac, fma(Ap, Ap, Wuu));
+    Wvv = fma(v[1], v[1], fma(v[0], v[0], Wvv));
+    Wr = fma(Ac, v[1], fma(Ap, v[0], Wr));   // Re(u conj v)
+    Wi = fma(-Ap, v[1], fma(Ac, v[0], Wi));  // Im(u conj v)
+  };
+  if constexpr (MASKED) {
+    for (int s = 0; s < net.n_sites; ++s) {
+      const SiteK& S = net.site[s];
+      const uint32_t all = (1u << S.narm) - 1u;
+      const uint32_t bits = (mask >> S.arm0) & all;
+      if (!bits) continue;
+      const bool full = bits == all;
+      const int cnt = full ? S.nsum : S.narm;
+      for (int j = 0; j < cnt; ++j) {
+        if (!full && !((bits >> j) & 1u)) continue;
+        do_arm(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s);
+      }
+    }
+  } else {
+    (void)mask;
+    for (int k = 0; k < net.n_sum_arms; ++k) do_arm(net.sum_arm[k], net.sum_arm[k].site);
+  }
+  // the factored rows: index, c_j = cr + i ci, s_j (HR / HI / HS: the parts
+  // that are not identically zero)
+  constexpr int FR[7] = {0, 1, 7, 2, 8, 9, 10};
+  constexpr bool HR[7] = {true, true, false, true, false, true, true};
+  constexpr bool HI[7] = {true, true, true, false, true, true, true};
+  constexpr bool HS[7] = {true, true, true, false, false, false, false};
+  const double ia = fabs(wp.A.v) > AMP_TINY ? 1.0 / wp.A.v : 0.0;
+  const bool earth = net.any_earth != 0;
+  double cr[7], ci[7], sj[7];
+  cr[0] = ld(L::AD) * ia;
+  ci[0] = -ld(L::PD);
+  sj[0] = earth ? -ld(L::TD) * INV_DAY : 0.0;
+  cr[1] = ld(L::AD + 1) * ia;
+  ci[1] = -ld(L::PD + 1);
+  sj[1] = earth ? -ld(L::TD + 1) * INV_DAY : 0.0;
+  cr[2] = 0.0;
+  ci[2] = 2. * PI * f;
+  sj[2] = INV_DAY;
+  cr[3] = -ev[EV_IDL];
+  ci[3] = 0.0;
+  cr[4] = 0.0;
+  ci[4] = -1.0;
+  cr[5] = ld(L::AD + 2) * ia;
+  ci[5] = -ld(L::PD + 2);
+  cr[6] = ld(L::AD + 3) * ia;
+  ci[6] = -ld(L::PD + 3);
+#pragma unroll
+  for (int j = 3; j < 7; ++j) sj[j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int k = SK[a];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+      const int r = FR[j];
+      double& x = G[k < r ? pk<P>(k, r) : pk<P>(r, k)];
+      double t = x;
+      if (HR[j]) t = fma(cr[j], Tr[a], t);
+      if (HI[j]) t = fma(-ci[j], Ti[a], t);
+      if (HS[j]) t = fma(sj[j], Tv[a], t);
+      x = t;
+    }
+  }
+  double z[7];  // Re(c_j W_uv)
+#pragma unroll
+  for (int j = 0; j < 7; ++j) z[j] = HR[j] ? (HI[j] ? fma(cr[j], Wr, -ci[j] * Wi) : cr[j] * Wr) : -ci[j] * Wi;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+#pragma unroll
+    for (int l = j; l < 7; ++l) {
+      const int r = FR[j], c = FR[l];
+      double& x = G[r < c ? pk<P>(r, c) : pk<P>(c, r)];
+      double t = x;
+      const bool rr = HR[j] && HR[l], ii = HI[j] && HI[l];
+      if (rr || ii) {
+        const double cc = rr ? (ii ? fma(cr[j], cr[l], ci[j] * ci[l]) : cr[j] * cr[l]) : ci[j] * ci[l];
+        t = fma(cc, Wuu, t);
+      }
+      if (HS[j] && HS[l]) t = fma(sj[j] * sj[l], Wvv, t);
+      if (HS[l]) t = fma(sj[l], z[j], t);
+      if (HS[j]) t = fma(sj[j], z[l], t);
+      x = t;
+    }
+  }
 }
 
 // --------------------------------------------------------------- SNR body

@@ -177,12 +177,12 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int PQ_XS = 17;
 
 // Deepest ring (<= PQ_NBUF buffers) whose chunks (nv doubles per point), the
-// headers, the producers' record copies and, with the matrix-core Gram, the
-// transposition tiles fit the LDS.
-constexpr int pq_ring_depth(int nv, bool mx) {
+// headers, the producers' record copies (rec_lds) and, with the matrix-core
+// Gram, the transposition tiles fit the LDS.
+constexpr int pq_ring_depth(int nv, bool mx, bool rec_lds) {
   int nb = PQ_NBUF;
-  while (nb > 2 &&
-         PC_EPB * (nb * (nv * 64 * 8 + 16) + 8 + (mx ? 64 * PQ_XS * 8 : 0) + REC_SITE * 8) > PQ_LDS_BYTES)
+  while (nb > 2 && PC_EPB * (nb * (nv * 64 * 8 + 16) + 8 + (mx ? 64 * PQ_XS * 8 : 0) + (rec_lds ? REC_SITE * 8 : 0)) >
+                       PQ_LDS_BYTES)
     --nb;
   return nb;
 }
@@ -513,9 +513,15 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
 #else
   constexpr bool PURE_MX = true;  // 12-14 parameters: pure rows factored out (pc_consume_pure_mx)
 #endif
+  // The producer reads its event's waveform slots and constants from an LDS
+  // copy, except for eccentric TaylorF2: its many phase coefficients, held in
+  // VGPRs when read from LDS (SGPRs through the scalar cache), spill (95 and
+  // 180 VGPRs at 12 and 14 parameters; 2.31 -> 2.36 and 2.44 -> 5.07 ms per
+  // 10^4 events)
+  constexpr bool REC_LDS = !(MODEL == GWF_MODEL_TAYLORF2 && (P == 12 || P == 14));
   // ring depth: PQ_NBUF buffers, fewer where they would not fit the CU's LDS
   // beside the MFMA tiles
-  constexpr int NB = pq_ring_depth(L::NV, MX);
+  constexpr int NB = pq_ring_depth(L::NV, MX, REC_LDS);
   __shared__ double buf[PC_EPB][NB][L::NV * 64];
   __shared__ int head[PC_EPB][NB][4];  // event, chunk, grid length of the item in each buffer
   __shared__ int cnt[PC_EPB][2];            // items produced, items consumed
@@ -523,7 +529,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   // each producer's copy of its event's waveform slots and event constants
   // (3.9 KB): read per chunk from LDS instead of through the scalar cache,
   // which the four pairs' records (5.6 KB each, both waves reading) thrash
-  __shared__ double recl[PC_EPB][REC_SITE];
+  __shared__ double recl[PC_EPB][REC_LDS ? REC_SITE : 1];
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const int slot = wave % PC_EPB;
   if (threadIdx.x < 2 * PC_EPB) cnt[threadIdx.x >> 1][threadIdx.x & 1] = 0;
@@ -552,9 +558,12 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       if (RAGGED) res = min(max(__builtin_amdgcn_readfirstlane(res_ev[e]), 2), 1 << 30);  // chunk counts stay in int
       const int nch = (res + 63) / 64;
       const double* rg = rec + e * (int64_t)REC_DOUBLES;
-      double* rl = recl[slot];  // the previous event's chunks are all in the ring: free to overwrite
-      for (int q = lane; q < REC_SITE; q += 64) rl[q] = rg[q];
-      const double* r = rl;
+      const double* r = rg;
+      if constexpr (REC_LDS) {
+        double* rl = recl[slot];  // the previous event's chunks are all in the ring: free to overwrite
+        for (int q = lane; q < REC_SITE; q += 64) rl[q] = rg[q];
+        r = rl;
+      }
       double a, b;
       grid_ends(LIN, net.fmin, fcut_eff(r[REC_EV + EV_FCUT], net.fmax), &a, &b);
       GridWalk<LIN> gw;
@@ -621,23 +630,42 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
     constexpr int NG = P * (P + 1) / 2;
     double G[NG];
     uint32_t mask = 0u;
-    pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
-                    [&](int e, int t, int res, const double* chunk) {
-                      if (t == 0) {
+    // KIND 0: no row factored, 1: the pure rows, 2: the pure rows and the
+    // Mc, eta, tcoal rows (pc_consume_semi).  Chosen once per wave, so that
+    // each consumer loop gets its own register allocation.
+    auto consume = [&](auto kind) {
+      constexpr int KIND = decltype(kind)::value;
+      pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
+                      [&](int e, int t, int res, const double* chunk) {
+                        if (t == 0) {
 #pragma unroll
-                        for (int q = 0; q < NG; ++q) G[q] = 0.0;
-                        if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
-                      }
-#ifndef GWF_NO_PURE_ROWS
-                      if constexpr (P == 11)
-                        pc_consume_pure<MODEL, NT, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane, chunk,
-                                                           G, mask);
-                      else
+                          for (int q = 0; q < NG; ++q) G[q] = 0.0;
+                          if (MASKED) mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
+                        }
+                        const double* r = rec + (int64_t)e * REC_DOUBLES;
+                        if constexpr (KIND == 2)
+                          pc_consume_semi<MODEL, NT, MASKED>(net, r, t * 64, res, lane, chunk, G, mask);
+                        else if constexpr (KIND == 1)
+                          pc_consume_pure<MODEL, NT, MASKED>(net, r, t * 64, res, lane, chunk, G, mask);
+                        else
+                          pc_consume<MODEL, NT, P, MASKED>(net, r, t * 64, res, lane, chunk, G, mask);
+                      },
+                      [&](int e) { store_packed<P>(G, F, n_ev, e, lane); });
+    };
+#ifdef GWF_NO_PURE_ROWS
+    consume(std::integral_constant<int, 0>{});
+#else
+    if constexpr (P == 11) {
+#ifndef GWF_NO_SEMI_ROWS
+      if (!net.mixed_motion)
+        consume(std::integral_constant<int, 2>{});
+      else
 #endif
-                        pc_consume<MODEL, NT, P, MASKED>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane,
-                                                         chunk, G, mask);
-                    },
-                    [&](int e) { store_packed<P>(G, F, n_ev, e, lane); });
+        consume(std::integral_constant<int, 1>{});
+    } else {
+      consume(std::integral_constant<int, 0>{});
+    }
+#endif
   }
 }
 
