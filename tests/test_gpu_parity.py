@@ -67,6 +67,36 @@ def test_fisher_parity(gpu, case):
     assert e_f <= 1e-6
 
 
+@pytest.mark.parametrize('motions', [('earth', 'fixed', 'none'), ('earth', 'none', 'earth'), ('fixed', 'none', 'fixed')],
+                         ids=['earth-fixed-none', 'earth-none', 'fixed-none'])
+def test_fisher_mixed_site_motion(gpu, motions):
+    """One network whose sites move differently (rotating Earth, Earth fixed
+    at tcoal, no motion).  The 11-parameter consumer factors the Mc, eta and
+    tcoal rows through the sites' common d t / dX, which a network with both
+    rotating- and fixed-Earth sites lacks: that one takes the pure-row path;
+    the others the factored one (a site without motion adds no rotation term).
+    Both against the oracle with the same per-site motion."""
+    from gwfast_amd import gwfastGlobals as glob
+    from gwfast_amd.network import DetNet
+    from gwfast_amd.signal import GWSignal
+    from oracle import detector as OD
+    wf, owf = _models('PD')
+    ev = bbh(6, seed=13)
+    sigs, odets = {}, []
+    for (name, psd), m in zip(NET_C1, motions):
+        d = glob.detectors[name]
+        kw = dict(useEarthMotion=m == 'earth', noMotion=m == 'none', fmin=2.)
+        sigs[name] = GWSignal(wf, psd_path=psd, detector_shape=d['shape'], det_lat=d['lat'], det_long=d['long'],
+                              det_xax=d['xax'], verbose=False, **kw)
+        odets.append(OD.Detector(psd, d['shape'], d['lat'], d['long'], d['xax'], **kw))
+    net = DetNet(sigs, verbose=False)
+    F = net.FisherMatr(dict(ev), res=900)
+    Fd = OD.network_fisher(odets, owf, ev, 900, mode='dual')
+    e_d = norm_err(F, Fd)
+    print(motions, 'Fisher err vs dual %.3e' % e_d, flush=True)
+    assert e_d <= 1e-10
+
+
 def test_fisher_per_arm_sums_to_network(gpu):
     wf, _ = _models('PD')
     ev = bbh(5, seed=7)
