@@ -102,7 +102,46 @@ GWF_HD double exp10d(double x) {
   return pow(10.0, x);
 #endif
 }
-GWF_HD double grid_f(double a, double b, int i, int div) { return exp10d(grid_lin(a, b, i, div)); }
+// 10^x for |x| <= 300 (no overflow, underflow or special operands), within
+// 2 ulp: x log2(10) = n + r with n an integer and |r| <= 1/2, the product in
+// double-double (FMA), 2^r = e^u, u = r ln 2, by the Taylor series of e^u to
+// the u^13 term (|u| <= 0.347: truncation below 4e-18), then 2^n exactly.
+// The library exp10 spends most of its instructions on the special operands.
+GWF_HD double exp10_kernel(double x) {
+  constexpr double L2T_HI = 3.321928094887362, L2T_LO = 1.661617516973592e-16;  // log2(10) = HI + LO
+  const double yh = x * L2T_HI;
+  const double yl = fma(x, L2T_HI, -yh) + x * L2T_LO;
+  const double n = rint(yh);
+  const double u = ((yh - n) + yl) * 0.6931471805599453;
+  double p = 1.0 / 6227020800.0;  // 1/13!
+  p = fma(p, u, 1.0 / 479001600.0);
+  p = fma(p, u, 1.0 / 39916800.0);
+  p = fma(p, u, 1.0 / 3628800.0);
+  p = fma(p, u, 1.0 / 362880.0);
+  p = fma(p, u, 1.0 / 40320.0);
+  p = fma(p, u, 1.0 / 5040.0);
+  p = fma(p, u, 1.0 / 720.0);
+  p = fma(p, u, 1.0 / 120.0);
+  p = fma(p, u, 1.0 / 24.0);
+  p = fma(p, u, 1.0 / 6.0);
+  p = fma(p, u, 0.5);
+  p = fma(p, u, 1.0);
+  p = fma(p, u, 1.0);
+  return ldexp(p, (int)n);
+}
+
+// 10^lin_i of grid point i: interior points by exp10_kernel, the end points
+// (i = 0, i >= div) by the library exp10, the value numpy's 10**x gives there
+// to the ulp in practice: a cut at the last point (IMRPhenomD's Mf = 0.2)
+// compares that frequency with the cut.
+GWF_HD double grid_pow10(double x, int i, int div) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return (i == 0 || i >= div) ? exp10d(x) : exp10_kernel(x);
+#else
+  return exp10d(x);
+#endif
+}
+GWF_HD double grid_f(double a, double b, int i, int div) { return grid_pow10(grid_lin(a, b, i, div), i, div); }
 
 // End points of an event's grid: (log10 fmin, log10 fcut) for geomspace,
 // (fmin, fcut) for linspace (jnp.linspace: f_i = fmin (1 - s) + fcut s, last
@@ -137,8 +176,9 @@ struct GridWalk {
     div = res - 1;
     lane = lane_;
     nlanes = nlanes_;
-    lin_cur = grid_lin(a, b, lane < res ? lane : 0, div);
-    f_cur = LIN ? lin_cur : exp10d(lin_cur);
+    const int i0 = lane < res ? lane : 0;
+    lin_cur = grid_lin(a, b, i0, div);
+    f_cur = LIN ? lin_cur : grid_pow10(lin_cur, i0, div);
     prev_last = 0.0;
   }
   // Call once per chunk with base = 0, nlanes, 2 nlanes, ... (all lanes);
@@ -146,8 +186,9 @@ struct GridWalk {
   GWF_HD void step(int base, int res, double* f, double* w, double* lnf) {
     const int i = base + lane;
     const int inext = i + nlanes;
-    lin_next = grid_lin(a, b, inext < res ? inext : 0, div);
-    f_next = LIN ? lin_next : exp10d(lin_next);
+    const int in = inext < res ? inext : 0;
+    lin_next = grid_lin(a, b, in, div);
+    f_next = LIN ? lin_next : grid_pow10(lin_next, in, div);
 #if defined(__HIP_DEVICE_COMPILE__)
     double fl = __shfl_up(f_cur, 1, 64);
     double fr = __shfl_down(f_cur, 1, 64);
@@ -458,6 +499,42 @@ struct Geo {
   double Sa_th, Ca_th, Cb_th, Sb_th;   // explicit d/d theta
 };
 
+// sin and cos of x for |x| < 2^30, within 2 ulp away from the zeros: x = k pi/2 + r
+// with pi/2 in three parts (Cody and Waite's reduction, with FMAs) and
+// |r| <= pi/4, the Taylor series of sin r to r^17 and of cos r to r^18
+// (truncation below 1e-19), then the quadrant k mod 4.
+GWF_HD void sincos_kernel(double x, double* s, double* c) {
+  constexpr double P1 = 1.5707963267948966, P2 = 6.123233995736766e-17, P3 = -1.4973849048591698e-33;
+  const double k = rint(x * 0.6366197723675814);
+  double r = fma(-k, P1, x);
+  r = fma(-k, P2, r);
+  r = fma(-k, P3, r);
+  const double r2 = r * r;
+  double ps = -1.0 / 355687428096000.0;  // -1/17!
+  ps = fma(ps, r2, 1.0 / 1307674368000.0);
+  ps = fma(ps, r2, -1.0 / 6227020800.0);
+  ps = fma(ps, r2, 1.0 / 39916800.0);
+  ps = fma(ps, r2, -1.0 / 362880.0);
+  ps = fma(ps, r2, 1.0 / 5040.0);
+  ps = fma(ps, r2, -1.0 / 120.0);
+  ps = fma(ps, r2, 1.0 / 6.0);
+  const double sr = fma(-r * r2, ps, r);  // r - r^3 (1/6 - r^2/120 + ...)
+  double pc = 1.0 / 6402373705728000.0;  // 1/18!
+  pc = fma(pc, r2, -1.0 / 20922789888000.0);
+  pc = fma(pc, r2, 1.0 / 87178291200.0);
+  pc = fma(pc, r2, -1.0 / 479001600.0);
+  pc = fma(pc, r2, 1.0 / 3628800.0);
+  pc = fma(pc, r2, -1.0 / 40320.0);
+  pc = fma(pc, r2, 1.0 / 720.0);
+  pc = fma(pc, r2, -1.0 / 24.0);
+  pc = fma(pc, r2, 0.5);
+  const double cr = fma(-r2, pc, 1.0);  // 1 - r^2 (1/2 - r^2/24 + ...)
+  const int q = (int)(long long)k & 3;
+  const double a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
+  *s = (q & 2) ? -a : a;
+  *c = ((q + 1) & 2) ? -b : b;
+}
+
 GWF_HD void sincos_d(double x, double* s, double* c) {
 #if defined(__HIP_DEVICE_COMPILE__)
   sincos(x, s, c);
@@ -517,7 +594,7 @@ GWF_HD void site_geo(const SiteK& S, const double* sc, const double* ev, double 
 // sincos of the site-independent Earth-rotation increment 2 pi tau / 86400.
 GWF_HD void earth_delta(bool any_earth, double tau, double* c, double* s) {
   if (any_earth) {
-    sincos_d((2. * PI * INV_DAY) * tau, s, c);
+    sincos_kernel((2. * PI * INV_DAY) * tau, s, c);
   } else {
     *c = 1.0;
     *s = 0.0;

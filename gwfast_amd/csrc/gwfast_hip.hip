@@ -774,7 +774,7 @@ __global__ void k_grid(NetK net, const double* __restrict__ rec, int64_t n_ev, i
   double a, b;
   grid_ends(net.lin != 0, net.fmin, fcut_eff(ev[EV_FCUT], net.fmax), &a, &b);
   const double lin = grid_lin(a, b, (int)i, res - 1);
-  const double f = net.lin ? lin : exp10d(lin);
+  const double f = net.lin ? lin : grid_pow10(lin, (int)i, res - 1);
   fo[idx] = f;
   if (So) So[idx] = psd_at(net.psd[0], f, net.lin ? log(f) : lin * 2.302585092994045684);
 }
@@ -788,7 +788,7 @@ __global__ void k_grid_fcut(NetK net, const double* __restrict__ fcut, int64_t n
   double a, b;
   grid_ends(net.lin != 0, net.fmin, fcut[e], &a, &b);
   const double lin = grid_lin(a, b, (int)i, res - 1);
-  const double f = net.lin ? lin : exp10d(lin);
+  const double f = net.lin ? lin : grid_pow10(lin, (int)i, res - 1);
   fo[idx] = f;
   if (So) So[idx] = psd_at(net.psd[0], f, net.lin ? log(f) : lin * 2.302585092994045684);
 }
