@@ -968,10 +968,20 @@ struct PcLayout {
                        TD = 7 + 2 * NT, CD = 9 + 2 * NT, SD = 10 + 2 * NT, NV = 11 + 2 * NT;
 };
 
+// Amplitudes at or below this (a Planck taper's last points, a cut) count as
+// zero in the pure-row factors c_p = A'/A - i Phi': 1/A overflows below
+// 5.6e-309, while such a point's rows, all of the order of A and of its
+// tangents (below ~1e-270 there, against ~1e-20 in band), add nothing to
+// the Fisher matrix at double precision.
+constexpr double AMP_TINY = 1e-280;
+
 // Producer: point base + lane of the grid into buf[NV][64] (nothing past res).
+// factored: rows AD hold A'/A (0 where A vanishes) and TD the factored
+// rows' t_noloc tangents -tau'/day (0 without a rotating Earth), as
+// pc_consume_semi reads them, instead of A' and tau'.
 template <int MODEL, int NT, bool LIN>
 GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec, GridWalk<LIN>& gw, int base, int res,
-                       int lane, double* buf) {
+                       int lane, double* buf, bool factored = false) {
   typedef PcLayout<NT> L;
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GWF_NO_LOAD_BARRIER)
   // keep the record loads inside the chunk (scalar loads) rather than hoisted
@@ -996,14 +1006,16 @@ GWF_HD void pc_produce(const gwf_waveform& w, const NetK& net, const double* rec
   buf[L::SW0 * 64 + lane] = sqrt((4.0 * wt) / s0);
   buf[L::SW1 * 64 + lane] = sqrt((4.0 * wt) / s1);
   buf[L::AV * 64 + lane] = wp.A.v;
+  const double ia = factored ? (fabs(wp.A.v) > AMP_TINY ? 1.0 / wp.A.v : 0.0) : 1.0;
+  const double td = factored ? (net.any_earth ? -INV_DAY : 0.0) : 1.0;
 #pragma unroll
   for (int k = 0; k < NT; ++k) {
-    buf[(L::AD + k) * 64 + lane] = wp.A.d[k];
+    buf[(L::AD + k) * 64 + lane] = wp.A.d[k] * ia;
     buf[(L::PD + k) * 64 + lane] = wp.Phi.d[k];
   }
   buf[L::TV * 64 + lane] = wp.tau.v;
-  buf[L::TD * 64 + lane] = wp.tau.d[0];
-  buf[(L::TD + 1) * 64 + lane] = wp.tau.d[1];
+  buf[L::TD * 64 + lane] = wp.tau.d[0] * td;
+  buf[(L::TD + 1) * 64 + lane] = wp.tau.d[1] * td;
   buf[L::CD * 64 + lane] = cdel;
   buf[L::SD * 64 + lane] = sdel;
 }
@@ -1100,12 +1112,6 @@ GWF_HD void pc_consume(const NetK& net, const double* rec, int base, int res, in
 // and Re(c_p conj(c_q)) W: per arm that leaves the 28-entry Gram of the 7
 // rows, T and W (86 FMAs instead of 132), and per point the 38 entries
 // involving a pure row.  Rows scaled by sqrt(weight) make T and W weighted.
-// Amplitudes at or below this (a Planck taper's last points, a cut) count as
-// zero in the pure-row factors c_p = A'/A - i Phi': 1/A overflows below
-// 5.6e-309, while such a point's rows, all of the order of A and of its
-// tangents (below ~1e-270 there, against ~1e-20 in band), add nothing to
-// the Fisher matrix at double precision.
-constexpr double AMP_TINY = 1e-280;
 
 template <int P>
 GWF_HD constexpr int pk(int a, int b) {  // packed upper-triangle index, a <= b
@@ -1309,20 +1315,19 @@ GWF_HD void pc_consume_semi(const NetK& net, const double* rec, int base, int re
     for (int k = 0; k < net.n_sum_arms; ++k) do_arm(net.sum_arm[k], net.sum_arm[k].site);
   }
   // the factored rows: index, c_j = cr + i ci, s_j (HR / HI / HS: the parts
-  // that are not identically zero)
+  // that are not identically zero); the producer wrote A'/A and s_j of Mc
+  // and eta (pc_produce, factored)
   constexpr int FR[7] = {0, 1, 7, 2, 8, 9, 10};
   constexpr bool HR[7] = {true, true, false, true, false, true, true};
   constexpr bool HI[7] = {true, true, true, false, true, true, true};
   constexpr bool HS[7] = {true, true, true, false, false, false, false};
-  const double ia = fabs(wp.A.v) > AMP_TINY ? 1.0 / wp.A.v : 0.0;
-  const bool earth = net.any_earth != 0;
   double cr[7], ci[7], sj[7];
-  cr[0] = ld(L::AD) * ia;
+  cr[0] = ld(L::AD);
   ci[0] = -ld(L::PD);
-  sj[0] = earth ? -ld(L::TD) * INV_DAY : 0.0;
-  cr[1] = ld(L::AD + 1) * ia;
+  sj[0] = ld(L::TD);
+  cr[1] = ld(L::AD + 1);
   ci[1] = -ld(L::PD + 1);
-  sj[1] = earth ? -ld(L::TD + 1) * INV_DAY : 0.0;
+  sj[1] = ld(L::TD + 1);
   cr[2] = 0.0;
   ci[2] = 2. * PI * f;
   sj[2] = INV_DAY;
@@ -1330,9 +1335,9 @@ GWF_HD void pc_consume_semi(const NetK& net, const double* rec, int base, int re
   ci[3] = 0.0;
   cr[4] = 0.0;
   ci[4] = -1.0;
-  cr[5] = ld(L::AD + 2) * ia;
+  cr[5] = ld(L::AD + 2);
   ci[5] = -ld(L::PD + 2);
-  cr[6] = ld(L::AD + 3) * ia;
+  cr[6] = ld(L::AD + 3);
   ci[6] = -ld(L::PD + 3);
 #pragma unroll
   for (int j = 3; j < 7; ++j) sj[j] = 0.0;
@@ -1350,9 +1355,26 @@ GWF_HD void pc_consume_semi(const NetK& net, const double* rec, int base, int re
       x = t;
     }
   }
-  double z[7];  // Re(c_j W_uv)
+  // F_jl = y_j^T M y_l, y = (cr, ci, s), M = [[Wuu, 0, Wr], [0, Wuu, -Wi],
+  // [Wr, -Wi, Wvv]]: m = M y_l once per row, then up to three FMAs per entry
+  double m1[7], m2[7], m3[7];
 #pragma unroll
-  for (int j = 0; j < 7; ++j) z[j] = HR[j] ? (HI[j] ? fma(cr[j], Wr, -ci[j] * Wi) : cr[j] * Wr) : -ci[j] * Wi;
+  for (int l = 0; l < 7; ++l) {
+    m1[l] = HR[l] ? (HS[l] ? fma(Wr, sj[l], Wuu * cr[l]) : Wuu * cr[l]) : (HS[l] ? Wr * sj[l] : 0.0);
+    m2[l] = HI[l] ? (HS[l] ? fma(-Wi, sj[l], Wuu * ci[l]) : Wuu * ci[l]) : (HS[l] ? -Wi * sj[l] : 0.0);
+    double t = 0.0;
+    bool any = false;
+    if (HR[l]) {
+      t = Wr * cr[l];
+      any = true;
+    }
+    if (HI[l]) {
+      t = any ? fma(-Wi, ci[l], t) : -Wi * ci[l];
+      any = true;
+    }
+    if (HS[l]) t = any ? fma(Wvv, sj[l], t) : Wvv * sj[l];
+    m3[l] = t;
+  }
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
 #pragma unroll
@@ -1360,14 +1382,9 @@ GWF_HD void pc_consume_semi(const NetK& net, const double* rec, int base, int re
       const int r = FR[j], c = FR[l];
       double& x = G[r < c ? pk<P>(r, c) : pk<P>(c, r)];
       double t = x;
-      const bool rr = HR[j] && HR[l], ii = HI[j] && HI[l];
-      if (rr || ii) {
-        const double cc = rr ? (ii ? fma(cr[j], cr[l], ci[j] * ci[l]) : cr[j] * cr[l]) : ci[j] * ci[l];
-        t = fma(cc, Wuu, t);
-      }
-      if (HS[j] && HS[l]) t = fma(sj[j] * sj[l], Wvv, t);
-      if (HS[l]) t = fma(sj[l], z[j], t);
-      if (HS[j]) t = fma(sj[j], z[l], t);
+      if (HR[j] && (HR[l] || HS[l])) t = fma(cr[j], m1[l], t);
+      if (HI[j] && (HI[l] || HS[l])) t = fma(ci[j], m2[l], t);
+      if (HS[j]) t = fma(sj[j], m3[l], t);
       x = t;
     }
   }

@@ -534,6 +534,12 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   const int slot = wave % PC_EPB;
   if (threadIdx.x < 2 * PC_EPB) cnt[threadIdx.x >> 1][threadIdx.x & 1] = 0;
   __syncthreads();
+  // the consumer is pc_consume_semi (it reads the producer's rows factored)
+#if defined(GWF_NO_PURE_ROWS) || defined(GWF_NO_SEMI_ROWS)
+  const bool semi = false;
+#else
+  const bool semi = P == 11 && !MX && !net.mixed_motion;
+#endif
   if (wave < PC_EPB) {
     int k = 0;  // items produced by this pair
     int64_t e = RAGGED ? 0 : (int64_t)blockIdx.x * PC_EPB + slot;
@@ -574,7 +580,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
           pq_fail(counter, lane);
           return;
         }
-        pc_produce<MODEL, NT, LIN>(w, net, r, gw, t * 64, res, lane, buf[slot][s]);
+        pc_produce<MODEL, NT, LIN>(w, net, r, gw, t * 64, res, lane, buf[slot][s], semi);
         if (lane == 0) {
           head[slot][s][0] = (int)e;
           head[slot][s][1] = t;
