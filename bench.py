@@ -31,10 +31,10 @@ SITES = [('ETS', 'ET'), ('CE1Id', 'CE'), ('CE2NM', 'CE')]
 GRAM_FLOP_PER_FISHER = 4 * 11 * 11 * RES * 5   # 4 P^2 N_f N_ifo = 4.84 MFLOP
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per k_fisher_pq launch at 10^4 events, from rocprofv3 --pmc FETCH_SIZE
-# (26,205.1 KiB, x2.0: a 256 MiB device copy reads 131,083.5 KiB by that counter)
-# and WRITE_SIZE (9,457.9 KiB, x1.0) runs of this script; summary in
+# (26,213.6 KiB, x2.0: a 256 MiB device copy reads 131,083.5 KiB by that counter)
+# and WRITE_SIZE (9,457.4 KiB, x1.0) runs of this script; summary in
 # profiles/k_fisher_imrphenomd_net_10k_2000f.md
-K_FISHER_TRAFFIC_10K = (26205.1 * 2.0 + 9457.9) * 1024
+K_FISHER_TRAFFIC_10K = (26213.6 * 2.0 + 9457.4) * 1024
 
 
 def build_net():
@@ -408,8 +408,11 @@ def c2_leg(wf, net, n=100_000, steps=5, warmup=2):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    # the GPU's clocks ramp over the first ~20 ms of sustained load (about a
+    # dozen steps here): 30 warm-up steps let the 50 timed ones see the
+    # steady state a catalogue run works in
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=30)
     ap.add_argument('--n-events', type=int, default=N_EV)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-per-core', type=int, default=512, help='CPU baseline events per core')
