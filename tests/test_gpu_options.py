@@ -147,16 +147,30 @@ def test_fishertools_variable_changes(gpu):
     assert np.allclose(FT.J_Mceta_m1m2(m1, m2), Jmm, rtol=1e-14, atol=1e-300)
 
 
-@pytest.mark.parametrize('model', ['PD', 'NRT'])
-def test_masked_fisher_kernel(gpu, model):
+def _net_with_motions(wf, motions):
+    from gwfast_amd import gwfastGlobals as glob
+    from gwfast_amd.network import DetNet
+    from gwfast_amd.signal import GWSignal
+    sigs = {}
+    for (name, psd), m in zip(NET_C1, motions):
+        d = glob.detectors[name]
+        sigs[name] = GWSignal(wf, psd_path=psd, detector_shape=d['shape'], det_lat=d['lat'], det_long=d['long'],
+                              det_xax=d['xax'], useEarthMotion=m == 'earth', noMotion=m == 'none', fmin=2.,
+                              verbose=False)
+    return DetNet(sigs, verbose=False)
+
+
+@pytest.mark.parametrize('model,motion', [('PD', 'earth'), ('NRT', 'earth'), ('PD', 'mixed')])
+def test_masked_fisher_kernel(gpu, model, motion):
     """gwf_fisher_masked: per event, the sum of the Fisher matrices of the
     arms whose bit is set, against per-arm launches; all bits set gives the
-    fused network Fisher, no bit gives zero."""
+    fused network Fisher, no bit gives zero.  'mixed': rotating-Earth,
+    fixed-Earth and motionless sites in one network (the pure-row consumer)."""
     from gwfast_amd import signal as S
     wf, _ = _models(model)
     n, res = 48, 500
     ev = (bbh if model == 'PD' else bns)(n, seed=51)
-    net = hip_net(wf, NET_C1, True)
+    net = hip_net(wf, NET_C1, True) if motion == 'earth' else _net_with_motions(wf, ('earth', 'fixed', 'none'))
     sigs = list(net.signals.values())
     sigs[0]._complete(ev)
     rows = sigs[0]._rows(ev)
@@ -173,7 +187,7 @@ def test_masked_fisher_kernel(gpu, model):
     assert np.all(Fm[:, :, bits.sum(axis=0) == 0] == 0)
     live = np.nonzero(bits.any(axis=0))[0]
     e = norm_err(Fm[:, :, live], want[:, :, live])
-    print(model, 'masked vs per-arm sum %.3e' % e, flush=True)
+    print(model, motion, 'masked vs per-arm sum %.3e' % e, flush=True)
     assert e <= 1e-13
     assert norm_err(Fm[:, :, 1:2], full[:, :, 1:2]) <= 1e-14
 
