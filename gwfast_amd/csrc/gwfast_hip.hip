@@ -446,6 +446,208 @@ __device__ __forceinline__ void store_pure_mx(const double* G, const v4d& acc, d
   }
 }
 
+// Consumer for 12-14 parameters with the Mc, eta and tcoal rows factored as
+// well (every row outside theta, phi, iota, psi is d_j = u c_j + v s_j, as
+// pc_consume_semi for 11).  On the VALU: per arm the 4 sky rows' Gram and
+// their sums Tu_k, Tv_k against u and v; per point the entries among the
+// NF = P - 4 factored rows, y_j^T M y_l with y = (Re c, Im c, s) and M the
+// point's 3 x 3 of the W sums, and s_j Tv_k of the three rows with s_j != 0.
+// On the matrix cores: the rest of the sky x factored entries,
+// sum_points (Re c_j Tr_k - Im c_j Ti_k), per chunk through the wave's LDS
+// tile ([Re c | Tr], then [-Im c | Ti]: c in columns 0..NF-1, T in 10..13)
+// with 32 v_mfma_f64_16x16x4f64, as pc_consume_pure_mx.
+template <int P>
+__device__ __forceinline__ constexpr int semi_row(int j) {  // factored row j: 0, 1, 7, 2, 8, 9, ..., P-1
+  return j < 3 ? (j == 2 ? 7 : j) : (j == 3 ? 2 : 4 + j);
+}
+__device__ __forceinline__ constexpr bool semi_hr(int j) { return j != 2 && j != 4; }  // Re c_j not identically 0
+__device__ __forceinline__ constexpr bool semi_hi(int j) { return j != 3; }            // Im c_j not identically 0
+__device__ __forceinline__ constexpr bool semi_hs(int j) { return j < 3; }             // s_j not identically 0
+template <int MODEL, int NT, int P>
+__device__ __forceinline__ void pc_consume_semi_vx(const NetK& net, const double* rec, int base, int res, int lane,
+                                                    const double* buf, double* X, double* G, v4d& acc) {
+  constexpr int NF = P - 4;
+  constexpr int NFF = NF * (NF + 1) / 2;  // G: [factored x factored | s x sky (12) | sky x sky (10)]
+  static_assert(NF >= 8 && NF <= 10, "12-14 parameters");
+  typedef PcLayout<NT> L;
+  __asm__ volatile("" ::: "memory");
+  const bool live = base + lane < res;
+  const double* ev = rec + REC_EV;
+  constexpr int SK[4] = {3, 4, 5, 6};
+  double cr[NF], ci[NF], Tr[4], Ti[4];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) cr[j] = ci[j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) Tr[a] = Ti[a] = 0.0;
+  if (live) {
+    auto ld = [&](int slot) { return buf[slot * 64 + lane]; };
+    const double f = ld(L::F);
+    WfPoint<MODEL, NT> wp;
+    wp.A.v = ld(L::AV);
+    wp.Phi.v = wp.tau.v = wp.tau.d[0] = wp.tau.d[1] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NT; ++k) wp.A.d[k] = wp.Phi.d[k] = 0.0;
+    const double tc = ev[EV_TC];
+    int cur_site = -1, cur_psd = -1;
+    double sw = 0.0;
+    Geo g;
+    double Tv[4], Wuu = 0.0, Wvv = 0.0, Wr = 0.0, Wi = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) Tv[a] = 0.0;
+    for (int k = 0; k < net.n_sum_arms; ++k) {
+      const ArmK& arm = net.sum_arm[k];
+      const int s = arm.site;
+      const SiteK& S = net.site[s];
+      const double* sc = rec + REC_SITE + s * SC_COUNT;
+      if (s != cur_site) {
+        if (S.psd != cur_psd) {
+          sw = S.psd == 0 ? ld(L::SW0)
+                          : (S.psd == 1 ? ld(L::SW1) : sqrt(ld(L::WT) / psd_at(net.psd[S.psd], f, ld(L::LNF))));
+          cur_psd = S.psd;
+        }
+        site_geo<true>(S, sc, ev, site_tn(S.motion, tc, ld(L::TV)), ld(L::CD), ld(L::SD), g);
+        cur_site = s;
+      }
+      double re[P], im[P], Ap, Ac, v[2];
+      arm_derivs<MODEL, NT, P>(wp, ev, sc, g, arm, S.motion, f, re, im, &Ap, &Ac, sw * arm.swgt, v);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = SK[a];
+#pragma unroll
+        for (int b = a; b < 4; ++b) {
+          double& x = G[NFF + 12 + pk<4>(a, b)];
+          x = fma(im[r], im[SK[b]], fma(re[r], re[SK[b]], x));
+        }
+        Tr[a] = fma(Ac, im[r], fma(Ap, re[r], Tr[a]));
+        Ti[a] = fma(-Ap, im[r], fma(Ac, re[r], Ti[a]));
+        Tv[a] = fma(v[1], im[r], fma(v[0], re[r], Tv[a]));
+      }
+      Wuu = fma(Ac, Ac, fma(Ap, Ap, Wuu));
+      Wvv = fma(v[1], v[1], fma(v[0], v[0], Wvv));
+      Wr = fma(Ac, v[1], fma(Ap, v[0], Wr));
+      Wi = fma(-Ap, v[1], fma(Ac, v[0], Wi));
+    }
+    // c_j, s_j; the producer wrote A'/A (tangent of row r >= 9 in slot
+    // r - 7) and the Mc, eta t_noloc tangents (pc_produce, factored)
+    double sj[3];
+    cr[0] = ld(L::AD);
+    ci[0] = -ld(L::PD);
+    sj[0] = ld(L::TD);
+    cr[1] = ld(L::AD + 1);
+    ci[1] = -ld(L::PD + 1);
+    sj[1] = ld(L::TD + 1);
+    ci[2] = 2. * PI * f;
+    sj[2] = INV_DAY;
+    cr[3] = -ev[EV_IDL];
+    ci[4] = -1.0;
+#pragma unroll
+    for (int j = 5; j < NF; ++j) {
+      cr[j] = ld(L::AD + j - 3);
+      ci[j] = -ld(L::PD + j - 3);
+    }
+    // s_j Tv_k of the rows with s_j != 0
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) G[NFF + 4 * j + a] = fma(sj[j], Tv[a], G[NFF + 4 * j + a]);
+    // F_jl = y_j^T M y_l, M = [[Wuu, 0, Wr], [0, Wuu, -Wi], [Wr, -Wi, Wvv]]
+    double m1[NF], m2[NF], m3[NF];
+#pragma unroll
+    for (int l = 0; l < NF; ++l) {
+      const double s_l = semi_hs(l) ? sj[l < 3 ? l : 0] : 0.0;
+      m1[l] = semi_hr(l) ? (semi_hs(l) ? fma(Wr, s_l, Wuu * cr[l]) : Wuu * cr[l]) : (semi_hs(l) ? Wr * s_l : 0.0);
+      m2[l] = semi_hi(l) ? (semi_hs(l) ? fma(-Wi, s_l, Wuu * ci[l]) : Wuu * ci[l]) : (semi_hs(l) ? -Wi * s_l : 0.0);
+      double t = 0.0;
+      bool any = false;
+      if (semi_hr(l)) {
+        t = Wr * cr[l];
+        any = true;
+      }
+      if (semi_hi(l)) {
+        t = any ? fma(-Wi, ci[l], t) : -Wi * ci[l];
+        any = true;
+      }
+      if (semi_hs(l)) t = any ? fma(Wvv, s_l, t) : Wvv * s_l;
+      m3[l] = t;
+    }
+    int q = 0;
+#pragma unroll
+    for (int j = 0; j < NF; ++j) {
+#pragma unroll
+      for (int l = j; l < NF; ++l, ++q) {
+        double t = G[q];
+        if (semi_hr(j) && (semi_hr(l) || semi_hs(l))) t = fma(cr[j], m1[l], t);
+        if (semi_hi(j) && (semi_hi(l) || semi_hs(l))) t = fma(ci[j], m2[l], t);
+        if (semi_hs(j)) t = fma(sj[j < 3 ? j : 0], m3[l], t);
+        G[q] = t;
+      }
+    }
+  }
+  // sum_points (Re c_j Tr_k - Im c_j Ti_k) on the matrix cores
+  const int rd = (lane >> 4) * PQ_XS;
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+#pragma unroll
+    for (int j = 0; j < NF; ++j) X[lane * PQ_XS + j] = part ? -ci[j] : cr[j];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) X[lane * PQ_XS + 10 + a] = part ? Ti[a] : Tr[a];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const double a = X[4 * t * PQ_XS + rd + (lane & 15)];
+      const double b = X[4 * t * PQ_XS + rd + 10 + (lane & 3)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Stores the Fisher of pc_consume_semi_vx.
+template <int P>
+__device__ __forceinline__ void store_semi_vx(const double* G, const v4d& acc, double* __restrict__ F, int64_t n_ev,
+                                              int64_t e, int lane) {
+  constexpr int NF = P - 4, NFF = NF * (NF + 1) / 2;
+  constexpr int SK[4] = {3, 4, 5, 6};
+  auto put = [&](int a, int b, double v) {
+    F[(int64_t)(a * P + b) * n_ev + e] = v;
+    if (a != b) F[(int64_t)(b * P + a) * n_ev + e] = v;
+  };
+  int q = 0;
+#pragma unroll
+  for (int j = 0; j < NF; ++j)
+#pragma unroll
+    for (int l = j; l < NF; ++l, ++q) {
+      const double s = wave_sum(G[q]);
+      if (lane == (q & 63)) put(semi_row<P>(j), semi_row<P>(l), s);
+    }
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b) {
+      const double s = wave_sum(G[NFF + 12 + pk<4>(a, b)]);
+      if (lane == ((NFF + pk<4>(a, b)) & 63)) put(SK[a], SK[b], s);
+    }
+  // sky x factored: the matrix-core part (element (j, k) in lane k + 16 (j % 4),
+  // item j / 4) plus, for the rows with s_j != 0, the wave sum of s_j Tv_k
+  double st[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) st[i] = wave_sum(G[NFF + i]);
+  const int k = lane & 15;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int j = (lane >> 4) + 4 * r;
+    if (j < NF && k < 4) {
+      double v = acc[r];
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          if (j == jj && k == kk) v += st[4 * jj + kk];
+      put(semi_row<P>(j), SK[k], v);
+    }
+  }
+}
+
 // Stores the event's Fisher from the NACC matrix-core accumulators (element
 // (i, j) in lane j + 16 (i % 4), item i / 4): the upper-triangle element
 // (i <= j) goes to both F[i][j] and F[j][i].
@@ -519,6 +721,13 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   // 180 VGPRs at 12 and 14 parameters; 2.31 -> 2.36 and 2.44 -> 5.07 ms per
   // 10^4 events)
   constexpr bool REC_LDS = !(MODEL == GWF_MODEL_TAYLORF2 && (P == 12 || P == 14));
+  // 12-13 parameters, network sum: pc_consume_semi_vx (pc_consume_pure_mx for
+  // a network that mixes rotating- and fixed-Earth sites, and for 14)
+#ifdef GWF_NO_SEMI_ROWS
+  constexpr bool SEMI_VX = false;
+#else
+  constexpr bool SEMI_VX = MX && MODE == PQ_SUM && PURE_MX && P <= 13;
+#endif
   // ring depth: PQ_NBUF buffers, fewer where they would not fit the CU's LDS
   // beside the MFMA tiles
   constexpr int NB = pq_ring_depth(L::NV, MX, REC_LDS);
@@ -538,7 +747,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
 #if defined(GWF_NO_PURE_ROWS) || defined(GWF_NO_SEMI_ROWS)
   const bool semi = false;
 #else
-  const bool semi = P == 11 && !MX && !net.mixed_motion;
+  const bool semi = ((P == 11 && !MX) || SEMI_VX) && !net.mixed_motion;
 #endif
   if (wave < PC_EPB) {
     int k = 0;  // items produced by this pair
@@ -590,9 +799,30 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
       }
       if (!RAGGED) e += stride;
     }
-  } else if constexpr (MX && MODE == PQ_SUM && PURE_MX) {  // pure rows factored out, see pc_consume_pure_mx
+  } else if constexpr (MX && MODE == PQ_SUM && PURE_MX) {  // rows factored out, see pc_consume_semi_vx
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;
+#ifndef GWF_NO_SEMI_ROWS
+    // the Mc, eta, tcoal rows factored as well, up to 13 parameters (at 14
+    // the consumer spills 88 VGPRs: 2.44 -> 3.07 ms per 10^4 events)
+    if (SEMI_VX && !net.mixed_motion) {
+      constexpr int NF = P - 4, NG = NF * (NF + 1) / 2 + 12 + 10;
+      double G[NG];
+      v4d acc;
+      pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
+                      [&](int e, int t, int res, const double* chunk) {
+                        if (t == 0) {
+#pragma unroll
+                          for (int q = 0; q < NG; ++q) G[q] = 0.0;
+                          acc = v4d{0.0, 0.0, 0.0, 0.0};
+                        }
+                        pc_consume_semi_vx<MODEL, NT, P>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res, lane,
+                                                         chunk, X, G, acc);
+                      },
+                      [&](int e) { store_semi_vx<P>(G, acc, F, n_ev, e, lane); });
+      return;
+    }
+#endif
     constexpr int NPU = P - 7, NG = 28 + NPU * (NPU + 1) / 2;
     double G[NG];
     v4d acc;

@@ -67,21 +67,23 @@ def test_fisher_parity(gpu, case):
     assert e_f <= 1e-6
 
 
+@pytest.mark.parametrize('model', ['PD', 'NRT'])
 @pytest.mark.parametrize('motions', [('earth', 'fixed', 'none'), ('earth', 'none', 'earth'), ('fixed', 'none', 'fixed')],
                          ids=['earth-fixed-none', 'earth-none', 'fixed-none'])
-def test_fisher_mixed_site_motion(gpu, motions):
+def test_fisher_mixed_site_motion(gpu, motions, model):
     """One network whose sites move differently (rotating Earth, Earth fixed
     at tcoal, no motion).  The 11-parameter consumer factors the Mc, eta and
     tcoal rows through the sites' common d t / dX, which a network with both
     rotating- and fixed-Earth sites lacks: that one takes the pure-row path;
     the others the factored one (a site without motion adds no rotation term).
-    Both against the oracle with the same per-site motion."""
+    Both against the oracle with the same per-site motion; 11 parameters
+    (IMRPhenomD) and 13 (IMRPhenomD_NRTidalv2, the matrix-core consumers)."""
     from gwfast_amd import gwfastGlobals as glob
     from gwfast_amd.network import DetNet
     from gwfast_amd.signal import GWSignal
     from oracle import detector as OD
-    wf, owf = _models('PD')
-    ev = bbh(6, seed=13)
+    wf, owf = _models(model)
+    ev = (bbh if model == 'PD' else bns)(6, seed=13)
     sigs, odets = {}, []
     for (name, psd), m in zip(NET_C1, motions):
         d = glob.detectors[name]
@@ -93,7 +95,7 @@ def test_fisher_mixed_site_motion(gpu, motions):
     F = net.FisherMatr(dict(ev), res=900)
     Fd = OD.network_fisher(odets, owf, ev, 900, mode='dual')
     e_d = norm_err(F, Fd)
-    print(motions, 'Fisher err vs dual %.3e' % e_d, flush=True)
+    print(model, motions, 'Fisher err vs dual %.3e' % e_d, flush=True)
     assert e_d <= 1e-10
 
 
