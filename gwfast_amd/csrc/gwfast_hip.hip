@@ -463,9 +463,10 @@ __device__ __forceinline__ constexpr int semi_row(int j) {  // factored row j: 0
 __device__ __forceinline__ constexpr bool semi_hr(int j) { return j != 2 && j != 4; }  // Re c_j not identically 0
 __device__ __forceinline__ constexpr bool semi_hi(int j) { return j != 3; }            // Im c_j not identically 0
 __device__ __forceinline__ constexpr bool semi_hs(int j) { return j < 3; }             // s_j not identically 0
-template <int MODEL, int NT, int P>
+template <int MODEL, int NT, int P, bool MASKED = false>
 __device__ __forceinline__ void pc_consume_semi_vx(const NetK& net, const double* rec, int base, int res, int lane,
-                                                    const double* buf, double* X, double* G, v4d& acc) {
+                                                    const double* buf, double* X, double* G, v4d& acc,
+                                                    uint32_t mask = 0u) {
   constexpr int NF = P - 4;
   constexpr int NFF = NF * (NF + 1) / 2;  // G: [factored x factored | s x sky (12) | sky x sky (10)]
   static_assert(NF >= 8 && NF <= 10, "12-14 parameters");
@@ -494,9 +495,7 @@ __device__ __forceinline__ void pc_consume_semi_vx(const NetK& net, const double
     double Tv[4], Wuu = 0.0, Wvv = 0.0, Wr = 0.0, Wi = 0.0;
 #pragma unroll
     for (int a = 0; a < 4; ++a) Tv[a] = 0.0;
-    for (int k = 0; k < net.n_sum_arms; ++k) {
-      const ArmK& arm = net.sum_arm[k];
-      const int s = arm.site;
+    auto do_arm = [&](const ArmK& arm, int s) {
       const SiteK& S = net.site[s];
       const double* sc = rec + REC_SITE + s * SC_COUNT;
       if (s != cur_site) {
@@ -526,6 +525,23 @@ __device__ __forceinline__ void pc_consume_semi_vx(const NetK& net, const double
       Wvv = fma(v[1], v[1], fma(v[0], v[0], Wvv));
       Wr = fma(Ac, v[1], fma(Ap, v[0], Wr));
       Wi = fma(-Ap, v[1], fma(Ac, v[0], Wi));
+    };
+    if constexpr (MASKED) {  // the arms whose bit is set; a site with all set through its sum arms
+      for (int s = 0; s < net.n_sites; ++s) {
+        const SiteK& S = net.site[s];
+        const uint32_t all = (1u << S.narm) - 1u;
+        const uint32_t bits = (mask >> S.arm0) & all;
+        if (!bits) continue;
+        const bool full = bits == all;
+        const int cnt = full ? S.nsum : S.narm;
+        for (int j = 0; j < cnt; ++j) {
+          if (!full && !((bits >> j) & 1u)) continue;
+          do_arm(full ? net.sum_arm[S.sum0 + j] : net.arm[S.arm0 + j], s);
+        }
+      }
+    } else {
+      (void)mask;
+      for (int k = 0; k < net.n_sum_arms; ++k) do_arm(net.sum_arm[k], net.sum_arm[k].site);
     }
     // c_j, s_j; the producer wrote A'/A (tangent of row r >= 9 in slot
     // r - 7) and the Mc, eta t_noloc tangents (pc_produce, factored)
@@ -728,6 +744,12 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
 #else
   constexpr bool SEMI_VX = MX && MODE == PQ_SUM && PURE_MX && P <= 13;
 #endif
+  // and with the per-event arm masks (duty cycle)
+  constexpr bool SEMI_VX_M = MX && MASKED && P <= 13 && !ARMS
+#ifdef GWF_NO_SEMI_ROWS
+                             && false
+#endif
+      ;
   // ring depth: PQ_NBUF buffers, fewer where they would not fit the CU's LDS
   // beside the MFMA tiles
   constexpr int NB = pq_ring_depth(L::NV, MX, REC_LDS);
@@ -747,7 +769,7 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
 #if defined(GWF_NO_PURE_ROWS) || defined(GWF_NO_SEMI_ROWS)
   const bool semi = false;
 #else
-  const bool semi = ((P == 11 && !MX) || SEMI_VX) && !net.mixed_motion;
+  const bool semi = ((P == 11 && !MX) || SEMI_VX || SEMI_VX_M) && !net.mixed_motion;
 #endif
   if (wave < PC_EPB) {
     int k = 0;  // items produced by this pair
@@ -840,6 +862,27 @@ __global__ __launch_bounds__(128 * PC_EPB, 4 / PC_EPB) void k_fisher_pq(gwf_wave
   } else if constexpr (MX) {  // Gram on the matrix cores
     double* X = xt[slot];
     for (int q = lane; q < 64 * PQ_XS; q += 64) X[q] = 0.0;  // padding columns stay zero
+    if constexpr (SEMI_VX_M) {  // masked network sum, rows factored as in the unmasked one
+      if (!net.mixed_motion) {
+        constexpr int NF = P - 4, NG = NF * (NF + 1) / 2 + 12 + 10;
+        double G[NG];
+        v4d acc1;
+        uint32_t mask = 0u;
+        pq_consumer<NB>(&cnt[slot][0], &cnt[slot][1], head[slot], buf[slot], counter, net.spin_limit, lane,
+                        [&](int e, int t, int res, const double* chunk) {
+                          if (t == 0) {
+#pragma unroll
+                            for (int q = 0; q < NG; ++q) G[q] = 0.0;
+                            acc1 = v4d{0.0, 0.0, 0.0, 0.0};
+                            mask = (uint32_t)__builtin_amdgcn_readfirstlane((int)arm_mask[e]);
+                          }
+                          pc_consume_semi_vx<MODEL, NT, P, true>(net, rec + (int64_t)e * REC_DOUBLES, t * 64, res,
+                                                                 lane, chunk, X, G, acc1, mask);
+                        },
+                        [&](int e) { store_semi_vx<P>(G, acc1, F, n_ev, e, lane); });
+        return;
+      }
+    }
     constexpr int NA = ARMS ? PQ_ARMS_MAX : PQ_NACC;        // ARMS: one accumulator per physical arm
     v4d acc[NA];
     uint32_t mask = 0u;

@@ -160,12 +160,13 @@ def _net_with_motions(wf, motions):
     return DetNet(sigs, verbose=False)
 
 
-@pytest.mark.parametrize('model,motion', [('PD', 'earth'), ('NRT', 'earth'), ('PD', 'mixed')])
+@pytest.mark.parametrize('model,motion', [('PD', 'earth'), ('NRT', 'earth'), ('PD', 'mixed'), ('NRT', 'mixed')])
 def test_masked_fisher_kernel(gpu, model, motion):
     """gwf_fisher_masked: per event, the sum of the Fisher matrices of the
     arms whose bit is set, against per-arm launches; all bits set gives the
     fused network Fisher, no bit gives zero.  'mixed': rotating-Earth,
-    fixed-Earth and motionless sites in one network (the pure-row consumer)."""
+    fixed-Earth and motionless sites in one network (the pure-row consumer
+    for 11 parameters, the matrix-core Gram for 13)."""
     from gwfast_amd import signal as S
     wf, _ = _models(model)
     n, res = 48, 500
